@@ -1,0 +1,206 @@
+/* lpm_gpu.h -- C ABI of liblpmg.so, the MI355X (gfx950) implementation of
+ * LPM-CPP's quasi-static Newton step: bond geometry and forces, the analytic
+ * tangent stiffness and its assembly, and the linear solve.
+ *
+ * Every entry point replaces one reference interface; the citation beside each
+ * declaration names it (paths relative to the LPM-CPP repository root).  All
+ * buffers in these signatures are HOST buffers; the particle system itself
+ * (positions, bonds, state, stiffness) lives in device memory for the lifetime
+ * of the handle.  Particle ids are local to a handle (0..n-1), so any number of
+ * systems can coexist in one process (the reference's process-global
+ * Particle::_ID, particle.h:17, allows one).
+ *
+ * Return values: 0 (or a non-negative count / flag) on success, negative on
+ * error: -1 invalid argument or state, -2 HIP error, -3 out of device memory,
+ * -4 not converged.  lpmg_last_error() describes the last error of the calling
+ * thread.  No entry point calls exit().
+ *
+ * Threading: one host thread at a time per handle; handles are independent and
+ * each owns its own HIP stream.
+ */
+#ifndef LPM_GPU_H
+#define LPM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lpmg_context* lpmg_handle;
+
+/* LatticeType (include/lpm.h:99-108); the BCC slip variants share one cell. */
+enum { LPMG_SQUARE2D = 0, LPMG_HEXAGON2D = 1, LPMG_SC3D = 2, LPMG_FCC3D = 3, LPMG_BCC3D = 4 };
+/* ParticleType (include/lpm.h:91-97) */
+enum { LPMG_ELASTIC = 0, LPMG_ELASTIC_DAMAGE = 1, LPMG_J2 = 2 };
+/* LoadMode (include/lpm.h:67-71) */
+enum { LPMG_RELATIVE = 0, LPMG_ABSOLUTE = 1 };
+/* material models of lpmg_set_material */
+enum {
+  LPMG_MAT_ELASTIC = 0,        /* nonlocal_L, plane_stress, E, mu, critical_bstrain */
+  LPMG_MAT_ELASTIC_CUBIC = 1,  /* nonlocal_L, C11, C12, C44, critical_bstrain */
+  LPMG_MAT_DAMAGE = 2,         /* nonlocal_L, plane_stress, E, mu, k0, k1, ct_ratio, d_thres */
+  LPMG_MAT_DAMAGE_CUBIC = 3,   /* nonlocal_L, C11, C12, C44, k0, k1, ct_ratio, d_thres */
+  LPMG_MAT_J2 = 4              /* nonlocal_L, plane_stress, E, mu, sigmay, xi, H, A, crit */
+};
+/* fields of lpmg_get_field / lpmg_set_field (lengths in doubles) */
+enum {
+  LPMG_F_XYZ = 0,          /* n*3   Particle::xyz          (set: yes) */
+  LPMG_F_XYZ_INITIAL = 1,  /* n*3   Particle::xyz_initial  */
+  LPMG_F_XYZ_LAST = 2,     /* n*3   Particle::xyz_last     (set: yes) */
+  LPMG_F_PIN = 3,          /* n*3   Particle::Pin          */
+  LPMG_F_PEX = 4,          /* n*3   Particle::Pex          (set: yes) */
+  LPMG_F_STRESS = 5,       /* n*6   Particle::stress       */
+  LPMG_F_DAMAGE = 6,       /* n     Particle::damage       (set: yes) */
+  LPMG_F_DAMAGE_VISUAL = 7,
+  LPMG_F_STATE_VAR = 8,    /* n*7   Particle::state_var    (set: yes) */
+  LPMG_F_RESIDUAL = 9,     /* n*dim Stiffness::residual    */
+  LPMG_F_BDAMAGE = 10,     /* nbond Bond::bdamage          (set: yes) */
+  LPMG_F_DLP = 11,         /* nbond Bond::dLp              (set: yes) */
+  LPMG_F_BFORCE = 12,
+  LPMG_F_DIS = 13,
+  LPMG_F_CS = 14,          /* nbond*3 Bond::csx/csy/csz    */
+  LPMG_F_CONSTRAINT = 15,  /* n*3   Particle::disp_constraint (set: yes) */
+  LPMG_F_DDOT_LOCAL = 16,
+  LPMG_F_DDOT_NONLOCAL = 17,
+  LPMG_F_DLE_TOTAL = 18,   /* n*2   Particle::dLe_total    */
+  LPMG_F_CS_SUM = 19,      /* n*6   Particle::cs_sum{x,y,z}[layer] */
+  LPMG_F_BSTRAIN = 20,
+  LPMG_F_DLE = 21,
+  LPMG_F_DAMAGE_LAST = 22,
+  LPMG_F_TYPE = 23,        /* n     Particle::type         */
+  LPMG_F_TDLE_TOTAL = 24   /* n*2   Particle::TdLe_total   */
+};
+
+/* Solver options, opts[8]:
+ *   [0] preconditioner: 0 none (the reference's MKL CG, solver.h:311), 1 Jacobi
+ *   [1] rtol, [2] atol: stop when ||r||^2 <= rtol*||r0||^2 + atol
+ *       (the reference's dpar[0], dpar[1], solver.h:312-313)
+ *   [3] maximum CG iterations (<= 0: the problem size, as ipar[4], solver.h:308)
+ *   [4] precision: 0 FP64 stiffness and CG; 1 FP32 stiffness and CG inside the
+ *       FP64 Newton (residual, refinement update and state stay FP64)
+ *   [5] CG iterations launched between convergence checks (<= 0: 16)
+ * Solver statistics, stats[4]: [0] CG iterations, [1] final ||r||^2 (CG) or
+ * ||r|| (Newton), [2] Newton tolerance multiplier, [3] Newton iterations. */
+
+int lpmg_version(void);
+const char* lpmg_last_error(void);
+
+/* UnitCell (include/unit_cell.h:29-122): out[16] = dim, nneighbors, volume,
+ * cutoff[0], cutoff[1], el_mapping[9] */
+int lpmg_unit_cell(int lattice, double radius, double* out);
+
+/* Assembly(p_xyz, box, cell, ptype) (include/assembly.h:97-106): createParticles,
+ * createBonds (assembly.h:212-234), updateConnections (assembly.h:236-266), plus
+ * Stiffness::initialize (stiffness.h:54-71).  xyz: n*3. */
+int lpmg_create(int lattice, double radius, int ptype, int n, const double* xyz, lpmg_handle* out);
+
+/* Assembly(dumpFile, bondFile, cell, ptype) (include/assembly.h:83-95) with the
+ * bond list of Assembly::readBond (assembly.h:390-424): nbond directed bonds
+ * (p1, p2, layer) in file order. */
+int lpmg_create_with_bonds(int lattice, double radius, int ptype, int n, const double* xyz,
+                           int64_t nbond, const int32_t* p1, const int32_t* p2,
+                           const int32_t* layer, lpmg_handle* out);
+
+/* Releases all device memory of the handle (the reference never frees its particles). */
+void lpmg_destroy(lpmg_handle h);
+
+/* out[8] = n, dim, nbond, nnz of the reference upper CSR, BSR blocks, max bonds
+ * per particle, max conns per particle, particle type */
+int lpmg_sizes(lpmg_handle h, int64_t* out);
+
+/* Bond lists: bptr[n+1], nb0[n] (layer-0 count), p2[nbond]; per particle the
+ * layer-0 bonds precede the layer-1 bonds (Particle::bond_layers, particle.h:33). */
+int lpmg_export_bonds(lpmg_handle h, int32_t* bptr, int32_t* nb0, int32_t* p2);
+
+/* Particle::conns / nconn_largeq (particle.h:22,35): cptr[n+1], conns[nnzb], ncl[n] */
+int lpmg_export_conns(lpmg_handle h, int32_t* cptr, int32_t* conns, int32_t* ncl);
+
+/* Stiffness::K_pointer, IK, JK, K_global (stiffness.h:24-26): the reference's
+ * upper-triangular 1-based CSR of the last assembled stiffness.  Any pointer may be NULL. */
+int lpmg_export_csr(lpmg_handle h, int64_t* K_pointer, int64_t* IK, int64_t* JK, double* K);
+
+/* Particle::type for every particle (used by undamaged_pt_type, solver_static.h:23) */
+int lpmg_set_types(lpmg_handle h, const int32_t* type);
+
+/* Particle*::setParticleProperty (particle_elastic.h:70-140,
+ * particle_elastic_damage.h:175-250, particle_j2plasticity.h:154-198) for the
+ * n_idx particles in idx, or for all particles when idx is NULL. */
+int lpmg_set_material(lpmg_handle h, int n_idx, const int32_t* idx, int model, const double* params);
+
+/* Assembly::updateGeometry + Assembly::updateForceState (assembly.h:132-138, 178-191) */
+int lpmg_forward(lpmg_handle h);
+
+/* Solver::updateRR (solver.h:111-126) and the norms of NewtonIteration
+ * (solver.h:63-64): norms[0] = ||residual||, norms[1] = ||all reactions so far||. */
+int lpmg_update_rr(lpmg_handle h, double* norms);
+
+/* Stiffness::reset + calcStiffness2D/3D + updateStiffnessDispBC
+ * (stiffness.h:73-89, 197-438), from the geometry of the last lpmg_forward.
+ * precision 1 also keeps an FP32 copy for the mixed-precision solve. */
+int lpmg_assemble(lpmg_handle h, int precision);
+
+/* y = K x with the assembled stiffness (the reference's mkl_sparse_d_mv on the
+ * symmetric upper CSR, solver.h:334).  x, y: n*dim host buffers. */
+int lpmg_spmv(lpmg_handle h, int precision, const double* x, double* y);
+
+/* Solver::solveLinearSystem (solver.h:92-109): solve K du = residual by
+ * (P)CG (replaces LPM_CG solver.h:276-354 and LPM_PARDISO solver.h:190-274),
+ * then move every particle by du.  Returns -4 if maxit was reached. */
+int lpmg_solve_linear(lpmg_handle h, const double* opts, double* stats);
+
+/* Solver::NewtonIteration (solver.h:54-90); returns the iteration count
+ * (max_iter on the abnormal return, like the reference). */
+int lpmg_newton(lpmg_handle h, int max_iter, double tol, const double* opts, double* stats);
+
+/* Assembly::updateStateVar (assembly.h:161-166) */
+int lpmg_update_state_var(lpmg_handle h);
+/* SolverStatic::updateStaticDamage (solver_static.h:31-62); returns 1 if any new damage */
+int lpmg_update_static_damage(lpmg_handle h, int undamaged_type);
+/* Assembly::storeStateVar / resetStateVar (assembly.h:140-159) */
+int lpmg_store_state(lpmg_handle h);
+int lpmg_reset_state(lpmg_handle h, int reset_xyz);
+/* Assembly::searchNonlocalNeighbors (assembly.h:117-130) */
+int lpmg_search_nonlocal(lpmg_handle h, double cutoff_ratio);
+
+/* A particle group for boundary conditions (DispBC/ForceBC::group, boundary.h:12,31);
+ * returns the group id. */
+int lpmg_define_group(lpmg_handle h, int n, const int32_t* idx);
+/* Solver::updateDisplacementBC for one DispBC (solver.h:128-164); flag 'x','y','z' */
+int lpmg_disp_bc(lpmg_handle h, int group, int load_mode, int flag, double step);
+/* Solver::updateForceBC for one ForceBC (solver.h:166-188) */
+int lpmg_force_bc(lpmg_handle h, int group, int load_mode, double fx, double fy, double fz);
+
+/* SolverStatic::solveProblemStep (solver_static.h:97-152) after the step's BCs
+ * were applied with lpmg_force_bc / lpmg_disp_bc.  Returns 1 converged, 0 not
+ * converged (the caller halves the step, as SolverStatic::solveProblem,
+ * solver_static.h:64-95).  stats: [0] CG iterations, [1] damage passes,
+ * [2] Newton iterations of the last pass. */
+int lpmg_solve_step(lpmg_handle h, int undamaged_type, int max_iter, double tol,
+                    const double* opts, double* stats);
+
+int64_t lpmg_field_len(lpmg_handle h, int field);
+int lpmg_get_field(lpmg_handle h, int field, double* out);
+int lpmg_set_field(lpmg_handle h, int field, const double* in);
+
+/* Bench instrumentation (no reference counterpart): while enabled, every CG
+ * SpMV launch is bracketed by HIP events on the handle's stream.
+ * lpmg_profile(h, enable) resets the counters; lpmg_profile_read returns
+ * out[0] = total SpMV ms, out[1] = number of SpMV launches, out[2] = algorithmic
+ * bytes of one FP64 SpMV, out[3] = of one FP32 SpMV. */
+int lpmg_profile(lpmg_handle h, int enable);
+int lpmg_profile_read(lpmg_handle h, double* out);
+
+/* Time `reps` back-to-back launches of one kernel on the handle's stream with
+ * HIP events (no reference counterpart): which 0 FP64 SpMV, 1 FP32 SpMV,
+ * 2 assembly (FP64), 3 forward pass.  out[0] = average ms per launch,
+ * out[1] = algorithmic bytes per launch. */
+int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out);
+
+int lpmg_synchronize(lpmg_handle h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LPM_GPU_H */

@@ -1,0 +1,305 @@
+// assemble.hip -- analytic tangent stiffness and its block-CSR assembly.
+//
+// Reference: Stiffness::calcStiffness3D/2D (stiffness.h:197-361),
+// Stiffness::localStiffnessANA (stiffness.h:108-142), Particle::hasAFEMneighbor
+// (particle.h:242-256), fdu2dxyz / fdu2dxyz1 / fdu2dxyz2 (derivative.h:7-239),
+// Stiffness::updateStiffnessDispBC (stiffness.h:363-438).
+//
+// Kernel 1 (tangent): one wavefront per particle row i, one lane per connection
+// j.  Every lane walks the same bond paths i->k->m of the row (wave-uniform
+// loads) and keeps the terms whose end is its own j, in the reference's loop
+// order, so each lane's K_local(i,j) is the reference's value bit for bit.
+// Kernel 2 (symmetrise + Dirichlet): owner-computes symmetrisation with no
+// atomics: the lane of block (i,j), i<j, reads K_local(i,j) and K_local(j,i)
+// through the precomputed transpose position, writes
+// 0.5*K_local(i,j) + 0.5*K_local(j,i)^T to (i,j) and its transpose to (j,i) --
+// the two-term sum the reference's scatter forms (stiffness.h:241-249, 267-275)
+// -- and applies the row/column elimination of constrained or fully damaged
+// DOFs with the diagonal set to ||diag(K)||_2.
+#include "lpmg_internal.h"
+#include "reduce.h"
+
+namespace lpmg {
+
+namespace {
+
+constexpr int kWaves = 4;
+
+template <int D>
+__global__ void __launch_bounds__(64 * kWaves)
+tangent_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
+               const int* __restrict__ p2, const double* __restrict__ cs,
+               const double* __restrict__ bdamage, const double* __restrict__ cs_sum,
+               const double* __restrict__ Kn, const double* __restrict__ Tv,
+               const int* __restrict__ cptr, const int* __restrict__ conns, int N,
+               double* __restrict__ H, double* __restrict__ diag_sq) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kWaves + w;
+  if (i >= N) return;
+  const int c_lo = cptr[i], c_hi = cptr[i + 1];
+  const int b_lo = bptr[i], b_mid = b_lo + nb0[i], b_hi = bptr[i + 1];
+  const double tv_i = Tv[i];
+  for (int e0 = c_lo; e0 < c_hi; e0 += 64) {
+    const int e = e0 + lane;
+    const bool active = e < c_hi;
+    const int j = active ? conns[e] : -1;
+    double K[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+    if (j == i) {
+      // fdu2dxyz (derivative.h:7-45): upper triangle of the diagonal block
+      for (int L = 0; L < 2; ++L) {
+        const int lo = L == 0 ? b_lo : b_mid, hi = L == 0 ? b_mid : b_hi;
+        const double kn = Kn[2 * i + L];
+        const double* csum = cs_sum + (2 * i + L) * 3;
+        for (int b = lo; b < hi; ++b) {
+          const double om = 1 - bdamage[b];
+          const double c[3] = {cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = r; s < D; ++s) {
+              const double geo = (r == s) ? (kn + tv_i) * (c[r] * c[r]) : (kn + tv_i) * c[s] * c[r];
+              K[r][s] += om * ((kn * c[s] + tv_i * csum[s]) * c[r] + geo);
+            }
+        }
+      }
+      double dsq = 0;
+#pragma unroll
+      for (int r = 0; r < D; ++r) dsq += K[r][r] * K[r][r];
+      diag_sq[i] = dsq;
+    } else if (active) {
+      for (int L = 0; L < 2; ++L) {
+        const int lo = L == 0 ? b_lo : b_mid, hi = L == 0 ? b_mid : b_hi;
+        int b1 = -1;
+        for (int b = lo; b < hi; ++b)
+          if (p2[b] == j && b1 < 0) b1 = b;
+        bool found = b1 >= 0;
+        double T[D][D];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int s = 0; s < D; ++s) T[r][s] = 0.0;
+        double c1[3] = {0, 0, 0}, om1 = 0;
+        if (b1 >= 0) {
+          c1[0] = cs[3 * b1]; c1[1] = cs[3 * b1 + 1]; c1[2] = cs[3 * b1 + 2];
+          om1 = 1 - bdamage[b1];
+        }
+        const double kn = Kn[2 * i + L];
+        for (int b2 = lo; b2 < hi; ++b2) {
+          const int k = p2[b2];
+          const double c2[3] = {cs[3 * b2], cs[3 * b2 + 1], cs[3 * b2 + 2]};
+          const double om2 = 1 - bdamage[b2];
+          if (b1 >= 0 && k == j) {
+            // direct bond term (derivative.h:59-88 / 156-185)
+            const double* csum = cs_sum + (2 * k + L) * 3;
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+              for (int s = 0; s < D; ++s) {
+                const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
+                const double geo = (r == s) ? (kn + tv_i) * (c2[r] * c2[r])
+                                            : (kn + tv_i) * c2[hi_rs] * c2[lo_rs];
+                T[r][s] += om2 * ((-kn * c2[s] + tv_i * csum[s]) * c2[r] - geo);
+              }
+            continue;
+          }
+          if (b1 >= 0) {
+            // common-neighbour term through i's dilatation (derivative.h:91-99 / 188-196)
+            const double coef = 0.5 * (om1 * tv_i + om2 * tv_i);
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+              for (int s = 0; s < D; ++s) T[r][s] -= coef * c2[r] * c1[s];
+          }
+          // two-hop paths i->k->j through k's dilatation
+          // (derivative.h:101-115 / 122-139 and 198-212 / 219-236)
+          const double tv_k = Tv[k];
+          const int klo = L == 0 ? bptr[k] : bptr[k] + nb0[k];
+          const int khi = L == 0 ? bptr[k] + nb0[k] : bptr[k + 1];
+          for (int b3 = klo; b3 < khi; ++b3) {
+            if (p2[b3] != j) continue;
+            found = true;
+            const double c3[3] = {cs[3 * b3], cs[3 * b3 + 1], cs[3 * b3 + 2]};
+            const double coef3 = 0.5 * ((1 - bdamage[b3]) * tv_k + om2 * tv_i);
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+              for (int s = 0; s < D; ++s) T[r][s] -= coef3 * c2[r] * c3[s];
+          }
+        }
+        if (found)
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) K[r][s] = K[r][s] + T[r][s];
+      }
+    }
+    if (active) {
+      double* out = H + (size_t)e * D * D;
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) out[r * D + s] = K[r][s];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+norm_diag_kernel(const double* __restrict__ diag_sq, int N, double* __restrict__ partials,
+                 Scalars* sc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  Vec<1> a;
+  a.v[0] = i < N ? diag_sq[i] : 0.0;
+  a = block_sum<256, 1>(a);
+  Vec<1> tot;
+  if (last_block_sum<256, 1>(a, partials, &sc->counter[1], &tot))
+    if (threadIdx.x == 0) sc->norm_diag = sqrt(tot.v[0]);
+}
+
+__device__ __forceinline__ bool dof_fixed(const int* cons, const double* dvis, int i, int k) {
+  return cons[3 * i + k] == 1 || fabs(dvis[i] - 1.0) < kEPS;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+symmetrize_bc_kernel(const int* __restrict__ brow, const int* __restrict__ conns,
+                     const int* __restrict__ tpos, int64_t nnzb, const int* __restrict__ cons,
+                     const double* __restrict__ dvis, const Scalars* __restrict__ sc,
+                     double* __restrict__ Kval, float* __restrict__ Kval32) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnzb) return;
+  const int i = brow[e], j = conns[e];
+  if (i > j) return;
+  const double nd = sc->norm_diag;
+  bool fi[D], fj[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    fi[k] = dof_fixed(cons, dvis, i, k);
+    fj[k] = dof_fixed(cons, dvis, j, k);
+  }
+  double* Ke = Kval + (size_t)e * D * D;
+  if (i == j) {
+    double v[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) v[r][s] = (s >= r) ? Ke[r * D + s] : Ke[s * D + r];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        if (fi[r] || fi[s]) v[r][s] = (r == s) ? nd : 0.0;
+        Ke[r * D + s] = v[r][s];
+        if (Kval32) Kval32[(size_t)e * D * D + r * D + s] = (float)v[r][s];
+      }
+    return;
+  }
+  const int t = tpos[e];
+  double* Kt = Kval + (size_t)t * D * D;
+  double a[D][D], b[D][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      a[r][s] = Ke[r * D + s];
+      b[r][s] = Kt[r * D + s];
+    }
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      double v = 0.5 * a[r][s] + 0.5 * b[s][r];
+      if (fi[r] || fj[s]) v = 0.0;
+      Ke[r * D + s] = v;
+      Kt[s * D + r] = v;
+      if (Kval32) {
+        Kval32[(size_t)e * D * D + r * D + s] = (float)v;
+        Kval32[(size_t)t * D * D + s * D + r] = (float)v;
+      }
+    }
+}
+
+// Reference upper-triangular 1-based CSR (IK/JK/K_global) from the BSR.
+template <int D>
+__global__ void export_csr_kernel(const int64_t* __restrict__ Kp, const int* __restrict__ cptr,
+                                  const int* __restrict__ conns, const int* __restrict__ dpos,
+                                  const int* __restrict__ ncl, const double* __restrict__ Kval,
+                                  int N, int64_t* __restrict__ IK, int64_t* __restrict__ JK,
+                                  double* __restrict__ K) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int64_t kp = Kp[i];
+  const int n = ncl[i];
+  const int dp = dpos[i];
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    const int64_t start = kp + (int64_t)r * D * n - r * (r + 1) / 2 + r;
+    IK[(size_t)D * i + r] = start + 1;
+    int64_t pos = start;
+    for (int s = r; s < D; ++s, ++pos) {
+      JK[pos] = (int64_t)D * i + s + 1;
+      K[pos] = Kval[(size_t)dp * D * D + r * D + s];
+    }
+    for (int e = dp + 1; e < cptr[i + 1]; ++e) {
+      const int j = conns[e];
+      for (int s = 0; s < D; ++s, ++pos) {
+        JK[pos] = (int64_t)D * j + s + 1;
+        K[pos] = Kval[(size_t)e * D * D + r * D + s];
+      }
+    }
+  }
+  if (i == N - 1) IK[(size_t)D * N] = Kp[N] + 1;
+}
+
+}  // namespace
+
+int launch_assemble(Context* c, int want32) {
+  const int N = c->N;
+  const int nbw = (N + kWaves - 1) / kWaves;
+  const int tb = 256;
+  const int nbn = (N + tb - 1) / tb;
+  const int64_t nbe = (c->nnzb + tb - 1) / tb;
+  if (nbn > c->n_partials) {
+    set_error("partials buffer too small");
+    return -1;
+  }
+  if (want32 && !c->Kval32 && dalloc(c, &c->Kval32, (size_t)c->nnzb * c->dim * c->dim)) return -3;
+  float* k32 = want32 ? c->Kval32 : nullptr;
+  if (c->dim == 3) {
+    tangent_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                         c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                         N, c->Kval, c->diag_sq);
+    norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
+    symmetrize_bc_kernel<3><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
+                                                      c->damage_visual, c->sc, c->Kval, k32);
+  } else {
+    tangent_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                         c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                         N, c->Kval, c->diag_sq);
+    norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
+    symmetrize_bc_kernel<2><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
+                                                      c->damage_visual, c->sc, c->Kval, k32);
+  }
+  LPMG_HIP(hipGetLastError());
+  c->assembled64 = true;
+  c->assembled32 = want32 != 0;
+  return 0;
+}
+
+int launch_export_csr(Context* c, int64_t* dKp, int64_t* dIK, int64_t* dJK, double* dK) {
+  (void)dKp;
+  const int N = c->N, tb = 256, nb = (N + tb - 1) / tb;
+  if (c->dim == 3)
+    export_csr_kernel<3><<<nb, tb, 0, c->stream>>>(c->Kp, c->cptr, c->conns, c->dpos, c->ncl, c->Kval,
+                                                  N, dIK, dJK, dK);
+  else
+    export_csr_kernel<2><<<nb, tb, 0, c->stream>>>(c->Kp, c->cptr, c->conns, c->dpos, c->ncl, c->Kval,
+                                                  N, dIK, dJK, dK);
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace lpmg

@@ -1,0 +1,170 @@
+// lpmg_internal.h -- device-resident context and kernel entry points of liblpmg.
+//
+// Layout in HBM (one context = one LPM particle system, ids 0..N-1 local to it):
+//   particles   xyz/xyz0/xyz_last as double4 {x,y,z,0}; per-particle SoA arrays
+//   bonds       per particle: layer-0 bonds then layer-1 bonds, each ascending by
+//               partner id (the order of Assembly::createBonds); SoA per bond
+//   conns       sorted 1-/2-hop connection lists; they are also the block-column
+//               lists of the full block-CSR (BSR, b = dim) stiffness used by SpMV
+//   stiffness   BSR values, b*b row-major per block, FP64 and (mixed) FP32 copies
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lpmg {
+
+constexpr double kEPS = 1e-6;  // include/lpm.h:41
+
+enum Lattice { kSquare2D = 0, kHexagon2D = 1, kSC3D = 2, kFCC3D = 3, kBCC3D = 4 };
+enum PType { kElastic = 0, kElasticDamage = 1, kJ2 = 2 };
+
+struct Cell {  // include/unit_cell.h:7-27
+  int lattice = 0, dim = 3, nneighbors = 0;
+  double radius = 0, volume = 0;
+  double cut[2] = {0, 0};
+  double el[9] = {0};
+};
+bool make_cell(int lattice, double r, Cell* c);
+
+// Device scalars of one CG solve and of the reductions around it.
+struct Scalars {
+  double rz, rr, rr0, alpha, beta, pq, thresh;  // CG
+  double rnorm_sq, reac_sq;                     // residual / reaction (updateRR)
+  double norm_diag;                             // updateStiffnessDispBC
+  int iter, done, maxit, any_flag;
+  unsigned int counter[4];                      // last-block-done tickets
+};
+
+template <typename T>
+struct DArr {  // device array owned by the context
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+struct Group {
+  int* idx = nullptr;  // device
+  int n = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Cell cell;
+  int ptype = 0, N = 0, dim = 3;
+  int64_t NB = 0, nnzb = 0, nnz_upper = 0;
+  int max_nb = 0, max_nconn = 0;
+
+  // particles
+  double4 *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr;
+  double *Pex = nullptr, *Pin = nullptr, *stress = nullptr;
+  double *damage = nullptr, *damage_last = nullptr, *damage_visual = nullptr;
+  double *Ddot_local = nullptr, *Ddot_nonlocal = nullptr;
+  double *sv = nullptr, *sv_last = nullptr;                 // 7 per particle
+  double *dLe_tot = nullptr, *TdLe_tot = nullptr;           // [2i+L]
+  double *cs_sum = nullptr;                                 // [(2i+L)*3+c]
+  int *type = nullptr, *cons = nullptr;                     // cons[3i+k]
+  // material
+  double *Kn = nullptr, *Tv = nullptr;                      // Kn[2i+L], Tv[i]
+  double *crit = nullptr, *nlL = nullptr, *mE = nullptr, *mMu = nullptr, *mSy = nullptr,
+         *mXi = nullptr, *mH = nullptr, *mA = nullptr, *k0 = nullptr, *k1 = nullptr,
+         *dthr = nullptr, *ctr = nullptr;
+  // bonds
+  int *bptr = nullptr, *nb0 = nullptr, *p2 = nullptr, *rev = nullptr;
+  double *dis0 = nullptr, *dis = nullptr, *dLe = nullptr, *bstrain = nullptr, *dLp = nullptr,
+         *dLp_last = nullptr, *cs = nullptr, *bforce = nullptr, *bforce_last = nullptr,
+         *bdamage = nullptr, *bdamage_last = nullptr, *dis_last = nullptr;
+  // conns / BSR pattern
+  int *cptr = nullptr, *conns = nullptr, *ncl = nullptr, *tpos = nullptr, *brow = nullptr,
+      *dpos = nullptr;
+  int64_t* Kp = nullptr;
+  double* Kval = nullptr;   // nnzb*b*b
+  float* Kval32 = nullptr;  // nnzb*b*b (mixed precision)
+  double* diag_sq = nullptr;
+  // nonlocal neighbours
+  int *nlptr = nullptr, *nlidx = nullptr;
+  int64_t n_nl = 0;
+  // solver vectors
+  double *residual = nullptr, *du = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr,
+         *cg_z = nullptr, *dinv = nullptr;
+  float *f_r = nullptr, *f_p = nullptr, *f_q = nullptr, *f_z = nullptr, *f_x = nullptr,
+        *f_dinv = nullptr;
+  double* partials = nullptr;  // per-block reduction partials
+  int n_partials = 0;
+  Scalars* sc = nullptr;       // device
+  Scalars* sc_host = nullptr;  // pinned host mirror
+  int* err_flag = nullptr;     // device error flag (setup kernels)
+  double reac_sumsq = 0;       // running sum of squares of every reaction pushed
+  bool assembled64 = false, assembled32 = false;
+  std::vector<Group> groups;
+  std::vector<void*> allocs;
+  // per-kernel event timing (bench instrumentation)
+  bool prof = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double prof_spmv_ms = 0;
+  int64_t prof_spmv_n = 0;
+  int prof_spmv_prec = -1;
+};
+
+// ---- helpers (api.hip)
+void set_error(const std::string& s);
+int hip_fail(hipError_t e, const char* what);
+#define LPMG_HIP(call)                                   \
+  do {                                                   \
+    hipError_t _e = (call);                              \
+    if (_e != hipSuccess) return hip_fail(_e, #call);    \
+  } while (0)
+
+template <typename T>
+int dalloc(Context* c, T** p, size_t n) {
+  if (n == 0) n = 1;
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  c->allocs.push_back(q);
+  *p = static_cast<T*>(q);
+  return 0;
+}
+
+// ---- setup.hip
+int build_bonds_from_search(Context* c, const double* host_xyz);
+int build_bonds_from_list(Context* c, int64_t nbond, const int32_t* p1, const int32_t* p2,
+                          const int32_t* layer);
+int build_topology(Context* c);  // rev, conns, Kp, BSR helpers
+int search_nonlocal(Context* c, double ratio);
+int exclusive_scan_i32(Context* c, const int* in, int* out, int n, int* total);
+int exclusive_scan_i64(Context* c, const int64_t* in, int64_t* out, int n);
+
+// ---- forward.hip
+int launch_forward(Context* c);
+int launch_update_rr(Context* c, double* norm_res, double* norm_reac_step_sq);
+int launch_disp_bc(Context* c, const Group& g, int mode, int k, double step);
+int launch_force_bc(Context* c, const Group& g, int mode, double fx, double fy, double fz);
+int launch_move_by(Context* c, const double* du);
+
+// ---- assemble.hip
+int launch_assemble(Context* c, int want32);
+int launch_export_csr(Context* c, int64_t* dKp, int64_t* dIK, int64_t* dJK, double* dK);
+
+// ---- solver.hip
+struct SolveOpts {
+  int precond = 1;      // 0 plain CG (the reference's MKL call), 1 Jacobi PCG
+  int precision = 0;    // 0 FP64, 1 FP32 inner solve
+  double rtol = 1e-12, atol = 1e-12;  // ||r||^2 <= rtol*||r0||^2 + atol
+  int64_t maxit = 0;    // 0 -> problem size
+  int chunk = 16;       // iterations launched between convergence checks
+};
+int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
+int launch_spmv(Context* c, int precision, const void* x, void* y);
+double spmv_bytes(const Context* c, int precision);
+
+// ---- state.hip
+int launch_update_state_var(Context* c);
+int launch_static_damage(Context* c, int undamaged, int* any);
+int store_state(Context* c);
+int reset_state(Context* c, int reset_xyz);
+
+}  // namespace lpmg

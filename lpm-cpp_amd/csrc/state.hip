@@ -1,0 +1,209 @@
+// state.hip -- converged-state updates of the quasi-static step.
+//
+// Reference: Assembly::updateStateVar (assembly.h:161-166) with
+// ParticleJ2Plasticity::updateParticleStateVariables (particle_j2plasticity.h:78-140)
+// and ParticleElasticDamage::updateParticleStateVariables /
+// calcStrainEnergyDensity (particle_elastic_damage.h:89-104, 150-161);
+// SolverStatic::updateStaticDamage (solver_static.h:31-62) with the three
+// updateParticleStaticDamage variants (particle_elastic.h:52-68,
+// particle_elastic_damage.h:163-173, particle_j2plasticity.h:142-152) and
+// func_phi (lpm.h:49-53); Assembly::storeStateVar / resetStateVar
+// (assembly.h:140-159).  All arithmetic is FP64: these are the threshold tests
+// (yield, bond stretch, damage) whose flags must match the reference exactly.
+#include "lpmg_internal.h"
+
+namespace lpmg {
+
+namespace {
+
+__global__ void __launch_bounds__(256)
+j2_update_kernel(const int* __restrict__ bptr, const double* __restrict__ cs,
+                 const double* __restrict__ dis, const double* __restrict__ stress,
+                 const double* __restrict__ mE, const double* __restrict__ mMu,
+                 const double* __restrict__ mSy, const double* __restrict__ mXi,
+                 const double* __restrict__ mH, const double* __restrict__ mA, int N,
+                 double* __restrict__ sv, double* __restrict__ dLp, double* __restrict__ Ddot_local) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double* s = sv + 7 * (size_t)i;
+  double st[6];
+  for (int k = 0; k < 6; ++k) st[k] = stress[6 * (size_t)i + k];
+  double dpl[6] = {0, 0, 0, 0, 0, 0};
+  double sigma_eq = 0, dlambda = 0, triax = 0;
+  const double sigma_m = 1.0 / 3.0 * (st[0] + st[1] + st[2]);
+  for (int k = 0; k < 3; ++k) st[k] -= sigma_m;
+  for (int k = 0; k < 6; ++k) st[k] -= s[k + 1];
+  for (int k = 0; k < 6; ++k) sigma_eq += (k < 3) ? st[k] * st[k] : 2.0 * st[k] * st[k];
+  sigma_eq = sqrt(3.0 / 2.0 * sigma_eq);
+  const double H = mH[i], xi = mXi[i];
+  const double yield = sigma_eq - (mSy[i] + (1.0 - xi) * H * s[0]);
+  const double G = mE[i] / 2 / (1 + mMu[i]);
+  if (yield > 0.0) dlambda = yield / (3 * G + H);
+  s[0] += dlambda;
+  for (int k = 0; k < 6; ++k)
+    if (fabs(sigma_eq) > kEPS) {
+      dpl[k] = dlambda * 1.5 * st[k] / sigma_eq;
+      s[k + 1] += 2. / 3. * xi * H * dpl[k];
+    }
+  for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+    const double cx = cs[3 * b], cy = cs[3 * b + 1], cz = cs[3 * b + 2];
+    const double ddLp = dis[b] * (dpl[0] * cx * cx + dpl[1] * cy * cy + dpl[2] * cz * cz +
+                                  2 * dpl[3] * cy * cz + 2 * dpl[4] * cx * cz + 2 * dpl[5] * cx * cy);
+    dLp[b] += ddLp;
+  }
+  if (sigma_eq > kEPS) triax = sigma_m / sigma_eq;
+  Ddot_local[i] = dlambda * (1.0 + mA[i] * triax);
+}
+
+__global__ void __launch_bounds__(256)
+damage_energy_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
+                     const double* __restrict__ dLe, const double* __restrict__ Kn,
+                     const double* __restrict__ dLe_tot, const double* __restrict__ TdLe_tot,
+                     const double* __restrict__ damage, const double* __restrict__ k0,
+                     const double* __restrict__ k1, double volume, double nneighbors, int N,
+                     double* __restrict__ sv, const double* __restrict__ sv_last,
+                     double* __restrict__ Ddot_local) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int lo = bptr[i], mid = lo + nb0[i], hi = bptr[i + 1];
+  const double V_m = volume * (hi - lo) / nneighbors;
+  double e = 0;
+  for (int L = 0; L < 2; ++L) {
+    const double kn = Kn[2 * i + L];
+    for (int b = (L == 0 ? lo : mid); b < (L == 0 ? mid : hi); ++b) e += 0.5 * (kn * dLe[b] * dLe[b]) / V_m;
+    e += 0.5 * TdLe_tot[2 * i + L] * dLe_tot[2 * i + L] / V_m;
+  }
+  sv[7 * (size_t)i] = e;
+  const double delta = e - sv_last[7 * (size_t)i];
+  const double f = e - k1[i] * damage[i] - k0[i];
+  Ddot_local[i] = (f > 0 && delta > 0) ? (1 / k1[i]) * delta : 0;
+}
+
+__device__ __forceinline__ double func_phi(double x, double L) {
+  const double t = 1 / (1 + pow(x / L, 8.0));
+  return t * t;
+}
+
+__global__ void __launch_bounds__(256)
+nonlocal_average_kernel(const double4* __restrict__ xyz, const int* __restrict__ type,
+                        const int* __restrict__ bptr, const int* __restrict__ nlptr,
+                        const int* __restrict__ nlidx, const double* __restrict__ nlL,
+                        const double* __restrict__ Ddot_local, double volume, double nneighbors,
+                        int undamaged, int N, double* __restrict__ Ddot_nonlocal) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || type[i] == undamaged) return;
+  double A = 0, Dn = 0;
+  const double4 pi = xyz[i];
+  const double L = nlL[i];
+  if (nlptr) {
+    for (int t = nlptr[i]; t < nlptr[i + 1]; ++t) {
+      const int j = nlidx[t];
+      if (type[j] == undamaged) continue;
+      const double4 pj = xyz[j];
+      const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+      const double d = sqrt(dx * dx + dy * dy + dz * dz);
+      const double V_m = volume * (bptr[j + 1] - bptr[j]) / nneighbors;
+      Dn += Ddot_local[j] * func_phi(d, L) * V_m;
+      A += func_phi(d, L) * V_m;
+    }
+  }
+  Ddot_nonlocal[i] = Dn / A;  // 0/0 = NaN without nonlocal neighbours, as the reference
+}
+
+__global__ void __launch_bounds__(256)
+particle_damage_kernel(const int* __restrict__ type, const int* __restrict__ bptr,
+                       const double* __restrict__ bstrain, const double* __restrict__ crit,
+                       const double* __restrict__ Ddot_nonlocal, const double* __restrict__ dthr,
+                       const double* __restrict__ damage_last, int ptype, int undamaged, int N,
+                       double* __restrict__ bdamage, double* __restrict__ damage, int* __restrict__ any) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || type[i] == undamaged) return;
+  bool mine = false;
+  if (ptype == kElastic) {
+    for (int b = bptr[i]; b < bptr[i + 1]; ++b)
+      if (bstrain[b] >= crit[i] && fabs(bdamage[b] - 1.0) > kEPS) {
+        mine = true;
+        bdamage[b] = 1;
+      }
+  } else {
+    double d = damage[i] + Ddot_nonlocal[i];
+    if (d >= dthr[i]) d = 1;
+    damage[i] = d;
+    mine = fabs(d - damage_last[i]) > 1e-3;
+  }
+  if (mine) atomicOr(any, 1);
+}
+
+__global__ void copy_xyz_kernel(double4* __restrict__ dst, const double4* __restrict__ src, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) dst[i] = src[i];
+}
+
+}  // namespace
+
+int launch_update_state_var(Context* c) {
+  const int N = c->N, tb = 256, nb = (N + tb - 1) / tb;
+  if (c->ptype == kJ2)
+    j2_update_kernel<<<nb, tb, 0, c->stream>>>(c->bptr, c->cs, c->dis, c->stress, c->mE, c->mMu, c->mSy,
+                                               c->mXi, c->mH, c->mA, N, c->sv, c->dLp, c->Ddot_local);
+  else if (c->ptype == kElasticDamage)
+    damage_energy_kernel<<<nb, tb, 0, c->stream>>>(c->bptr, c->nb0, c->dLe, c->Kn, c->dLe_tot,
+                                                   c->TdLe_tot, c->damage, c->k0, c->k1, c->cell.volume,
+                                                   (double)c->cell.nneighbors, N, c->sv, c->sv_last,
+                                                   c->Ddot_local);
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_static_damage(Context* c, int undamaged, int* any) {
+  const int N = c->N, tb = 256, nb = (N + tb - 1) / tb;
+  LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
+  nonlocal_average_kernel<<<nb, tb, 0, c->stream>>>(c->xyz, c->type, c->bptr, c->nlptr, c->nlidx,
+                                                    c->nlL, c->Ddot_local, c->cell.volume,
+                                                    (double)c->cell.nneighbors, undamaged, N,
+                                                    c->Ddot_nonlocal);
+  particle_damage_kernel<<<nb, tb, 0, c->stream>>>(c->type, c->bptr, c->bstrain, c->crit,
+                                                   c->Ddot_nonlocal, c->dthr, c->damage_last, c->ptype,
+                                                   undamaged, N, c->bdamage, c->damage, c->err_flag);
+  LPMG_HIP(hipGetLastError());
+  int h = 0;
+  LPMG_HIP(hipMemcpyAsync(&h, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  LPMG_HIP(hipStreamSynchronize(c->stream));
+  LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
+  *any = h;
+  return 0;
+}
+
+int store_state(Context* c) {
+  const size_t N = c->N, NB = c->NB;
+  auto cp = [&](void* d, const void* s, size_t bytes) {
+    return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream);
+  };
+  copy_xyz_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->xyz_last, c->xyz, c->N);
+  LPMG_HIP(cp(c->damage_last, c->damage, N * sizeof(double)));
+  LPMG_HIP(cp(c->sv_last, c->sv, 7 * N * sizeof(double)));
+  LPMG_HIP(cp(c->dis_last, c->dis, NB * sizeof(double)));
+  LPMG_HIP(cp(c->bdamage_last, c->bdamage, NB * sizeof(double)));
+  LPMG_HIP(cp(c->bforce_last, c->bforce, NB * sizeof(double)));
+  LPMG_HIP(cp(c->dLp_last, c->dLp, NB * sizeof(double)));
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+int reset_state(Context* c, int reset_xyz) {
+  const size_t N = c->N, NB = c->NB;
+  auto cp = [&](void* d, const void* s, size_t bytes) {
+    return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream);
+  };
+  if (reset_xyz) copy_xyz_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->xyz, c->xyz_last, c->N);
+  LPMG_HIP(cp(c->damage, c->damage_last, N * sizeof(double)));
+  LPMG_HIP(cp(c->sv, c->sv_last, 7 * N * sizeof(double)));
+  LPMG_HIP(cp(c->dis, c->dis_last, NB * sizeof(double)));
+  LPMG_HIP(cp(c->bdamage, c->bdamage_last, NB * sizeof(double)));
+  LPMG_HIP(cp(c->bforce, c->bforce_last, NB * sizeof(double)));
+  LPMG_HIP(cp(c->dLp, c->dLp_last, NB * sizeof(double)));
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace lpmg

@@ -1,0 +1,1291 @@
+// lpm_oracle.cpp -- CPU FP64 restatement of LPM-CPP's quasi-static Newton step.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: tests/, the smoke check
+// (__graft_entry__.smoke) and bench.py's cpu_baseline leg load it to compare the
+// HIP path against.  The product (liblpmg.so) never links, loads or calls it.
+//
+// Every function restates the reference function cited beside it (paths are
+// relative to the LPM-CPP repository root).  Arithmetic keeps the reference's
+// operation order; the file is compiled with -ffp-contract=off because the
+// reference's x86-64 Release build has no FMA contraction either.
+//
+// Third-party arithmetic restated from published documentation (not present in
+// the reference tree): Intel MKL 2021.4 RCI CG (dcg_init/dcg_check/dcg/dcg_get)
+// -- the stopping test is the documented one on squared norms,
+// dpar[4] <= dpar[3] = dpar[0]*dpar[2] + dpar[1]; cblas_dnrm2 -- Euclidean norm;
+// cblas_dgemm for the 3x3*3x1 el_mapping product.
+//
+// How each part is pinned (tests/test_oracle_*.py):
+//   unit cell / KnTv          closed form of the el_mapping algebra
+//   bonds, layers, conns      brute-force O(N^2) search; shell-count invariants
+//   geometry/force/stress     rigid motion -> zero force; affine stretch closed
+//                             form; global force balance; pair antisymmetry
+//   tangent blocks            central finite differences of Pin at a bforce=0
+//                             state (where the reference's analytic tangent is
+//                             the exact Jacobian); symmetry
+//   CSR indices               closed form from conns (row starts, columns)
+//   CG / Newton               scipy sparse direct solve on small systems;
+//                             residual of the converged state
+//   J2 return / damage        textbook radial return; energy closed form
+// CG iteration counts are parity unpinned (MKL's internals are closed).
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+constexpr double kEPS = 1e-6;  // include/lpm.h:41
+
+enum Lattice { kSquare2D = 0, kHexagon2D = 1, kSC3D = 2, kFCC3D = 3, kBCC3D = 4 };
+enum PType { kElastic = 0, kElasticDamage = 1, kJ2 = 2 };
+
+thread_local std::string g_err;
+
+struct Cell {  // include/unit_cell.h:7-27
+  int lattice = 0, dim = 3, nneighbors = 0;
+  double radius = 0, volume = 0;
+  double cut[2] = {0, 0};
+  double el[9] = {0};
+};
+
+// UnitCell::UnitCell and set* (include/unit_cell.h:29-122)
+bool make_cell(int lattice, double r, Cell* c) {
+  c->lattice = lattice;
+  c->radius = r;
+  switch (lattice) {
+    case kSquare2D: {  // unit_cell.h:44-58
+      const double e[9] = {1. / 2., -1. / 2., 0., 0., 0., 1. / 2., 0., 1. / 12., -1. / 12.};
+      std::memcpy(c->el, e, sizeof e);
+      c->dim = 2; c->nneighbors = 8;
+      c->cut[0] = 2.0 * r; c->cut[1] = 2.0 * std::sqrt(2.0) * r;
+      c->volume = 4 * (r * r);
+      return true;
+    }
+    case kHexagon2D: {  // unit_cell.h:60-74
+      const double e[9] = {std::sqrt(3.0) / 12.0, -std::sqrt(3.0) / 12.0, 0.,
+                           -std::sqrt(3.0) / 144.0, std::sqrt(3.0) / 48.0, 0.0,
+                           0.0, 0.0, 1.0};
+      std::memcpy(c->el, e, sizeof e);
+      c->dim = 2; c->nneighbors = 12;
+      c->cut[0] = 2.0 * r; c->cut[1] = 2.0 * std::sqrt(3.0) * r;
+      c->volume = 2 * std::sqrt(3.0) * (r * r);
+      return true;
+    }
+    case kSC3D: {  // unit_cell.h:76-90
+      const double e[9] = {1, -1, -1.0, 0.0, 0.0, 1., 0.0, 1 / 18.0, -1 / 18.0};
+      std::memcpy(c->el, e, sizeof e);
+      c->dim = 3; c->nneighbors = 18;
+      c->cut[0] = 2.0 * r; c->cut[1] = 2.0 * std::sqrt(2.0) * r;
+      c->volume = std::pow(2 * r, 3.0);
+      return true;
+    }
+    case kFCC3D: {  // unit_cell.h:92-106
+      const double e[9] = {0.0, 0.0, std::sqrt(2.0),
+                           std::sqrt(2.0) / 4, -std::sqrt(2.0) / 4, -std::sqrt(2.0) / 4,
+                           0.0, std::sqrt(2.0) / 24, -std::sqrt(2.0) / 24};
+      std::memcpy(c->el, e, sizeof e);
+      c->dim = 3; c->nneighbors = 18;
+      c->cut[0] = 2.0 * r; c->cut[1] = 2.0 * std::sqrt(2.0) * r;
+      c->volume = 4.0 * std::sqrt(2.0) * std::pow(r, 3.0);
+      return true;
+    }
+    case kBCC3D: {  // unit_cell.h:108-122
+      const double e[9] = {0.0, 0.0, std::sqrt(3.0),
+                           1 / std::sqrt(3.0), -1 / std::sqrt(3.0), 0.,
+                           0.0, std::sqrt(3.0) / 14, std::sqrt(3.0) / 14};
+      std::memcpy(c->el, e, sizeof e);
+      c->dim = 3; c->nneighbors = 14;
+      c->cut[0] = 2.0 * r; c->cut[1] = 4.0 / std::sqrt(3.0) * r;
+      c->volume = 32.0 * std::sqrt(3.0) / 9.0 * std::pow(r, 3.0);
+      return true;
+    }
+  }
+  return false;
+}
+
+struct Group { std::vector<int> idx; };
+
+struct Ctx {
+  Cell cell;
+  int ptype = 0, N = 0, dim = 3;
+  // particles (xyz always 3 components; z = 0 in 2D)
+  std::vector<double> xyz, xyz0, xyz_last, Pex, Pin, stress;
+  std::vector<int> type, cons;
+  std::vector<double> damage, damage_last, damage_visual, Ddot_local, Ddot_nonlocal;
+  std::vector<double> sv, sv_last;           // 7 per particle
+  std::vector<double> dLe_tot, TdLe_tot;     // [i*2+L]
+  std::vector<double> cs_sum;                // [(i*2+L)*3+c]
+  // material (Particle*::setParticleProperty)
+  std::vector<double> Kn, Tv;                // Kn[i*2+L], Tv[i]
+  std::vector<double> crit, nlL, mE, mMu, mSy, mXi, mH, mA, k0, k1, dthr, ctr;
+  // bonds (per particle: layer-0 bonds then layer-1 bonds)
+  std::vector<int> bptr, nb0, p2, rev;
+  std::vector<double> dis0, dis, dL, dLe, bstrain, dLp, dLp_last, cs, bforce,
+      bforce_last, bdamage, bdamage_last, dis_last;
+  // conns
+  std::vector<int> cptr, conns, ncl;
+  // reference upper-triangular 1-based CSR
+  std::vector<int64_t> Kp, IK, JK;
+  std::vector<double> K, residual;
+  // full CSR (0-based, both triangles) built from K for threaded SpMV
+  std::vector<int64_t> fptr; std::vector<int> fcol; std::vector<double> fval;
+  // nonlocal neighbours
+  std::vector<int> nlptr, nlidx;
+  double reac_sumsq = 0;
+  std::vector<Group> groups;
+  int threads = 1;
+
+  int nb(int i) const { return bptr[i + 1] - bptr[i]; }
+  int lb(int i, int L) const { return L == 0 ? bptr[i] : bptr[i] + nb0[i]; }
+  int le(int i, int L) const { return L == 0 ? bptr[i] + nb0[i] : bptr[i + 1]; }
+};
+
+inline double dist(const Ctx& c, int a, int b) {  // Particle::distanceTo (particle.h:276-280)
+  const double dx = c.xyz[3 * a] - c.xyz[3 * b];
+  const double dy = c.xyz[3 * a + 1] - c.xyz[3 * b + 1];
+  const double dz = c.xyz[3 * a + 2] - c.xyz[3 * b + 2];
+  return std::sqrt(dx * dx + dy * dy + dz * dz);
+}
+
+void alloc_particles(Ctx* c) {
+  const int N = c->N;
+  c->xyz_last = c->xyz; c->xyz0 = c->xyz;
+  c->Pex.assign(3 * N, 0); c->Pin.assign(3 * N, 0); c->stress.assign(6 * N, 0);
+  c->type.assign(N, 0); c->cons.assign(3 * N, 0);
+  c->damage.assign(N, 0); c->damage_last.assign(N, 0); c->damage_visual.assign(N, 0);
+  c->Ddot_local.assign(N, 0); c->Ddot_nonlocal.assign(N, 0);
+  c->sv.assign(7 * N, 0); c->sv_last.assign(7 * N, 0);
+  c->dLe_tot.assign(2 * N, 0); c->TdLe_tot.assign(2 * N, 0); c->cs_sum.assign(6 * N, 0);
+  c->Kn.assign(2 * N, 0); c->Tv.assign(N, 0);
+  for (auto* v : {&c->crit, &c->nlL, &c->mE, &c->mMu, &c->mSy, &c->mXi, &c->mH, &c->mA,
+                  &c->k0, &c->k1, &c->dthr, &c->ctr})
+    v->assign(N, 0);
+  c->nlptr.assign(N + 1, 0);
+}
+
+void alloc_bond_state(Ctx* c) {
+  const size_t NB = c->p2.size();
+  for (auto* v : {&c->dis, &c->dL, &c->dLe, &c->bstrain, &c->dLp, &c->dLp_last, &c->bforce,
+                  &c->bforce_last, &c->bdamage, &c->bdamage_last})
+    v->assign(NB, 0);
+  c->cs.assign(3 * NB, 0);
+  c->dis_last = c->dis0;
+  c->dis = c->dis0;
+}
+
+// Bond(p1,p2,layer,dis) initial direction cosines (bond.h:59-72)
+void init_bond_cs(Ctx* c) {
+  for (int i = 0; i < c->N; ++i)
+    for (int b = c->bptr[i]; b < c->bptr[i + 1]; ++b) {
+      const int j = c->p2[b];
+      const double d = c->dis0[b];
+      for (int k = 0; k < 3; ++k) c->cs[3 * b + k] = (c->xyz[3 * i + k] - c->xyz[3 * j + k]) / d;
+    }
+}
+
+// Assembly::createBonds (assembly.h:212-234).  The reference tests all N^2
+// pairs; a uniform cell list with cells of edge >= 1.01*cutoff[1] visits a
+// superset of every pair that can pass the test, and the candidates are then
+// ordered by partner id, which reproduces the reference's per-layer order.
+// brute != 0 runs the literal O(N^2) loop (used by tests to pin the cell list).
+void create_bonds(Ctx* c, int brute) {
+  const int N = c->N;
+  const double rc1 = 1.01 * c->cell.cut[1], rc0 = 1.01 * c->cell.cut[0];
+  std::vector<std::vector<int>> l0(N), l1(N);
+  std::vector<std::vector<double>> d0(N), d1(N);
+  auto consider = [&](int i, int j) {
+    const double distance = dist(*c, i, j);
+    if (distance < rc1 && i != j) {
+      if (distance < rc0) { l0[i].push_back(j); d0[i].push_back(distance); }
+      else { l1[i].push_back(j); d1[i].push_back(distance); }
+    }
+  };
+  if (brute) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) consider(i, j);
+  } else {
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) { lo[k] = 1e300; hi[k] = -1e300; }
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], c->xyz[3 * i + k]);
+        hi[k] = std::max(hi[k], c->xyz[3 * i + k]);
+      }
+    const double h = rc1 * 1.0000001;
+    int64_t nc[3];
+    for (int k = 0; k < 3; ++k) nc[k] = std::max<int64_t>(1, (int64_t)std::floor((hi[k] - lo[k]) / h) + 1);
+    std::vector<int64_t> cell_of(N);
+    std::vector<int64_t> cstart(nc[0] * nc[1] * nc[2] + 1, 0);
+    for (int i = 0; i < N; ++i) {
+      int64_t ci[3];
+      for (int k = 0; k < 3; ++k) ci[k] = std::min(nc[k] - 1, (int64_t)std::floor((c->xyz[3 * i + k] - lo[k]) / h));
+      cell_of[i] = (ci[2] * nc[1] + ci[1]) * nc[0] + ci[0];
+      cstart[cell_of[i] + 1]++;
+    }
+    for (size_t q = 1; q < cstart.size(); ++q) cstart[q] += cstart[q - 1];
+    std::vector<int> items(N);
+    {
+      std::vector<int64_t> fill(cstart.begin(), cstart.end() - 1);
+      for (int i = 0; i < N; ++i) items[fill[cell_of[i]]++] = i;
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < N; ++i) {
+      const int64_t cc = cell_of[i];
+      const int64_t cx = cc % nc[0], cy = (cc / nc[0]) % nc[1], cz = cc / (nc[0] * nc[1]);
+      std::vector<int> cand;
+      for (int64_t z = std::max<int64_t>(0, cz - 1); z <= std::min(nc[2] - 1, cz + 1); ++z)
+        for (int64_t y = std::max<int64_t>(0, cy - 1); y <= std::min(nc[1] - 1, cy + 1); ++y)
+          for (int64_t x = std::max<int64_t>(0, cx - 1); x <= std::min(nc[0] - 1, cx + 1); ++x) {
+            const int64_t q = (z * nc[1] + y) * nc[0] + x;
+            for (int64_t t = cstart[q]; t < cstart[q + 1]; ++t) cand.push_back(items[t]);
+          }
+      std::sort(cand.begin(), cand.end());
+      for (int j : cand) consider(i, j);
+    }
+  }
+  c->bptr.assign(N + 1, 0);
+  c->nb0.assign(N, 0);
+  for (int i = 0; i < N; ++i) {
+    c->nb0[i] = (int)l0[i].size();
+    c->bptr[i + 1] = c->bptr[i] + (int)(l0[i].size() + l1[i].size());
+  }
+  c->p2.resize(c->bptr[N]);
+  c->dis0.resize(c->bptr[N]);
+  for (int i = 0; i < N; ++i) {
+    int b = c->bptr[i];
+    for (size_t t = 0; t < l0[i].size(); ++t, ++b) { c->p2[b] = l0[i][t]; c->dis0[b] = d0[i][t]; }
+    for (size_t t = 0; t < l1[i].size(); ++t, ++b) { c->p2[b] = l1[i][t]; c->dis0[b] = d1[i][t]; }
+  }
+}
+
+// Assembly::readBond (assembly.h:390-424): bonds kept in file order per layer.
+bool set_bonds(Ctx* c, int64_t nbond, const int32_t* p1, const int32_t* p2v, const int32_t* layer) {
+  const int N = c->N;
+  std::vector<std::vector<int>> l0(N), l1(N);
+  for (int64_t t = 0; t < nbond; ++t) {
+    if (p1[t] < 0 || p1[t] >= N || p2v[t] < 0 || p2v[t] >= N || (layer[t] != 0 && layer[t] != 1)) {
+      g_err = "bond list holds an id or layer out of range";
+      return false;
+    }
+    (layer[t] == 0 ? l0 : l1)[p1[t]].push_back(p2v[t]);
+  }
+  c->bptr.assign(N + 1, 0);
+  c->nb0.assign(N, 0);
+  for (int i = 0; i < N; ++i) {
+    c->nb0[i] = (int)l0[i].size();
+    c->bptr[i + 1] = c->bptr[i] + (int)(l0[i].size() + l1[i].size());
+  }
+  c->p2.resize(c->bptr[N]);
+  c->dis0.resize(c->bptr[N]);
+  for (int i = 0; i < N; ++i) {
+    int b = c->bptr[i];
+    for (int j : l0[i]) { c->p2[b] = j; c->dis0[b] = dist(*c, i, j); ++b; }
+    for (int j : l1[i]) { c->p2[b] = j; c->dis0[b] = dist(*c, i, j); ++b; }
+  }
+  return true;
+}
+
+// Opposite bond lookup: std::find_if over bd->p2->bond_layers[layer] for the
+// first bond pointing back (particle.h:207-209).
+bool build_rev(Ctx* c) {
+  c->rev.assign(c->p2.size(), -1);
+  for (int i = 0; i < c->N; ++i)
+    for (int L = 0; L < 2; ++L)
+      for (int b = c->lb(i, L); b < c->le(i, L); ++b) {
+        const int j = c->p2[b];
+        for (int q = c->lb(j, L); q < c->le(j, L); ++q)
+          if (c->p2[q] == i) { c->rev[b] = q; break; }
+        if (c->rev[b] < 0) { g_err = "bond list is not symmetric (no opposite bond)"; return false; }
+      }
+  return true;
+}
+
+// Assembly::updateConnections (assembly.h:236-266)
+bool update_connections(Ctx* c) {
+  const int N = c->N;
+  std::vector<std::vector<int>> cl(N);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < N; ++i) {
+    std::vector<int>& v = cl[i];
+    for (int L = 0; L < 2; ++L)
+      for (int b = c->lb(i, L); b < c->le(i, L); ++b) {
+        const int k = c->p2[b];
+        v.push_back(k);
+        for (int q = c->lb(k, L); q < c->le(k, L); ++q) v.push_back(c->p2[q]);
+      }
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  }
+  c->cptr.assign(N + 1, 0);
+  c->ncl.assign(N, 0);
+  for (int i = 0; i < N; ++i) c->cptr[i + 1] = c->cptr[i] + (int)cl[i].size();
+  c->conns.resize(c->cptr[N]);
+  for (int i = 0; i < N; ++i) {
+    std::copy(cl[i].begin(), cl[i].end(), c->conns.begin() + c->cptr[i]);
+    auto it = std::find(cl[i].begin(), cl[i].end(), i);
+    c->ncl[i] = (int)std::distance(it, cl[i].end());
+    if (c->ncl[i] == 0) { g_err = "a particle has no bonds (no self connection)"; return false; }
+  }
+  return true;
+}
+
+// Stiffness::initialize (stiffness.h:54-71)
+void stiffness_initialize(Ctx* c) {
+  const int N = c->N, d = c->dim;
+  c->Kp.assign(N + 1, 0);
+  for (int i = 0; i < N; ++i) c->Kp[i + 1] = c->Kp[i] + (int64_t)d * d * c->ncl[i] - (d == 2 ? 1 : 3);
+  c->IK.assign((size_t)d * N + 1, 0);
+  c->JK.assign(c->Kp[N], 0);
+  c->K.assign(c->Kp[N], 0);
+  c->residual.assign((size_t)d * N, 0);
+}
+
+// ---------------------------------------------------------------- forward pass
+
+// Particle::updateBondsGeometry + Bond::updatebGeometry (particle.h:163-181, bond.h:30-39)
+void update_geometry(Ctx* c) {
+  for (int i = 0; i < c->N; ++i)
+    for (int L = 0; L < 2; ++L) {
+      double dLe_total = 0, TdLe_total = 0, sx = 0, sy = 0, sz = 0;
+      for (int b = c->lb(i, L); b < c->le(i, L); ++b) {
+        const int j = c->p2[b];
+        const double dis = dist(*c, i, j);
+        const bool broken = std::fabs(c->bdamage[b] - 1.0) < kEPS;
+        c->dis[b] = dis;
+        c->dL[b] = broken ? 0.0 : (dis - c->dis0[b]);
+        c->bstrain[b] = broken ? 0.0 : (c->dL[b] / dis);
+        c->dLe[b] = broken ? 0.0 : (dis - c->dis0[b] - c->dLp[b]);
+        for (int k = 0; k < 3; ++k)
+          c->cs[3 * b + k] = broken ? 0.0 : ((c->xyz[3 * i + k] - c->xyz[3 * j + k]) / dis);
+        dLe_total += c->dLe[b];
+        TdLe_total += c->Tv[i] * c->dLe[b];
+        sx += c->cs[3 * b]; sy += c->cs[3 * b + 1]; sz += c->cs[3 * b + 2];
+      }
+      c->dLe_tot[2 * i + L] = dLe_total;
+      c->TdLe_tot[2 * i + L] = TdLe_total;
+      c->cs_sum[(2 * i + L) * 3] = sx;
+      c->cs_sum[(2 * i + L) * 3 + 1] = sy;
+      c->cs_sum[(2 * i + L) * 3 + 2] = sz;
+    }
+}
+
+inline double std_max(double a, double b) { return (a < b) ? b : a; }  // std::max semantics
+
+// Assembly::updateForceState (assembly.h:178-191) with updateBondsForce of
+// ParticleElastic (particle_elastic.h:38-50), ParticleElasticDamage
+// (particle_elastic_damage.h:74-87) and ParticleJ2Plasticity
+// (particle_j2plasticity.h:63-76); updateParticleForce (particle.h:197-215);
+// updateParticleStress (particle.h:217-240); updateParticleDamageVisual (particle.h:150-161)
+void update_force_state(Ctx* c) {
+  const int N = c->N;
+  for (int i = 0; i < N; ++i)
+    for (int b = c->bptr[i]; b < c->bptr[i + 1]; ++b) {
+      const int L = (b < c->bptr[i] + c->nb0[i]) ? 0 : 1;
+      const double kn = c->Kn[2 * i + L], tv = c->Tv[i];
+      double f = 2. * kn * c->dLe[b] + 2. * tv * c->dLe_tot[2 * i + L];
+      if (c->ptype != kElastic)
+        c->bdamage[b] = std_max(c->bdamage[b], std_max(c->damage[i], c->damage[c->p2[b]]));
+      f *= (1.0 - c->bdamage[b]);
+      c->bforce[b] = f;
+    }
+  for (int i = 0; i < N; ++i) {
+    double P[3] = {0., 0., 0.};
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    const double V_m = c->cell.volume * c->nb(i) / c->cell.nneighbors;
+    double dv = 0;
+    for (int b = c->bptr[i]; b < c->bptr[i + 1]; ++b) {
+      const double fsum = c->bforce[b] + c->bforce[c->rev[b]];
+      const double cx = c->cs[3 * b], cy = c->cs[3 * b + 1], cz = c->cs[3 * b + 2];
+      P[0] += cx * 0.5 * fsum;
+      P[1] += cy * 0.5 * fsum;
+      P[2] += cz * 0.5 * fsum;
+    }
+    for (int b = c->bptr[i]; b < c->bptr[i + 1]; ++b) {
+      const double fsum = c->bforce[b] + c->bforce[c->rev[b]];
+      const double cx = c->cs[3 * b], cy = c->cs[3 * b + 1], cz = c->cs[3 * b + 2];
+      const double w = 0.5 / V_m * (c->dis[b]) * 0.5 * fsum;
+      s[0] += w * cx * cx;
+      s[1] += w * cy * cy;
+      s[2] += w * cz * cz;
+      s[3] += w * cy * cz;
+      s[4] += w * cx * cz;
+      s[5] += w * cx * cy;
+      dv += c->bdamage[b];
+    }
+    for (int k = 0; k < 3; ++k) c->Pin[3 * i + k] = P[k];
+    for (int k = 0; k < 6; ++k) c->stress[6 * i + k] = s[k];
+    c->damage_visual[i] = dv / (double)c->nb(i);
+  }
+}
+
+// Solver::updateRR (solver.h:111-126).  The reaction vector of the reference is
+// never cleared; only its Euclidean norm is ever read (solver.h:64), so its
+// running sum of squares is kept instead.
+void update_rr(Ctx* c, double* norm_res, double* norm_reac) {
+  const int d = c->dim;
+  double rr = 0;
+  for (int i = 0; i < c->N; ++i)
+    for (int k = 0; k < d; ++k) {
+      const double r = (1 - c->cons[3 * i + k]) * (c->Pex[3 * i + k] - c->Pin[3 * i + k]);
+      c->residual[d * i + k] = r;
+      rr += r * r;
+      if (c->cons[3 * i + k] == 1) c->reac_sumsq += c->Pin[3 * i + k] * c->Pin[3 * i + k];
+    }
+  if (norm_res) *norm_res = std::sqrt(rr);
+  if (norm_reac) *norm_reac = std::sqrt(c->reac_sumsq);
+}
+
+// ---------------------------------------------------------------- tangent
+
+typedef double Mat3[3][3];
+
+// Particle::hasAFEMneighbor (particle.h:242-256)
+bool has_afem_neighbor(const Ctx* c, int i, int j, int L) {
+  for (int b1 = c->lb(i, L); b1 < c->le(i, L); ++b1) {
+    const int k = c->p2[b1];
+    if (k == j) return true;
+    for (int b2 = c->lb(k, L); b2 < c->le(k, L); ++b2)
+      if (c->p2[b2] == j) return true;
+  }
+  return false;
+}
+
+// fdu2dxyz: the diagonal block (derivative.h:7-45)
+void fdu2dxyz(const Ctx* c, int i, Mat3 du) {
+  for (int r = 0; r < 3; ++r)
+    for (int s = 0; s < 3; ++s) du[r][s] = 0;
+  for (int L = 0; L < 2; ++L) {
+    const double* csum = &c->cs_sum[(2 * i + L) * 3];
+    for (int b = c->lb(i, L); b < c->le(i, L); ++b) {
+      const double* cs = &c->cs[3 * b];
+      const double kn = c->Kn[2 * i + L], tv = c->Tv[i], om = 1 - c->bdamage[b];
+      for (int r = 0; r < 3; ++r)
+        for (int s = r; s < 3; ++s) {
+          const double geo = (r == s) ? (kn + tv) * (cs[r] * cs[r]) : (kn + tv) * cs[s] * cs[r];
+          du[r][s] += om * ((kn * cs[s] + tv * csum[s]) * cs[r] + geo);
+        }
+    }
+  }
+  // du2dxyz[1][0] = du2dxyz[1][0] (derivative.h:38) leaves it 0
+  du[2][0] = du[0][2];
+  du[2][1] = du[1][2];
+}
+
+// fdu2dxyz1 (layer 0, derivative.h:47-142) and fdu2dxyz2 (layer 1,
+// derivative.h:144-239): the two reference functions are the same text with the
+// layer index changed, so one function restates both.
+void fdu2dxyz_layer(const Ctx* c, int i, int j, int L, Mat3 du) {
+  for (int r = 0; r < 3; ++r)
+    for (int s = 0; s < 3; ++s) du[r][s] = 0;
+  const double tv_i = c->Tv[i];
+  for (int b1 = c->lb(i, L); b1 < c->le(i, L); ++b1) {
+    if (c->p2[b1] != j) continue;
+    const double* cs1 = &c->cs[3 * b1];
+    const double om1 = 1 - c->bdamage[b1];
+    for (int b2 = c->lb(i, L); b2 < c->le(i, L); ++b2) {
+      const int k = c->p2[b2];
+      const double* cs2 = &c->cs[3 * b2];
+      const double om2 = 1 - c->bdamage[b2];
+      if (c->p2[b1] == k) {
+        const double kn = c->Kn[2 * i + L];
+        const double* csum = &c->cs_sum[(2 * k + L) * 3];
+        for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s) {
+            const double geo = (r == s) ? (kn + tv_i) * (cs2[r] * cs2[r])
+                                        : (kn + tv_i) * cs2[std::max(r, s)] * cs2[std::min(r, s)];
+            du[r][s] += om2 * ((-kn * cs2[s] + tv_i * csum[s]) * cs2[r] - geo);
+          }
+      } else {
+        const double coef = 0.5 * (om1 * tv_i + om2 * tv_i);
+        for (int r = 0; r < 3; ++r)
+          for (int s = 0; s < 3; ++s) du[r][s] -= coef * cs2[r] * cs1[s];
+        const double tv_k = c->Tv[k];
+        for (int b3 = c->lb(k, L); b3 < c->le(k, L); ++b3) {
+          if (c->p2[b3] != j) continue;
+          const double* cs3 = &c->cs[3 * b3];
+          const double coef3 = 0.5 * ((1 - c->bdamage[b3]) * tv_k + om2 * tv_i);
+          for (int r = 0; r < 3; ++r)
+            for (int s = 0; s < 3; ++s) du[r][s] -= coef3 * cs2[r] * cs3[s];
+        }
+      }
+    }
+    return;
+  }
+  for (int b1 = c->lb(i, L); b1 < c->le(i, L); ++b1) {
+    const int k = c->p2[b1];
+    const double* cs1 = &c->cs[3 * b1];
+    const double om1 = 1 - c->bdamage[b1];
+    const double tv_k = c->Tv[k];
+    for (int b2 = c->lb(k, L); b2 < c->le(k, L); ++b2) {
+      if (c->p2[b2] != j) continue;
+      const double* cs2 = &c->cs[3 * b2];
+      const double coef = 0.5 * ((1 - c->bdamage[b2]) * tv_k + om1 * tv_i);
+      for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) du[r][s] -= coef * cs1[r] * cs2[s];
+    }
+  }
+}
+
+// Stiffness::localStiffnessANA (stiffness.h:108-142)
+void local_stiffness(const Ctx* c, int i, int j, Mat3 K) {
+  Mat3 T1, T2;
+  bool f1 = false, f2 = false;
+  for (int r = 0; r < 3; ++r)
+    for (int s = 0; s < 3; ++s) K[r][s] = T1[r][s] = T2[r][s] = 0;
+  if (has_afem_neighbor(c, i, j, 0)) { fdu2dxyz_layer(c, i, j, 0, T1); f1 = true; }
+  if (has_afem_neighbor(c, i, j, 1)) { fdu2dxyz_layer(c, i, j, 1, T2); f2 = true; }
+  if (i == j) {
+    fdu2dxyz(c, i, K);
+  } else if (f1 && f2) {
+    for (int r = 0; r < 3; ++r)
+      for (int s = 0; s < 3; ++s) K[r][s] = T1[r][s] + T2[r][s];
+  } else if (f1) {
+    std::memcpy(K, T1, sizeof(Mat3));
+  } else if (f2) {
+    std::memcpy(K, T2, sizeof(Mat3));
+  }
+}
+
+// Offset of row r inside a particle's row group: row r holds the entries
+// (r, s>=r) of the diagonal block followed by d entries per greater conn.
+inline int64_t row_off(int r, int d, int ncl) { return (int64_t)r * d * ncl - r * (r + 1) / 2; }
+
+// Stiffness::reset + calcStiffness3D/2D (stiffness.h:73-89, 197-361).  The
+// reference scatters with += from an OpenMP loop; every entry receives at most
+// two terms (stiffness.h:241-249 and 267-275), so the single-thread result is
+// order independent.  The blocks are computed in parallel and scattered in the
+// reference's loop order.
+void calc_stiffness(Ctx* c) {
+  const int N = c->N, d = c->dim;
+  std::fill(c->JK.begin(), c->JK.end(), 0);
+  std::fill(c->K.begin(), c->K.end(), 0.0);
+  std::vector<double> blk((size_t)c->cptr[N] * 9);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < N; ++i)
+    for (int e = c->cptr[i]; e < c->cptr[i + 1]; ++e) {
+      Mat3 Kl;
+      local_stiffness(c, i, c->conns[e], Kl);
+      std::memcpy(&blk[(size_t)e * 9], Kl, sizeof(Mat3));
+    }
+  for (int i = 0; i < N; ++i) {
+    const int nconn = c->cptr[i + 1] - c->cptr[i];
+    const int64_t Kp = c->Kp[i];
+    const int ncl = c->ncl[i];
+    for (int idx_j = 0; idx_j < nconn; ++idx_j) {
+      const int e = c->cptr[i] + idx_j, j = c->conns[e];
+      const double* Kl = &blk[(size_t)e * 9];
+      if (i == j) {
+        for (int r = 0; r < d; ++r)
+          for (int s = r; s < d; ++s) {
+            c->K[Kp + row_off(r, d, ncl) + s] += Kl[3 * r + s];
+            c->JK[Kp + row_off(r, d, ncl) + s] = (int64_t)d * (j + 1) - (d - 1) + s;
+          }
+      } else if (j > i) {
+        const int num1 = ncl - (nconn - idx_j);
+        for (int r = 0; r < d; ++r)
+          for (int s = 0; s < d; ++s) {
+            c->K[Kp + row_off(r, d, ncl) + (int64_t)d * num1 + s] += 0.5 * Kl[3 * r + s];
+            c->JK[Kp + row_off(r, d, ncl) + (int64_t)d * num1 + s] = (int64_t)d * (j + 1) - (d - 1) + s;
+          }
+      } else {
+        const int* cj = &c->conns[c->cptr[j]];
+        const int nconn_j = c->cptr[j + 1] - c->cptr[j];
+        const int pos = (int)(std::find(cj, cj + nconn_j, i) - cj);
+        const int num2 = c->ncl[j] - (nconn_j - pos);
+        for (int r = 0; r < d; ++r)
+          for (int s = 0; s < d; ++s)
+            c->K[c->Kp[j] + row_off(r, d, c->ncl[j]) + (int64_t)d * num2 + s] += 0.5 * Kl[3 * s + r];
+      }
+    }
+    for (int r = 0; r < d; ++r) c->IK[(size_t)d * i + r] = Kp + row_off(r, d, ncl) + r + 1;
+  }
+  c->IK[(size_t)d * N] = c->Kp[N] + 1;
+}
+
+double dnrm2(const double* x, int64_t n) {
+  double s = 0;
+  for (int64_t t = 0; t < n; ++t) s += x[t] * x[t];
+  return std::sqrt(s);
+}
+
+// Stiffness::updateStiffnessDispBC (stiffness.h:363-438)
+void update_stiffness_disp_bc(Ctx* c) {
+  const int N = c->N, d = c->dim;
+  std::vector<double> diag((size_t)d * N);
+  for (int i = 0; i < N; ++i) {
+    const int64_t Kp = c->Kp[i];
+    diag[d * i] = c->K[Kp];
+    diag[d * i + 1] = c->K[Kp + (int64_t)d * c->ncl[i]];
+    if (d == 3) diag[d * i + 2] = c->K[Kp + 2 * (int64_t)d * c->ncl[i] - 1];
+  }
+  const double norm_diag = dnrm2(diag.data(), (int64_t)d * N);
+  for (int i = 0; i < N; ++i) {
+    const int64_t Kp = c->Kp[i];
+    const int64_t ncl = c->ncl[i];
+    for (int k = 0; k < d; ++k) {
+      if (!(c->cons[3 * i + k] == 1 || std::fabs(c->damage_visual[i] - 1.0) < kEPS)) continue;
+      for (int e = c->cptr[i]; e < c->cptr[i + 1]; ++e) {
+        const int j = c->conns[e];
+        if (j > i) continue;
+        if (j == i) {
+          if (k == 0) {
+            const int64_t start = (d == 2) ? Kp : Kp + 1;
+            for (int64_t kk = start; kk <= Kp + d * ncl - 1; ++kk) c->K[kk] = 0.0;
+            c->K[Kp] = norm_diag;
+          } else if (k == 1) {
+            c->K[Kp + 1] = 0.0;
+            const int64_t start = (d == 2) ? (Kp + d * ncl) : (Kp + d * ncl + 1);
+            const int64_t end = (d == 2) ? (c->Kp[i + 1] - 1) : Kp + 2 * d * ncl - 2;
+            for (int64_t kk = start; kk <= end; ++kk) c->K[kk] = 0.0;
+            c->K[Kp + d * ncl] = norm_diag;
+          } else {
+            c->K[Kp + 2] = 0.0;
+            c->K[Kp + d * ncl + 1] = 0.0;
+            for (int64_t kk = Kp + 2 * d * ncl - 1; kk <= c->Kp[i + 1] - 1; ++kk) c->K[kk] = 0.0;
+            c->K[Kp + 2 * d * ncl - 1] = norm_diag;
+          }
+        } else {
+          const int* cj = &c->conns[c->cptr[j]];
+          const int nconn_j = c->cptr[j + 1] - c->cptr[j];
+          const int pos = (int)(std::find(cj, cj + nconn_j, i) - cj);
+          const int64_t num2 = c->ncl[j] - (nconn_j - pos);
+          const int64_t Kpj = c->Kp[j], nclj = c->ncl[j];
+          c->K[Kpj + d * num2 + k] = 0;
+          c->K[Kpj + d * nclj + d * num2 - 1 + k] = 0;
+          if (d == 3) c->K[Kpj + 2 * d * nclj + d * num2 - 3 + k] = 0;
+        }
+      }
+      c->residual[(size_t)d * i + k] = 0.0;
+    }
+  }
+}
+
+// Expand the symmetric upper CSR to a full 0-based CSR once per assembly so the
+// SpMV can run row-parallel (the reference hands the upper CSR to
+// mkl_sparse_d_mv with SPARSE_MATRIX_TYPE_SYMMETRIC, solver.h:287-296).
+void build_full(Ctx* c) {
+  const int64_t n = (int64_t)c->dim * c->N;
+  std::vector<int64_t> cnt(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t t = c->IK[r] - 1; t < c->IK[r + 1] - 1; ++t) {
+      const int64_t col = c->JK[t] - 1;
+      cnt[r + 1]++;
+      if (col != r) cnt[col + 1]++;
+    }
+  for (int64_t r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+  c->fptr = cnt;
+  c->fcol.assign(cnt[n], 0);
+  c->fval.assign(cnt[n], 0);
+  std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t t = c->IK[r] - 1; t < c->IK[r + 1] - 1; ++t) {
+      const int64_t col = c->JK[t] - 1;
+      const double v = c->K[t];
+      c->fcol[fill[r]] = (int)col; c->fval[fill[r]++] = v;
+      if (col != r) { c->fcol[fill[col]] = (int)r; c->fval[fill[col]++] = v; }
+    }
+}
+
+void spmv_full(const Ctx* c, const double* x, double* y) {
+  const int64_t n = (int64_t)c->dim * c->N;
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) {
+    double s = 0;
+    for (int64_t t = c->fptr[r]; t < c->fptr[r + 1]; ++t) s += c->fval[t] * x[c->fcol[t]];
+    y[r] = s;
+  }
+}
+
+double pdot(const double* a, const double* b, int64_t n) {
+  double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int64_t t = 0; t < n; ++t) s += a[t] * b[t];
+  return s;
+}
+
+// Solver::LPM_CG (solver.h:276-354): MKL RCI CG restated from its published
+// algorithm (plain Hestenes-Stiefel CG from x0 = 0; residual stopping test on
+// squared norms, ||r||^2 <= rtol*||r0||^2 + atol; at most maxit iterations).
+// precond = 1 adds a Jacobi preconditioner (not used by the reference).
+int cg(Ctx* c, double* x, const double* b, int precond, double rtol, double atol, int64_t maxit,
+       double* out_rr) {
+  const int64_t n = (int64_t)c->dim * c->N;
+  std::vector<double> r(b, b + n), p(n), q(n), z(n), dinv(n, 1.0);
+  if (precond) {
+    for (int64_t row = 0; row < n; ++row)
+      for (int64_t t = c->fptr[row]; t < c->fptr[row + 1]; ++t)
+        if (c->fcol[t] == row) dinv[row] = 1.0 / c->fval[t];
+  }
+  for (int64_t t = 0; t < n; ++t) x[t] = 0;
+  for (int64_t t = 0; t < n; ++t) z[t] = dinv[t] * r[t];
+  p = z;
+  double rr = pdot(r.data(), r.data(), n), rz = pdot(r.data(), z.data(), n);
+  const double thresh = rtol * rr + atol;
+  int64_t it = 0;
+  while (!(rr <= thresh) && it < maxit) {
+    spmv_full(c, p.data(), q.data());
+    const double alpha = rz / pdot(p.data(), q.data(), n);
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < n; ++t) {
+      x[t] += alpha * p[t];
+      r[t] -= alpha * q[t];
+      z[t] = dinv[t] * r[t];
+    }
+    const double rz_new = pdot(r.data(), z.data(), n);
+    rr = pdot(r.data(), r.data(), n);
+    const double beta = rz_new / rz;
+    rz = rz_new;
+#pragma omp parallel for schedule(static)
+    for (int64_t t = 0; t < n; ++t) p[t] = z[t] + beta * p[t];
+    ++it;
+  }
+  if (out_rr) *out_rr = rr;
+  return (int)it;
+}
+
+// ---------------------------------------------------------------- state
+
+// ParticleElasticDamage::calcStrainEnergyDensity (particle_elastic_damage.h:89-104)
+double strain_energy(const Ctx* c, int i) {
+  const double V_m = c->cell.volume * c->nb(i) / c->cell.nneighbors;
+  double e = 0;
+  for (int L = 0; L < 2; ++L) {
+    for (int b = c->lb(i, L); b < c->le(i, L); ++b)
+      e += 0.5 * (c->Kn[2 * i + L] * c->dLe[b] * c->dLe[b]) / V_m;
+    e += 0.5 * c->TdLe_tot[2 * i + L] * c->dLe_tot[2 * i + L] / V_m;
+  }
+  return e;
+}
+
+// Assembly::updateStateVar (assembly.h:161-166) -> updateParticleStateVariables
+// of ParticleElasticDamage (particle_elastic_damage.h:150-161) and
+// ParticleJ2Plasticity (particle_j2plasticity.h:78-140); the elastic particle
+// has none (particle.h:62).
+void update_state_var(Ctx* c) {
+  const int N = c->N;
+  if (c->ptype == kElasticDamage) {
+    for (int i = 0; i < N; ++i) {
+      c->sv[7 * i] = strain_energy(c, i);
+      const double delta = c->sv[7 * i] - c->sv_last[7 * i];
+      const double f = c->sv[7 * i] - c->k1[i] * c->damage[i] - c->k0[i];
+      c->Ddot_local[i] = (f > 0 && delta > 0) ? (1 / c->k1[i]) * delta : 0;
+    }
+  } else if (c->ptype == kJ2) {
+    for (int i = 0; i < N; ++i) {
+      double* sv = &c->sv[7 * i];
+      double st[6];
+      for (int k = 0; k < 6; ++k) st[k] = c->stress[6 * i + k];
+      double dpl[6] = {0, 0, 0, 0, 0, 0};
+      double sigma_eq = 0, dlambda = 0, triax = 0;
+      const double sigma_m = 1.0 / 3.0 * (st[0] + st[1] + st[2]);
+      for (int k = 0; k < 3; ++k) st[k] -= sigma_m;
+      for (int k = 0; k < 6; ++k) st[k] -= sv[k + 1];
+      for (int k = 0; k < 6; ++k) sigma_eq += (k < 3) ? st[k] * st[k] : 2.0 * st[k] * st[k];
+      sigma_eq = std::sqrt(3.0 / 2.0 * sigma_eq);
+      const double yield = sigma_eq - (c->mSy[i] + (1.0 - c->mXi[i]) * c->mH[i] * sv[0]);
+      const double G = c->mE[i] / 2 / (1 + c->mMu[i]);
+      if (yield > 0.0) dlambda = yield / (3 * G + c->mH[i]);
+      sv[0] += dlambda;
+      for (int k = 0; k < 6; ++k)
+        if (std::fabs(sigma_eq) > kEPS) {
+          dpl[k] = dlambda * 1.5 * st[k] / sigma_eq;
+          sv[k + 1] += 2. / 3. * c->mXi[i] * c->mH[i] * dpl[k];
+        }
+      for (int b = c->bptr[i]; b < c->bptr[i + 1]; ++b) {
+        const double cx = c->cs[3 * b], cy = c->cs[3 * b + 1], cz = c->cs[3 * b + 2];
+        const double ddLp = c->dis[b] * (dpl[0] * cx * cx + dpl[1] * cy * cy + dpl[2] * cz * cz +
+                                         2 * dpl[3] * cy * cz + 2 * dpl[4] * cx * cz +
+                                         2 * dpl[5] * cx * cy);
+        c->dLp[b] += ddLp;
+      }
+      if (sigma_eq > kEPS) triax = sigma_m / sigma_eq;
+      c->Ddot_local[i] = dlambda * (1.0 + c->mA[i] * triax);
+    }
+  }
+}
+
+// func_phi (lpm.h:49-53)
+inline double func_phi(double x, double L) {
+  const double t = 1 / (1 + std::pow(x / L, 8.0));
+  return t * t;
+}
+
+// SolverStatic::updateStaticDamage (solver_static.h:31-62) with
+// updateParticleStaticDamage of ParticleElastic (particle_elastic.h:52-68),
+// ParticleElasticDamage (particle_elastic_damage.h:163-173) and
+// ParticleJ2Plasticity (particle_j2plasticity.h:142-152).
+bool update_static_damage(Ctx* c, int undamaged) {
+  const int N = c->N;
+  for (int i = 0; i < N; ++i) {
+    if (c->type[i] == undamaged) continue;
+    double A = 0, Dn = 0;
+    for (int t = c->nlptr[i]; t < c->nlptr[i + 1]; ++t) {
+      const int j = c->nlidx[t];
+      if (c->type[j] == undamaged) continue;
+      const double dis = dist(*c, i, j);
+      const double V_m = c->cell.volume * c->nb(j) / c->cell.nneighbors;
+      Dn += c->Ddot_local[j] * func_phi(dis, c->nlL[i]) * V_m;
+      A += func_phi(dis, c->nlL[i]) * V_m;
+    }
+    c->Ddot_nonlocal[i] = Dn / A;
+  }
+  bool any = false;
+  for (int i = 0; i < N; ++i) {
+    if (c->type[i] == undamaged) continue;
+    bool mine = false;
+    if (c->ptype == kElastic) {
+      for (int b = c->bptr[i]; b < c->bptr[i + 1]; ++b)
+        if (c->bstrain[b] >= c->crit[i] && std::fabs(c->bdamage[b] - 1.0) > kEPS) {
+          mine = true;
+          c->bdamage[b] = 1;
+        }
+    } else {
+      c->damage[i] += c->Ddot_nonlocal[i];
+      if (c->damage[i] >= c->dthr[i]) c->damage[i] = 1;
+      mine = std::fabs(c->damage[i] - c->damage_last[i]) > 1e-3;
+    }
+    any = mine || any;
+  }
+  return any;
+}
+
+// Assembly::storeStateVar / resetStateVar (assembly.h:140-159) with
+// Particle::store/resetParticleStateVariables (particle.h:108-148)
+void store_state(Ctx* c) {
+  c->xyz_last = c->xyz;
+  c->damage_last = c->damage;
+  c->sv_last = c->sv;
+  c->dis_last = c->dis;
+  c->bdamage_last = c->bdamage;
+  c->bforce_last = c->bforce;
+  c->dLp_last = c->dLp;
+}
+void reset_state(Ctx* c, int reset_xyz) {
+  if (reset_xyz) c->xyz = c->xyz_last;
+  c->damage = c->damage_last;
+  c->sv = c->sv_last;
+  c->dis = c->dis_last;
+  c->bdamage = c->bdamage_last;
+  c->bforce = c->bforce_last;
+  c->dLp = c->dLp_last;
+}
+
+// Assembly::searchNonlocalNeighbors (assembly.h:117-130): every particle p2
+// (self included) with distance < cutoff_ratio * nonlocal_L of p1, in id order.
+void search_nonlocal(Ctx* c, double ratio) {
+  const int N = c->N;
+  std::vector<std::vector<int>> nl(N);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j)
+      if (dist(*c, i, j) < ratio * c->nlL[i]) nl[i].push_back(j);
+  c->nlptr.assign(N + 1, 0);
+  for (int i = 0; i < N; ++i) c->nlptr[i + 1] = c->nlptr[i] + (int)nl[i].size();
+  c->nlidx.resize(c->nlptr[N]);
+  for (int i = 0; i < N; ++i) std::copy(nl[i].begin(), nl[i].end(), c->nlidx.begin() + c->nlptr[i]);
+}
+
+// Particle*::setParticleProperty KnTv mapping (particle_elastic.h:70-140 and the
+// identical blocks in particle_elastic_damage.h:175-250, particle_j2plasticity.h:154-198)
+void kntv_from_ce(const Cell& cell, const double Ce[3], double out[3]) {
+  const double alpha = (cell.dim == 2) ? 1.0 : cell.radius;
+  for (int r = 0; r < 3; ++r) {
+    double s = 0;
+    for (int l = 0; l < 3; ++l) s += (alpha * Ce[l]) * cell.el[3 * r + l];
+    out[r] = s;
+  }
+}
+void ce_isotropic(int plane_stress, double E, double mu, double Ce[3]) {
+  if (!plane_stress) {
+    Ce[0] = E * (1.0 - mu) / (1.0 + mu) / (1.0 - 2.0 * mu);
+    Ce[1] = E * mu / (1.0 + mu) / (1.0 - 2.0 * mu);
+    Ce[2] = E / 2.0 / (1.0 + mu);
+  } else {
+    Ce[0] = E / (1.0 - mu) / (1.0 + mu);
+    Ce[1] = E * mu / (1.0 + mu) / (1.0 - mu);
+    Ce[2] = E / 2.0 / (1.0 + mu);
+  }
+}
+void assign_kntv(Ctx* c, int i, const double KnTv[3]) {
+  if (c->cell.lattice == kHexagon2D) {
+    c->Kn[2 * i] = KnTv[0]; c->Kn[2 * i + 1] = KnTv[0]; c->Tv[i] = KnTv[1];
+  } else {
+    c->Kn[2 * i] = KnTv[0]; c->Kn[2 * i + 1] = KnTv[1]; c->Tv[i] = KnTv[2];
+  }
+}
+
+bool finish_build(Ctx* c) {
+  if (!build_rev(c)) return false;
+  if (!update_connections(c)) return false;
+  alloc_bond_state(c);
+  init_bond_cs(c);
+  stiffness_initialize(c);
+  return true;
+}
+
+enum Field {
+  F_XYZ = 0, F_XYZ_INITIAL, F_XYZ_LAST, F_PIN, F_PEX, F_STRESS, F_DAMAGE, F_DAMAGE_VISUAL,
+  F_STATE_VAR, F_RESIDUAL, F_BDAMAGE, F_DLP, F_BFORCE, F_DIS, F_CS, F_CONSTRAINT,
+  F_DDOT_LOCAL, F_DDOT_NONLOCAL, F_DLE_TOTAL, F_CS_SUM, F_BSTRAIN, F_DLE, F_DAMAGE_LAST,
+  F_TYPE, F_TDLE_TOTAL, F_NFIELDS
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* lpmo_last_error(void) { return g_err.c_str(); }
+
+void lpmo_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
+int lpmo_unit_cell(int lattice, double radius, double* out /*16*/) {
+  Cell c;
+  if (!make_cell(lattice, radius, &c)) { g_err = "unknown lattice"; return -1; }
+  out[0] = c.dim; out[1] = c.nneighbors; out[2] = c.volume; out[3] = c.cut[0]; out[4] = c.cut[1];
+  for (int k = 0; k < 9; ++k) out[5 + k] = c.el[k];
+  return 0;
+}
+
+// Assembly(p_xyz, box, cell, ptype) (assembly.h:97-106) + Stiffness::initialize
+void* lpmo_create(int lattice, double radius, int ptype, int n, const double* xyz, int brute) {
+  Ctx* c = new Ctx;
+  if (!make_cell(lattice, radius, &c->cell) || n <= 0 || ptype < 0 || ptype > 2) {
+    g_err = "bad lattice, particle count or particle type";
+    delete c;
+    return nullptr;
+  }
+  c->dim = c->cell.dim; c->ptype = ptype; c->N = n;
+  c->xyz.assign(xyz, xyz + 3 * (size_t)n);
+  alloc_particles(c);
+  create_bonds(c, brute);
+  if (!finish_build(c)) { delete c; return nullptr; }
+  return c;
+}
+
+// Assembly(dump, bond, cell, ptype) (assembly.h:83-95) with readBond (assembly.h:390-424)
+void* lpmo_create_with_bonds(int lattice, double radius, int ptype, int n, const double* xyz,
+                             int64_t nbond, const int32_t* p1, const int32_t* p2,
+                             const int32_t* layer) {
+  Ctx* c = new Ctx;
+  if (!make_cell(lattice, radius, &c->cell) || n <= 0 || ptype < 0 || ptype > 2) {
+    g_err = "bad lattice, particle count or particle type";
+    delete c;
+    return nullptr;
+  }
+  c->dim = c->cell.dim; c->ptype = ptype; c->N = n;
+  c->xyz.assign(xyz, xyz + 3 * (size_t)n);
+  alloc_particles(c);
+  if (!set_bonds(c, nbond, p1, p2, layer) || !finish_build(c)) { delete c; return nullptr; }
+  return c;
+}
+
+void lpmo_destroy(void* h) { delete static_cast<Ctx*>(h); }
+
+// sizes: N, dim, NB, nnz(upper CSR), nnzb(sum nconn), max nb, max nconn, ptype
+int lpmo_sizes(void* h, int64_t* out) {
+  Ctx* c = static_cast<Ctx*>(h);
+  int mb = 0, mc = 0;
+  for (int i = 0; i < c->N; ++i) {
+    mb = std::max(mb, c->nb(i));
+    mc = std::max(mc, c->cptr[i + 1] - c->cptr[i]);
+  }
+  out[0] = c->N; out[1] = c->dim; out[2] = (int64_t)c->p2.size(); out[3] = c->Kp[c->N];
+  out[4] = c->cptr[c->N]; out[5] = mb; out[6] = mc; out[7] = c->ptype;
+  return 0;
+}
+
+int lpmo_export_bonds(void* h, int32_t* bptr, int32_t* nb0, int32_t* p2) {
+  Ctx* c = static_cast<Ctx*>(h);
+  std::copy(c->bptr.begin(), c->bptr.end(), bptr);
+  std::copy(c->nb0.begin(), c->nb0.end(), nb0);
+  std::copy(c->p2.begin(), c->p2.end(), p2);
+  return 0;
+}
+
+int lpmo_export_conns(void* h, int32_t* cptr, int32_t* conns, int32_t* ncl) {
+  Ctx* c = static_cast<Ctx*>(h);
+  std::copy(c->cptr.begin(), c->cptr.end(), cptr);
+  std::copy(c->conns.begin(), c->conns.end(), conns);
+  std::copy(c->ncl.begin(), c->ncl.end(), ncl);
+  return 0;
+}
+
+int lpmo_export_csr(void* h, int64_t* Kp, int64_t* IK, int64_t* JK, double* K) {
+  Ctx* c = static_cast<Ctx*>(h);
+  if (Kp) std::copy(c->Kp.begin(), c->Kp.end(), Kp);
+  if (IK) std::copy(c->IK.begin(), c->IK.end(), IK);
+  if (JK) std::copy(c->JK.begin(), c->JK.end(), JK);
+  if (K) std::copy(c->K.begin(), c->K.end(), K);
+  return 0;
+}
+
+int lpmo_set_types(void* h, const int32_t* type) {
+  Ctx* c = static_cast<Ctx*>(h);
+  for (int i = 0; i < c->N; ++i) c->type[i] = type[i];
+  return 0;
+}
+
+// model: 0 elastic (E,nu), 1 elastic cubic (C11,C12,C44), 2 elastic damage (E,nu),
+// 3 elastic damage cubic, 4 J2 (E,nu).  params (up to 10):
+//  0: nonlocal_L, plane_stress, E, mu, crit
+//  1: nonlocal_L, C11, C12, C44, crit
+//  2: nonlocal_L, plane_stress, E, mu, k0, k1, ct_ratio, d_thres
+//  3: nonlocal_L, C11, C12, C44, k0, k1, ct_ratio, d_thres
+//  4: nonlocal_L, plane_stress, E, mu, sigmay, xi, H, A, crit
+int lpmo_set_material(void* h, int n_idx, const int32_t* idx, int model, const double* p) {
+  Ctx* c = static_cast<Ctx*>(h);
+  double Ce[3], KnTv[3];
+  switch (model) {
+    case 0: case 2: case 4: ce_isotropic((int)p[1], p[2], p[3], Ce); break;
+    case 1: case 3: Ce[0] = p[1]; Ce[1] = p[2]; Ce[2] = p[3]; break;
+    default: g_err = "unknown material model"; return -1;
+  }
+  kntv_from_ce(c->cell, Ce, KnTv);
+  const int n = idx ? n_idx : c->N;
+  for (int t = 0; t < n; ++t) {
+    const int i = idx ? idx[t] : t;
+    if (i < 0 || i >= c->N) { g_err = "particle index out of range"; return -1; }
+    assign_kntv(c, i, KnTv);
+    c->nlL[i] = p[0];
+    if (model == 0) c->crit[i] = p[4];
+    if (model == 1) c->crit[i] = p[4];
+    if (model == 2 || model == 3) {
+      c->k0[i] = p[4]; c->k1[i] = p[5]; c->ctr[i] = p[6]; c->dthr[i] = p[7];
+    }
+    if (model == 4) {
+      c->mE[i] = p[2]; c->mMu[i] = p[3]; c->mSy[i] = p[4]; c->mXi[i] = p[5];
+      c->mH[i] = p[6]; c->mA[i] = p[7]; c->crit[i] = p[8];
+    }
+  }
+  return 0;
+}
+
+int lpmo_update_geometry(void* h) { update_geometry(static_cast<Ctx*>(h)); return 0; }
+int lpmo_update_force_state(void* h) { update_force_state(static_cast<Ctx*>(h)); return 0; }
+int lpmo_forward(void* h) {
+  Ctx* c = static_cast<Ctx*>(h);
+  update_geometry(c);
+  update_force_state(c);
+  return 0;
+}
+int lpmo_update_rr(void* h, double* norms) {
+  update_rr(static_cast<Ctx*>(h), &norms[0], &norms[1]);
+  return 0;
+}
+int lpmo_assemble(void* h) {
+  Ctx* c = static_cast<Ctx*>(h);
+  calc_stiffness(c);
+  update_stiffness_disp_bc(c);
+  build_full(c);
+  return 0;
+}
+// K (full symmetric, from the upper CSR) times x
+int lpmo_spmv(void* h, const double* x, double* y) {
+  spmv_full(static_cast<Ctx*>(h), x, y);
+  return 0;
+}
+
+// opts: [precond, rtol, atol, maxit(<=0 -> n)]; stats: [iters, rr_final]
+// Solver::solveLinearSystem (solver.h:92-109): solve K du = residual, then moveBy.
+int lpmo_solve_linear(void* h, const double* opts, double* stats) {
+  Ctx* c = static_cast<Ctx*>(h);
+  const int64_t n = (int64_t)c->dim * c->N;
+  std::vector<double> du(n);
+  int64_t maxit = opts[3] > 0 ? (int64_t)opts[3] : n;
+  double rr = 0;
+  const int it = cg(c, du.data(), c->residual.data(), (int)opts[0], opts[1], opts[2], maxit, &rr);
+  for (int i = 0; i < c->N; ++i)
+    for (int k = 0; k < c->dim; ++k) c->xyz[3 * i + k] += du[c->dim * i + k];
+  if (stats) { stats[0] = it; stats[1] = rr; }
+  return 0;
+}
+
+// Solver::NewtonIteration (solver.h:54-90).  Returns the iteration count
+// (max_iter on the abnormal return).  stats: [total cg iters, final |r|, tol multiplier]
+int lpmo_newton(void* h, int max_iter, double tol, const double* opts, double* stats) {
+  Ctx* c = static_cast<Ctx*>(h);
+  update_geometry(c);
+  update_force_state(c);
+  double nr, nreac;
+  update_rr(c, &nr, &nreac);
+  const double mult = std::max(nr, nreac);
+  int ni = 0;
+  double cgit = 0, st[2];
+  while (nr > tol * mult) {
+    if (++ni > max_iter) { ni = max_iter; break; }
+    lpmo_solve_linear(h, opts, st);
+    cgit += st[0];
+    update_geometry(c);
+    update_force_state(c);
+    double dummy;
+    update_rr(c, &nr, &dummy);
+  }
+  if (stats) { stats[0] = cgit; stats[1] = nr; stats[2] = mult; }
+  return ni;
+}
+
+int lpmo_update_state_var(void* h) { update_state_var(static_cast<Ctx*>(h)); return 0; }
+int lpmo_update_static_damage(void* h, int undamaged) {
+  return update_static_damage(static_cast<Ctx*>(h), undamaged) ? 1 : 0;
+}
+int lpmo_store_state(void* h) { store_state(static_cast<Ctx*>(h)); return 0; }
+int lpmo_reset_state(void* h, int reset_xyz) { reset_state(static_cast<Ctx*>(h), reset_xyz); return 0; }
+int lpmo_search_nonlocal(void* h, double ratio) { search_nonlocal(static_cast<Ctx*>(h), ratio); return 0; }
+
+int lpmo_define_group(void* h, int n, const int32_t* idx) {
+  Ctx* c = static_cast<Ctx*>(h);
+  Group g;
+  g.idx.assign(idx, idx + n);
+  for (int i : g.idx)
+    if (i < 0 || i >= c->N) { g_err = "group index out of range"; return -1; }
+  c->groups.push_back(g);
+  return (int)c->groups.size() - 1;
+}
+
+// Solver::updateDisplacementBC for one DispBC (solver.h:128-164).  flag 'x','y','z';
+// mode 0 Relative, 1 Absolute (moveTo the vector {step,0,0} etc., as the reference).
+int lpmo_disp_bc(void* h, int gid, int mode, int flag, double step) {
+  Ctx* c = static_cast<Ctx*>(h);
+  if (gid < 0 || gid >= (int)c->groups.size()) { g_err = "bad group"; return -1; }
+  const int k = flag - 'x';
+  if (k < 0 || k > 2) { g_err = "bad flag"; return -1; }
+  for (int i : c->groups[gid].idx) {
+    double d[3] = {0, 0, 0};
+    d[k] = step;
+    if (mode == 0)
+      for (int q = 0; q < 3; ++q) c->xyz[3 * i + q] = c->xyz[3 * i + q] + d[q];
+    else
+      for (int q = 0; q < 3; ++q) c->xyz[3 * i + q] = d[q];
+    c->cons[3 * i + k] = 1;
+  }
+  return 0;
+}
+
+// Solver::updateForceBC for one ForceBC (solver.h:166-188)
+int lpmo_force_bc(void* h, int gid, int mode, double fx, double fy, double fz) {
+  Ctx* c = static_cast<Ctx*>(h);
+  if (gid < 0 || gid >= (int)c->groups.size()) { g_err = "bad group"; return -1; }
+  const int num = (int)c->groups[gid].idx.size();
+  for (int i : c->groups[gid].idx) {
+    if (mode == 0) {
+      c->Pex[3 * i] += fx / num; c->Pex[3 * i + 1] += fy / num; c->Pex[3 * i + 2] += fz / num;
+    } else {
+      c->Pex[3 * i] = fx / num; c->Pex[3 * i + 1] = fy / num; c->Pex[3 * i + 2] = fz / num;
+    }
+  }
+  return 0;
+}
+
+// SolverStatic::solveProblemStep (solver_static.h:97-152).  The load step's BCs
+// are applied by the caller (force BCs first, then displacement BCs) before this
+// call.  Returns 1 when converged.  stats: [total cg iters, damage passes,
+// newton iters of the last pass]
+int lpmo_solve_step(void* h, int undamaged, int max_iter, double tol, const double* opts,
+                    double* stats) {
+  Ctx* c = static_cast<Ctx*>(h);
+  int n_newton = 0, passes = 0;
+  double cgit = 0;
+  bool new_damaged = false;
+  do {
+    lpmo_assemble(h);
+    double st[3];
+    n_newton = lpmo_newton(h, max_iter, tol, opts, st);
+    cgit += st[0];
+    ++passes;
+    if (n_newton >= max_iter) break;
+    update_state_var(c);
+    new_damaged = update_static_damage(c, undamaged);
+    store_state(c);
+    if (new_damaged) {
+      update_geometry(c);
+      update_force_state(c);
+    }
+  } while (new_damaged);
+  if (stats) { stats[0] = cgit; stats[1] = passes; stats[2] = n_newton; }
+  return n_newton < max_iter ? 1 : 0;
+}
+
+static size_t field_len(const Ctx* c, int f) {
+  const size_t N = c->N, NB = c->p2.size();
+  switch (f) {
+    case F_XYZ: case F_XYZ_INITIAL: case F_XYZ_LAST: case F_PIN: case F_PEX: case F_CONSTRAINT: return 3 * N;
+    case F_STRESS: case F_CS_SUM: return 6 * N;
+    case F_DAMAGE: case F_DAMAGE_VISUAL: case F_DDOT_LOCAL: case F_DDOT_NONLOCAL: case F_DAMAGE_LAST: case F_TYPE: return N;
+    case F_STATE_VAR: return 7 * N;
+    case F_RESIDUAL: return (size_t)c->dim * N;
+    case F_BDAMAGE: case F_DLP: case F_BFORCE: case F_DIS: case F_BSTRAIN: case F_DLE: return NB;
+    case F_CS: return 3 * NB;
+    case F_DLE_TOTAL: case F_TDLE_TOTAL: return 2 * N;
+  }
+  return 0;
+}
+
+int64_t lpmo_field_len(void* h, int f) { return (int64_t)field_len(static_cast<Ctx*>(h), f); }
+
+int lpmo_get_field(void* h, int f, double* out) {
+  Ctx* c = static_cast<Ctx*>(h);
+  const std::vector<double>* v = nullptr;
+  switch (f) {
+    case F_XYZ: v = &c->xyz; break;
+    case F_XYZ_INITIAL: v = &c->xyz0; break;
+    case F_XYZ_LAST: v = &c->xyz_last; break;
+    case F_PIN: v = &c->Pin; break;
+    case F_PEX: v = &c->Pex; break;
+    case F_STRESS: v = &c->stress; break;
+    case F_DAMAGE: v = &c->damage; break;
+    case F_DAMAGE_VISUAL: v = &c->damage_visual; break;
+    case F_STATE_VAR: v = &c->sv; break;
+    case F_RESIDUAL: v = &c->residual; break;
+    case F_BDAMAGE: v = &c->bdamage; break;
+    case F_DLP: v = &c->dLp; break;
+    case F_BFORCE: v = &c->bforce; break;
+    case F_DIS: v = &c->dis; break;
+    case F_CS: v = &c->cs; break;
+    case F_DDOT_LOCAL: v = &c->Ddot_local; break;
+    case F_DDOT_NONLOCAL: v = &c->Ddot_nonlocal; break;
+    case F_DLE_TOTAL: v = &c->dLe_tot; break;
+    case F_TDLE_TOTAL: v = &c->TdLe_tot; break;
+    case F_CS_SUM: v = &c->cs_sum; break;
+    case F_BSTRAIN: v = &c->bstrain; break;
+    case F_DLE: v = &c->dLe; break;
+    case F_DAMAGE_LAST: v = &c->damage_last; break;
+    case F_CONSTRAINT:
+      for (size_t t = 0; t < c->cons.size(); ++t) out[t] = c->cons[t];
+      return 0;
+    case F_TYPE:
+      for (size_t t = 0; t < c->type.size(); ++t) out[t] = c->type[t];
+      return 0;
+    default: g_err = "unknown field"; return -1;
+  }
+  std::copy(v->begin(), v->end(), out);
+  return 0;
+}
+
+int lpmo_set_field(void* h, int f, const double* in) {
+  Ctx* c = static_cast<Ctx*>(h);
+  std::vector<double>* v = nullptr;
+  switch (f) {
+    case F_XYZ: v = &c->xyz; break;
+    case F_XYZ_LAST: v = &c->xyz_last; break;
+    case F_PEX: v = &c->Pex; break;
+    case F_DAMAGE: v = &c->damage; break;
+    case F_STATE_VAR: v = &c->sv; break;
+    case F_BDAMAGE: v = &c->bdamage; break;
+    case F_DLP: v = &c->dLp; break;
+    case F_CONSTRAINT:
+      for (size_t t = 0; t < c->cons.size(); ++t) c->cons[t] = (int)in[t];
+      return 0;
+    default: g_err = "field is read-only"; return -1;
+  }
+  std::copy(in, in + v->size(), v->begin());
+  return 0;
+}
+
+}  // extern "C"
