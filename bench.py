@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X quasi-static Newton step (LPM-CPP's SolverStatic path).
+
+Workload: the FP64 drop-in configuration -- a 3D simple-cubic elastic cube of
+40^3 = 64,000 particles (r = 0.1 mm, shells 6+12), E = 70 GPa, nu = 0.33, bottom
+face fixed, top face pulled 0.05 mm in 20 quasi-static load steps.  One bench
+step is one complete solve (20 load steps: tangent assembly, modified-Newton
+iterations with FP64 forward passes and Jacobi-preconditioned CG, state update)
+of a device-resident system; the particle topology (bond search, conns, CSR
+pattern) is built once before timing and reported as setup_ms.
+
+Printed JSON: value = FP64 solves/s; the FP32-inner mixed-precision path on the
+same workload, its error against the FP64 solution, p50/p99 solve latency, the
+SpMV roofline (algorithmic bytes over the CG SpMV's average duration, timed with
+HIP events on the system's stream inside the timed solves), assembly and
+forward-pass rates, and the CPU baseline: the oracle (a multithreaded FP64
+restatement of the reference) on one load step, with the reference's CG and
+Newton settings, scaled to solves/s.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lpm-cpp_amd"))
+
+METRIC = "solves/sec + p99 latency at 1e-8 rel. err vs FP64; SpMV/assembly HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+NEWTON_TOL = 1e-10
+FP64_OPTS = dict(precond=1, rtol=1e-12, atol=0.0, precision=0)
+MIXED_OPTS = dict(precond=1, rtol=1e-10, atol=0.0, precision=1)
+# the reference's settings: MKL CG unpreconditioned, dpar[0] = dpar[1] = 1e-12
+# (solver.h:311-313), Newton tol_iter = 1e-5 and 30 iterations (examples)
+REF_OPTS = dict(precond=0, rtol=1e-12, atol=1e-12)
+
+
+def solve_once(s, gids, prob, opts, P):
+    s.restart()
+    t0 = time.perf_counter()
+    cg = newton = 0
+    for step in prob.steps:
+        P.apply_step(s, gids, step)
+        ok, st = s.solve_step(prob.undamaged, prob.max_iter, NEWTON_TOL, **opts)
+        if not ok:
+            raise RuntimeError("a load step did not converge")
+        cg += st[0]
+        newton += st[2]
+    s.synchronize()
+    return time.perf_counter() - t0, cg, newton
+
+
+def percentile(xs, q):
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def cpu_baseline(prob, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from lpmg import problems as P
+
+    oracle.set_threads(threads)
+    t0 = time.perf_counter()
+    o, gids = P.build(oracle.Oracle, prob)
+    o.forward()
+    setup = time.perf_counter() - t0
+    n_sample = 3
+    t0 = time.perf_counter()
+    cg, ok = 0, True
+    for step in prob.steps[:n_sample]:
+        P.apply_step(o, gids, step)
+        ok_k, st = o.solve_step(prob.undamaged, prob.max_iter, 1e-5, **REF_OPTS)
+        ok = ok and ok_k
+        cg += int(st[0])
+    t_step = (time.perf_counter() - t0) / n_sample
+    return dict(value=1.0 / (t_step * len(prob.steps)), unit="solves/s", cores=threads, kind="port",
+                sample=f"load steps 1-{n_sample} of {len(prob.steps)} of the same workload (assembly + "
+                       f"Newton with unpreconditioned CG, rtol=atol=1e-12 on squared norms, tol_iter=1e-5), "
+                       f"{cg} CG iterations, {t_step:.2f} s per load step, scaled to {len(prob.steps)} "
+                       f"steps; setup {setup:.2f} s not included",
+                step_s=t_step, converged=bool(ok))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed solves")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=40, help="particles per cube edge")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every solve of this workload runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        sys.exit(2)
+
+    import lpmg
+    from lpmg import problems as P
+
+    prob = P.sc_cube(n=args.n, steps=20, total=0.05, E=70e3, nu=0.33)
+    t0 = time.perf_counter()
+    s, gids = P.build(lpmg.System, prob)
+    s.synchronize()
+    setup_ms = (time.perf_counter() - t0) * 1e3
+    print(f"# setup {setup_ms:.1f} ms: {s.N} particles, {s.nbond} bonds, {s.nnzb} blocks", file=sys.stderr,
+          flush=True)
+
+    for _ in range(args.warmup):
+        solve_once(s, gids, prob, FP64_OPTS, P)
+    s.profile(True)
+    times, cgs, newtons = [], [], []
+    for k in range(args.steps):
+        t, cg, nw = solve_once(s, gids, prob, FP64_OPTS, P)
+        times.append(t)
+        cgs.append(cg)
+        newtons.append(nw)
+        print(f"# fp64 solve {k}: {t * 1e3:.1f} ms, {cg:.0f} CG iterations, {nw:.0f} Newton", file=sys.stderr,
+              flush=True)
+    prof = s.profile_read()
+    s.profile(False)
+    u64 = s.get("XYZ") - s.get("XYZ_INITIAL")
+    f64 = s.get("PIN")
+    spmv_ms = prof["spmv_ms"] / max(prof["spmv_launches"], 1)
+    spmv_bytes = prof["spmv_bytes_fp64"]
+    achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
+
+    mixed = None
+    if not args.no_mixed:
+        for _ in range(args.warmup):
+            solve_once(s, gids, prob, MIXED_OPTS, P)
+        s.profile(True)
+        mt, mcg = [], []
+        for k in range(args.steps):
+            t, cg, nw = solve_once(s, gids, prob, MIXED_OPTS, P)
+            mt.append(t)
+            mcg.append(cg)
+            print(f"# mixed solve {k}: {t * 1e3:.1f} ms, {cg:.0f} CG iterations, {nw:.0f} Newton",
+                  file=sys.stderr, flush=True)
+        mprof = s.profile_read()
+        s.profile(False)
+        um = s.get("XYZ") - s.get("XYZ_INITIAL")
+        fm = s.get("PIN")
+        m_spmv_ms = mprof["spmv_ms"] / max(mprof["spmv_launches"], 1)
+        mixed = dict(value=len(mt) / sum(mt), unit="solves/s", p50_ms=percentile(mt, 50) * 1e3,
+                     p99_ms=percentile(mt, 99) * 1e3, cg_iterations=float(np.mean(mcg)),
+                     speedup_vs_fp64=(len(mt) / sum(mt)) / (len(times) / sum(times)),
+                     disp_rel_err_vs_fp64=float(np.linalg.norm(um - u64) / np.linalg.norm(u64)),
+                     force_rel_err_vs_fp64=float(np.linalg.norm(fm - f64) / np.linalg.norm(f64)),
+                     spmv_fp32_ms=m_spmv_ms,
+                     spmv_fp32_GBps=mprof["spmv_bytes_fp32"] / (m_spmv_ms * 1e-3) / 1e9)
+
+    asm_ms, asm_bytes = s.time_kernel(2, reps=10)
+    fwd_ms, fwd_bytes = s.time_kernel(3, reps=20)
+    copy_ms, copy_bytes = s.time_kernel(4, reps=10)
+    cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_threads)
+
+    value = len(times) / sum(times)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sum(times) / len(times) * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"SC {args.n}^3 elastic cube, FP64 drop-in path (BSR3 SpMV + Jacobi-PCG), "
+                               f"20 load steps of 0.0025 mm, r=0.1 mm, E=70 GPa, nu=0.33",
+                   "particles": s.N, "dof": s.dim * s.N, "bonds": s.nbond, "bsr_blocks": s.nnzb,
+                   "load_steps": len(prob.steps), "newton_tol": NEWTON_TOL, "cg_rtol_sq": FP64_OPTS["rtol"]},
+        "p50_ms": percentile(times, 50) * 1e3,
+        "p99_ms": percentile(times, 99) * 1e3,
+        "setup_ms": setup_ms,
+        "cg_iterations_per_solve": float(np.mean(cgs)),
+        "newton_iterations_per_solve": float(np.mean(newtons)),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "cg_spmv_kernel<3,double>", "avg_ms": spmv_ms, "launches": prof["spmv_launches"],
+                     "algorithmic_bytes": spmv_bytes},
+        "assembly": {"ms": asm_ms, "algorithmic_bytes": asm_bytes, "GBps": asm_bytes / (asm_ms * 1e-3) / 1e9},
+        "forward": {"ms": fwd_ms, "algorithmic_bytes": fwd_bytes, "GBps": fwd_bytes / (fwd_ms * 1e-3) / 1e9},
+        "stream_copy": {"ms": copy_ms, "bytes": copy_bytes, "GBps": copy_bytes / (copy_ms * 1e-3) / 1e9},
+        "mixed": mixed,
+        "cpu_baseline": cpu,
+    }
+    traffic_file = os.path.join(ROOT, "profiles", "spmv_fp64_sc40_traffic.json")
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -179,6 +179,14 @@ int lpmg_force_bc(lpmg_handle h, int group, int load_mode, double fx, double fy,
 int lpmg_solve_step(lpmg_handle h, int undamaged_type, int max_iter, double tol,
                     const double* opts, double* stats);
 
+/* Restart from the initial configuration without rebuilding the topology (no
+ * reference counterpart; the reference builds a new Assembly per run): positions
+ * back to xyz_initial, every load, constraint, damage, plastic and reaction
+ * history cleared, then updateGeometry + updateForceState as the examples do
+ * before solving (e.g. examples/ex1_isoelasticity_3d_sc.cpp:104-105).  Groups
+ * and materials are kept. */
+int lpmg_restart(lpmg_handle h);
+
 int64_t lpmg_field_len(lpmg_handle h, int field);
 int lpmg_get_field(lpmg_handle h, int field, double* out);
 int lpmg_set_field(lpmg_handle h, int field, const double* in);
@@ -193,7 +201,8 @@ int lpmg_profile_read(lpmg_handle h, double* out);
 
 /* Time `reps` back-to-back launches of one kernel on the handle's stream with
  * HIP events (no reference counterpart): which 0 FP64 SpMV, 1 FP32 SpMV,
- * 2 assembly (FP64), 3 forward pass.  out[0] = average ms per launch,
+ * 2 assembly (FP64), 3 forward pass, 4 a plain copy of the FP64 stiffness
+ * values (byte-counter calibration).  out[0] = average ms per launch,
  * out[1] = algorithmic bytes per launch. */
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out);
 

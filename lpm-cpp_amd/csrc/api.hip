@@ -118,6 +118,13 @@ __global__ void f2d_kernel(const float* __restrict__ a, double* __restrict__ b, 
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = (double)a[t];
 }
+// Plain streaming copy of known size (read n doubles, write n doubles): the
+// reference kernel that calibrates HBM byte counters against algorithmic bytes.
+__global__ void __launch_bounds__(256) stream_copy_kernel(const double* __restrict__ a,
+                                                          double* __restrict__ b, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    b[t] = a[t];
+}
 __global__ void i2d_kernel(const int* __restrict__ a, double* __restrict__ b, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = (double)a[t];
@@ -737,6 +744,29 @@ int lpmg_set_field(lpmg_handle h, int f, const double* in) {
   return 0;
 }
 
+int lpmg_restart(lpmg_handle h) {
+  CHECK_HANDLE(h);
+  Context* c = &h->c;
+  const size_t N = c->N, NB = c->NB;
+  API_HIP(hipMemcpyAsync(c->xyz, c->xyz0, sizeof(double4) * N, hipMemcpyDeviceToDevice, c->stream));
+  API_HIP(hipMemcpyAsync(c->xyz_last, c->xyz0, sizeof(double4) * N, hipMemcpyDeviceToDevice, c->stream));
+  for (double* p : {c->Pex, c->Pin})
+    API_HIP(hipMemsetAsync(p, 0, sizeof(double) * 3 * N, c->stream));
+  API_HIP(hipMemsetAsync(c->cons, 0, sizeof(int) * 3 * N, c->stream));
+  for (double* p : {c->damage, c->damage_last, c->Ddot_local, c->Ddot_nonlocal})
+    API_HIP(hipMemsetAsync(p, 0, sizeof(double) * N, c->stream));
+  API_HIP(hipMemsetAsync(c->sv, 0, sizeof(double) * 7 * N, c->stream));
+  API_HIP(hipMemsetAsync(c->sv_last, 0, sizeof(double) * 7 * N, c->stream));
+  for (double* p : {c->dLp, c->dLp_last, c->bdamage, c->bdamage_last, c->bforce, c->bforce_last})
+    API_HIP(hipMemsetAsync(p, 0, sizeof(double) * NB, c->stream));
+  API_HIP(hipMemcpyAsync(c->dis, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));
+  API_HIP(hipMemcpyAsync(c->dis_last, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));
+  c->reac_sumsq = 0;
+  c->assembled64 = c->assembled32 = false;
+  API_HIP(hipStreamSynchronize(c->stream));
+  return launch_forward(c);
+}
+
 int lpmg_profile(lpmg_handle h, int enable) {
   CHECK_HANDLE(h);
   h->c.prof = enable != 0;
@@ -757,9 +787,12 @@ int lpmg_profile_read(lpmg_handle h, double* out) {
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (reps <= 0 || which < 0 || which > 3) { set_error("bad kernel id or repetitions"); return -1; }
+  if (reps <= 0 || which < 0 || which > 4) { set_error("bad kernel id or repetitions"); return -1; }
   if ((which == 0 || which == 2) && !c->assembled64) { set_error("stiffness not assembled"); return -1; }
   if (which == 1 && !c->assembled32) { set_error("FP32 stiffness not assembled"); return -1; }
+  const int64_t ncopy = (int64_t)c->nnzb * c->dim * c->dim;
+  double* copy_dst = nullptr;
+  if (which == 4) API_HIP(hipMalloc(&copy_dst, sizeof(double) * ncopy));
   hipEvent_t e0, e1;
   API_HIP(hipEventCreate(&e0));
   API_HIP(hipEventCreate(&e1));
@@ -769,7 +802,8 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
     if (which == 0) rc = launch_spmv(c, 0, c->cg_p, c->cg_q);
     else if (which == 1) rc = launch_spmv(c, 1, c->f_p, c->f_q);
     else if (which == 2) rc = launch_assemble(c, c->assembled32 ? 1 : 0);
-    else rc = launch_forward(c);
+    else if (which == 3) rc = launch_forward(c);
+    else stream_copy_kernel<<<2048, 256, 0, c->stream>>>(c->Kval, copy_dst, ncopy);
   }
   API_HIP(hipEventRecord(e1, c->stream));
   API_HIP(hipEventSynchronize(e1));
@@ -777,9 +811,10 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   API_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (copy_dst) (void)hipFree(copy_dst);
   out[0] = ms / reps;
   out[1] = which == 0 ? spmv_bytes(c, 0) : which == 1 ? spmv_bytes(c, 1)
-           : which == 2 ? assembly_bytes(c) : forward_bytes(c);
+           : which == 2 ? assembly_bytes(c) : which == 3 ? forward_bytes(c) : 16.0 * (double)ncopy;
   return rc;
 }
 

@@ -30,7 +30,7 @@ EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "
            "lpmg_set_types", "lpmg_set_material", "lpmg_forward", "lpmg_update_rr", "lpmg_assemble",
            "lpmg_spmv", "lpmg_solve_linear", "lpmg_newton", "lpmg_update_state_var",
            "lpmg_update_static_damage", "lpmg_store_state", "lpmg_reset_state", "lpmg_search_nonlocal",
-           "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_field_len",
+           "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_restart", "lpmg_field_len",
            "lpmg_get_field", "lpmg_set_field", "lpmg_profile", "lpmg_profile_read", "lpmg_time_kernel",
            "lpmg_synchronize")
 
@@ -70,7 +70,8 @@ def lib():
         L.lpmg_export_csr.argtypes = [vp, i64p, i64p, i64p, dp]
         L.lpmg_set_types.argtypes = [vp, i32p]
         L.lpmg_set_material.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int, dp]
-        for name in ("lpmg_forward", "lpmg_update_state_var", "lpmg_store_state", "lpmg_synchronize"):
+        for name in ("lpmg_forward", "lpmg_update_state_var", "lpmg_store_state", "lpmg_synchronize",
+                     "lpmg_restart"):
             getattr(L, name).argtypes = [vp]
         L.lpmg_update_rr.argtypes = [vp, dp]
         L.lpmg_assemble.argtypes = [vp, ctypes.c_int]
@@ -255,6 +256,10 @@ class System:
 
     def synchronize(self):
         self._chk(lib().lpmg_synchronize(self.h))
+
+    def restart(self):
+        """Back to the initial configuration, topology kept (see lpmg_restart)."""
+        self._chk(lib().lpmg_restart(self.h))
 
     # -- instrumentation
     def profile(self, enable=True):

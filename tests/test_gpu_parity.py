@@ -134,12 +134,26 @@ def test_forward_pass_matches_oracle(name, L, gen, ptype, model, params):
 
 
 def _csr_close(g, o):
+    """Entries within 1e-13 of their row's largest magnitude.  The diagonal of an
+    eliminated DOF is ||diag(K)||_2 (stiffness.h:378), a sum over all DOFs whose
+    rounding depends on the summation order (the oracle sums serially, the GPU
+    by a tree): those entries are compared at 1e-12."""
     Kg, Ko = g.csr()[3], o.csr()[3]
     Kp, IK, JK, _ = o.csr()
     rows = np.repeat(np.arange(len(IK) - 1), np.diff(IK))
     rowmax = np.zeros(len(IK) - 1)
     np.maximum.at(rowmax, rows, np.abs(Ko))
-    return np.abs(Kg - Ko) <= 1e-13 * rowmax[rows] + 1e-300
+    ok = np.abs(Kg - Ko) <= 1e-13 * rowmax[rows] + 1e-300
+    cons = o.get("CONSTRAINT").reshape(-1, 3)[:, :o.dim].ravel() == 1
+    diag_pos = IK[:-1] - 1
+    elim = diag_pos[cons]
+    if elim.size:
+        nd_o, nd_g = Ko[elim], Kg[elim]
+        assert np.all(nd_o == nd_o[0]) and np.all(nd_g == nd_g[0])
+        gap = abs(nd_g[0] - nd_o[0]) / nd_o[0]
+        print(f"||diag||_2 relative gap GPU vs oracle: {gap:.2e}")
+        ok[elim] = gap <= 1e-12
+    return ok
 
 
 @pytest.mark.parametrize("name,L,gen", LATTICES)
@@ -294,6 +308,20 @@ def test_elastic_damage_with_nonlocal_averaging_matches_oracle():
     np.testing.assert_array_equal(dg >= 1, do >= 1)
     assert np.abs(dg - do).max() < 1e-8
     assert _relerr(g.get("XYZ") - o.get("XYZ_INITIAL"), o.get("XYZ") - o.get("XYZ_INITIAL")) < 1e-8
+
+
+def test_restart_and_rerun_is_bit_identical():
+    # reductions are ordered (last-workgroup sums), so a re-run on the same device
+    # reproduces every bit; lpmg_restart returns to the initial configuration
+    prob = P.sc_cube(n=8, steps=3)
+    s, gids = P.build(lpmg.System, prob)
+    outs = []
+    for _ in range(2):
+        s.restart()
+        P.solve_problem(s, gids, P.Problem(**{**prob.__dict__, "tol": 1e-11}), GPU_OPTS)
+        outs.append((s.get("XYZ"), s.get("PIN")))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
 def test_benchmark_size_properties():
