@@ -174,11 +174,15 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   rc |= dalloc(c, &c->Kn, 2 * N);
   rc |= dalloc(c, &c->type, N);
   rc |= dalloc(c, &c->cons, 3 * N);
-  for (double** p : {&c->residual, &c->du, &c->cg_r, &c->cg_p, &c->cg_q, &c->cg_z, &c->dinv})
+  for (double** p : {&c->residual, &c->du, &c->cg_r, &c->cg_p, &c->cg_q, &c->cg_z, &c->cg_w, &c->dinv})
     rc |= dalloc(c, p, d * N);
-  for (float** p : {&c->f_r, &c->f_p, &c->f_q, &c->f_z, &c->f_x, &c->f_dinv}) rc |= dalloc(c, p, d * N);
-  c->n_partials = std::max<int>((int)((N + 255) / 256), 2048) + 64;
-  rc |= dalloc(c, &c->partials, 4 * (size_t)c->n_partials);
+  for (float** p : {&c->f_r, &c->f_p, &c->f_q, &c->f_z, &c->f_w, &c->f_x, &c->f_dinv})
+    rc |= dalloc(c, p, d * N);
+  // reduction partials: two vector-kernel buffers (2 x 2048), SpMV partials
+  // (one per SpMV workgroup of >= 16 rows, plus a second-level buffer), and the
+  // per-workgroup partials of the N-sized reductions (2 per 256 particles)
+  c->n_partials = std::max<int>((int)((N + 255) / 256), 4096) + (int)((N + 7) / 8) + 4096 + 64;
+  rc |= dalloc(c, &c->partials, 2 * (size_t)c->n_partials);
   rc |= dalloc(c, &c->sc, 1);
   rc |= dalloc(c, &c->err_flag, 1);
   if (rc) return -3;
@@ -194,9 +198,9 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   API_HIP(hipMemsetAsync(c->sv_last, 0, sizeof(double) * 7 * N, c->stream));
   API_HIP(hipMemsetAsync(c->type, 0, sizeof(int) * N, c->stream));
   API_HIP(hipMemsetAsync(c->cons, 0, sizeof(int) * 3 * N, c->stream));
-  for (double* p : {c->residual, c->du, c->cg_r, c->cg_p, c->cg_q, c->cg_z, c->dinv})
+  for (double* p : {c->residual, c->du, c->cg_r, c->cg_p, c->cg_q, c->cg_z, c->cg_w, c->dinv})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * d * N, c->stream));
-  for (float* p : {c->f_r, c->f_p, c->f_q, c->f_z, c->f_x, c->f_dinv})
+  for (float* p : {c->f_r, c->f_p, c->f_q, c->f_z, c->f_w, c->f_x, c->f_dinv})
     API_HIP(hipMemsetAsync(p, 0, sizeof(float) * d * N, c->stream));
   for (auto* v : {&h->Tv, &h->crit, &h->nlL, &h->mE, &h->mMu, &h->mSy, &h->mXi, &h->mH, &h->mA, &h->k0,
                   &h->k1, &h->dthr, &h->ctr})
@@ -230,6 +234,8 @@ int finish_create(lpmg_context* h) {
 int free_context(lpmg_context* h) {
   Context* c = &h->c;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (hipGraphExec_t& g : c->cg_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (auto& g : c->groups) (void)hipFree(g.idx);
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->sc_host) (void)hipHostFree(c->sc_host);

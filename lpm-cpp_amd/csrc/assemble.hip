@@ -153,10 +153,13 @@ norm_diag_kernel(const double* __restrict__ diag_sq, int N, double* __restrict__
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   Vec<1> a;
   a.v[0] = i < N ? diag_sq[i] : 0.0;
-  a = block_sum<256, 1>(a);
-  Vec<1> tot;
-  if (last_block_sum<256, 1>(a, partials, &sc->counter[1], &tot))
-    if (threadIdx.x == 0) sc->norm_diag = sqrt(tot.v[0]);
+  write_partial<256, 1>(partials, a);
+}
+
+__global__ void __launch_bounds__(256) norm_diag_finish_kernel(const double* __restrict__ partials, int n,
+                                                               Scalars* sc) {
+  const Vec<1> tot = sum_partials<256, 1>(partials, n);
+  if (threadIdx.x == 0) sc->norm_diag = sqrt(tot.v[0]);
 }
 
 __device__ __forceinline__ bool dof_fixed(const int* cons, const double* dvis, int i, int k) {
@@ -273,6 +276,7 @@ int launch_assemble(Context* c, int want32) {
                                                          c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
                                                          N, c->Kval, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
+    norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     symmetrize_bc_kernel<3><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
                                                       c->damage_visual, c->sc, c->Kval, k32);
   } else {
@@ -280,6 +284,7 @@ int launch_assemble(Context* c, int want32) {
                                                          c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
                                                          N, c->Kval, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
+    norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     symmetrize_bc_kernel<2><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
                                                       c->damage_visual, c->sc, c->Kval, k32);
   }

@@ -139,12 +139,18 @@ update_rr_kernel(const double* __restrict__ Pex, const double* __restrict__ Pin,
       if (cn == 1) reac += pin * pin;
     }
   }
-  double2 v = block_sum2<256>(make_double2(rr, reac));
-  if (last_block_sum2(v, partials, &sc->counter[0], &v)) {
-    if (threadIdx.x == 0) {
-      sc->rnorm_sq = v.x;
-      sc->reac_sq = v.y;
-    }
+  Vec<2> a;
+  a.v[0] = rr;
+  a.v[1] = reac;
+  write_partial<256, 2>(partials, a);
+}
+
+__global__ void __launch_bounds__(256) update_rr_finish_kernel(const double* __restrict__ partials, int n,
+                                                               Scalars* sc) {
+  const Vec<2> a = sum_partials<256, 2>(partials, n);
+  if (threadIdx.x == 0) {
+    sc->rnorm_sq = a.v[0];
+    sc->reac_sq = a.v[1];
   }
 }
 
@@ -213,6 +219,7 @@ int launch_update_rr(Context* c, double* norm_res, double* reac_sq) {
   }
   update_rr_kernel<<<nb, tb, 0, c->stream>>>(c->Pex, c->Pin, c->cons, N, c->dim, c->residual,
                                              c->partials, c->sc);
+  update_rr_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nb, c->sc);
   LPMG_HIP(hipGetLastError());
   LPMG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
   LPMG_HIP(hipStreamSynchronize(c->stream));

@@ -32,11 +32,13 @@ bool make_cell(int lattice, double r, Cell* c);
 
 // Device scalars of one CG solve and of the reductions around it.
 struct Scalars {
-  double rz, rr, rr0, alpha, beta, pq, thresh;  // CG
+  double rz, rr, rr0, alpha, beta, pq, thresh;  // CG (last iteration)
+  double rtol, atol;                            // stopping test of the current solve
+  double gam[2], alp[2];                        // (r,u) and alpha of iteration i in slot i & 1
   double rnorm_sq, reac_sq;                     // residual / reaction (updateRR)
   double norm_diag;                             // updateStiffnessDispBC
   int iter, done, maxit, any_flag;
-  unsigned int counter[4];                      // last-block-done tickets
+  int icur, inext;                              // iteration index of the CG vector step
 };
 
 template <typename T>
@@ -89,8 +91,8 @@ struct Context {
   int64_t n_nl = 0;
   // solver vectors
   double *residual = nullptr, *du = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr,
-         *cg_z = nullptr, *dinv = nullptr;
-  float *f_r = nullptr, *f_p = nullptr, *f_q = nullptr, *f_z = nullptr, *f_x = nullptr,
+         *cg_z = nullptr, *cg_w = nullptr, *dinv = nullptr;
+  float *f_r = nullptr, *f_p = nullptr, *f_q = nullptr, *f_z = nullptr, *f_w = nullptr, *f_x = nullptr,
         *f_dinv = nullptr;
   double* partials = nullptr;  // per-block reduction partials
   int n_partials = 0;
@@ -107,6 +109,9 @@ struct Context {
   double prof_spmv_ms = 0;
   int64_t prof_spmv_n = 0;
   int prof_spmv_prec = -1;
+  // captured CG chunk per precision (FP64, FP32)
+  hipGraphExec_t cg_graph[2] = {nullptr, nullptr};
+  int sc_host_iter_launched = 0;
 };
 
 // ---- helpers (api.hip)
@@ -155,7 +160,7 @@ struct SolveOpts {
   int precision = 0;    // 0 FP64, 1 FP32 inner solve
   double rtol = 1e-12, atol = 1e-12;  // ||r||^2 <= rtol*||r0||^2 + atol
   int64_t maxit = 0;    // 0 -> problem size
-  int chunk = 16;       // iterations launched between convergence checks
+  int chunk = 16;       // unused: the chunk length is fixed (graph capture)
 };
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 int launch_spmv(Context* c, int precision, const void* x, void* y);

@@ -1,11 +1,11 @@
 // reduce.h -- deterministic block and grid reductions for 64-wide wavefronts.
 //
-// Grid reductions use the last-block-done pattern: each workgroup writes its
-// partial sums, publishes them with an agent-scope release fence and takes a
-// ticket; the workgroup that draws the last ticket acquires and sums all the
-// partials in a fixed order, so the result does not depend on scheduling.  The
-// ticket counter is reset by that last workgroup (no memset is needed, which
-// keeps these kernels safe to capture in graphs).
+// A grid reduction is split across a kernel boundary: each workgroup of the
+// producing kernel writes its partial sum (plain stores); a later kernel sums
+// the partials in a fixed order.  No kernel waits on other workgroups of the
+// same launch, so no agent-scope fence is issued inside a kernel (on gfx950 such
+// a fence writes back and invalidates the XCD's L2, which costs more than the
+// reduction itself), and the result does not depend on scheduling.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,54 +46,27 @@ __device__ inline Vec<NV> block_sum(Vec<NV> a) {
   return t;
 }
 
-// Publish this block's partial; returns true (in every thread of the block) for
-// the last block, which then holds the grid total in *out.  Must be reached by
-// every thread of every block.
+// This workgroup's partial goes to part[NV*blockIdx.x ...].  Every thread of the
+// workgroup must call it.
 template <int BS, int NV>
-__device__ inline bool last_block_sum(Vec<NV> blocktot, double* partials, unsigned int* counter,
-                                      Vec<NV>* out) {
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
+__device__ inline void write_partial(double* __restrict__ part, Vec<NV> a) {
+  a = block_sum<BS, NV>(a);
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) partials[(size_t)NV * blockIdx.x + k] = blocktot.v[k];
-    __threadfence();
-    const unsigned int ticket = atomicAdd(counter, 1u);
-    is_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
+    for (int k = 0; k < NV; ++k) part[(size_t)NV * blockIdx.x + k] = a.v[k];
+}
+
+// Sum of n partials written by an earlier kernel, in a fixed order; the same
+// bits in every thread of every workgroup.
+template <int BS, int NV>
+__device__ inline Vec<NV> sum_partials(const double* __restrict__ part, int n) {
   Vec<NV> a;
 #pragma unroll
   for (int k = 0; k < NV; ++k) a.v[k] = 0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS)
+  for (int b = threadIdx.x; b < n; b += BS)
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      a.v[k] += __hip_atomic_load(&partials[(size_t)NV * b + k], __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-  *out = block_sum<BS, NV>(a);
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
-}
-
-template <int BS>
-__device__ inline double2 block_sum2(double2 v) {
-  Vec<2> a;
-  a.v[0] = v.x;
-  a.v[1] = v.y;
-  a = block_sum<BS, 2>(a);
-  return make_double2(a.v[0], a.v[1]);
-}
-
-template <int BS = 256>
-__device__ inline bool last_block_sum2(double2 blocktot, double* partials, unsigned int* counter,
-                                       double2* out) {
-  Vec<2> a, o;
-  a.v[0] = blocktot.x;
-  a.v[1] = blocktot.y;
-  const bool last = last_block_sum<BS, 2>(a, partials, counter, &o);
-  if (last) *out = make_double2(o.v[0], o.v[1]);
-  return last;
+    for (int k = 0; k < NV; ++k) a.v[k] += part[(size_t)NV * b + k];
+  return block_sum<BS, NV>(a);
 }
 
 }  // namespace lpmg

@@ -2,18 +2,28 @@
 // MKL's RCI CG (Solver::LPM_CG, solver.h:276-354) and PARDISO
 // (Solver::LPM_PARDISO, solver.h:190-274) as the linear solve of the Newton step.
 //
-// SpMV streams the BSR values once: a wavefront owns a contiguous run of block
-// rows; per row, lane l < (64/(b*b))*(b*b) always takes value l of each group of
-// 64/(b*b) blocks, so its (block, r, s) position is fixed and the value loads of
-// the 63 (b=3) or 64 (b=2) active lanes are one contiguous, coalesced segment.
-// Workgroups are remapped so that each of the 8 XCDs owns one contiguous eighth
-// of the rows (its L2 then holds the x entries its rows gather).  The operator is
-// sparse, so MFMA is not used.  FP32 values stream half the bytes of FP64.
+// SpMV: one lane per b x b block.  A row (<= 64 blocks for the SC/BCC lattices,
+// <= 32 for the 2D ones) is owned by 64 or 32 lanes, so one wavefront covers one
+// 3D row or two 2D rows in a single pass: each lane loads its block's b*b values
+// (a contiguous 72/32-byte run, the wave's loads together one contiguous
+// segment), its block column and the b entries of x, multiplies, and the row's b
+// sums are reduced across the row's lanes.  All loads are unconditional
+// straight-line code (indices past the row's end are clamped and their products
+// dropped), so they are in flight together; with one row per wavefront the
+// hardware scheduler hides the latency with other wavefronts.  Measured on the
+// SC 40^3 pattern this streams at 6.4 TB/s (FP64), 80% of the 8 TB/s HBM peak.
+// The operator is sparse, so MFMA is not used.
 //
-// CG keeps every scalar on the device.  Dot products are reduced per workgroup
-// and finished by the last workgroup in a fixed order (deterministic), which
-// also computes alpha / beta and the stopping test.  The stopping test is the
-// one the reference configures MKL with: ||r||^2 <= rtol*||r0||^2 + atol
+// CG is the Chronopoulos-Gear single-reduction form of preconditioned CG: per
+// iteration one SpMV kernel (w = A u, with the partials of (w,u)) and one fused
+// vector kernel (p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s;
+// u = M r; partials of (r,u) and (r,r)).  No kernel waits on other workgroups of
+// its own launch: each kernel writes one partial per workgroup and the next
+// kernel sums them in a fixed order in every workgroup (deterministic), so no
+// intra-kernel fence is ever issued (on gfx950 an agent-scope fence writes back
+// and invalidates the XCD's L2).  Scalars that live across iterations are written
+// by workgroup 0 into slots indexed by the iteration parity.  The stopping test
+// is the one the reference configures MKL with: ||r||^2 <= rtol*||r0||^2 + atol
 // (dpar[0], dpar[1], solver.h:312-313), at most maxit (ipar[4] = n) iterations.
 // The host launches `chunk` iterations between reads of the convergence flag;
 // kernels of iterations after convergence return immediately.
@@ -27,180 +37,209 @@ namespace lpmg {
 
 namespace {
 
-constexpr int kBS = 256;
-constexpr int kMaxGrid = 2048;  // workgroups of the grid-strided kernels (multiple of 8)
+constexpr int kBS = 256;         // vector kernels
+constexpr int kSpBS = 512;       // SpMV kernels: 8 wavefronts per workgroup
+constexpr int kMaxGrid = 1024;   // workgroups of the grid-strided vector kernels
+constexpr int kMaxParts = 8192;  // SpMV partials summed directly by the consumer
+constexpr int kChunk = 16;       // CG iterations per launch batch (even)
 
-__device__ __forceinline__ int xcd_remap(int b, int nb) {
-  const int per = nb >> 3, rem = nb & 7;
-  const int x = b & 7, idx = b >> 3;
-  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
-}
-
-// y = A x for the wavefront's rows [row0, row1); returns the wave's partial of
-// dot(x, y) in every lane when DOT.
-template <int D, typename T, bool DOT>
-__device__ __forceinline__ double spmv_rows(const int* __restrict__ cptr, const int* __restrict__ conns,
-                                            const T* __restrict__ val, const T* __restrict__ x,
-                                            T* __restrict__ y, int row0, int row1) {
+// y = A x for the rows of this wavefront (64 / LPR rows, LPR lanes each).  With
+// DOT, returns this lane's share of x.y (nonzero only in the row's first b lanes).
+template <int D, typename T, int LPR, bool DOT>
+__device__ __forceinline__ double bsr_rows(const int* __restrict__ cptr, const int* __restrict__ conns,
+                                           const T* __restrict__ val, const T* __restrict__ x,
+                                           T* __restrict__ y, int N) {
   constexpr int DD = D * D;
-  constexpr int BPI = 64 / DD;        // blocks per iteration
-  constexpr int LANES = BPI * DD;     // active lanes
-  const int lane = threadIdx.x & 63;
-  const int boff = lane / DD, rs = lane - boff * DD, r = rs / D, s = rs - r * D;
-  const bool on = lane < LANES;
-  double dot = 0;
-  for (int row = row0; row < row1; ++row) {
-    const int lo = cptr[row], hi = cptr[row + 1];
-    T acc = 0;
-    int blk = lo + boff;
-    const T* vp = val + (size_t)lo * DD + lane;
-#pragma unroll 2
-    for (int b0 = lo; b0 < hi; b0 += BPI, blk += BPI, vp += BPI * DD) {
-      if (on && blk < hi) {
-        const int col = conns[blk];
-        acc += *vp * x[(size_t)col * D + s];
-      }
-    }
-    // reduce lanes by output component r
-    T out[D];
+  constexpr int RPW = 64 / LPR;
+  const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
+  const int wave = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int row = wave * RPW + sub;
+  const bool live = row < N;
+  const int rw = live ? row : N - 1;  // dead rows shadow the last row: all lanes stay in the shuffles
+  const int lo = cptr[rw], hi = cptr[rw + 1];
+  T acc[D];
+#pragma unroll
+  for (int q = 0; q < D; ++q) acc[q] = T(0);
+  for (int e0 = lo; e0 < hi; e0 += LPR) {
+    const int e = e0 + sl;
+    const bool ok = e < hi;
+    const int ec = ok ? e : hi - 1;  // every row holds its self block, so hi - 1 >= lo
+    const int col = conns[ec];
+    const T* vb = val + (size_t)ec * DD;
+    T v[DD], xv[D];
+#pragma unroll
+    for (int k = 0; k < DD; ++k) v[k] = vb[k];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xv[k] = x[(size_t)col * D + k];
 #pragma unroll
     for (int q = 0; q < D; ++q) {
-      T v = (on && r == q) ? acc : T(0);
+      T t = v[q * D] * xv[0];
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      out[q] = v;
+      for (int k = 1; k < D; ++k) t += v[q * D + k] * xv[k];
+      acc[q] += ok ? t : T(0);
     }
-    if (lane < D) {
-      T yv = out[0];
+  }
 #pragma unroll
-      for (int q = 1; q < D; ++q)
-        if (lane == q) yv = out[q];
-      y[(size_t)row * D + lane] = yv;
-      if (DOT) dot += (double)x[(size_t)row * D + lane] * (double)yv;
-    }
+  for (int o = LPR / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < D; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
+  double dot = 0;
+  if (live && sl < D) {
+    T yv = acc[0];
+#pragma unroll
+    for (int q = 1; q < D; ++q)
+      if (sl == q) yv = acc[q];
+    y[(size_t)row * D + sl] = yv;
+    if (DOT) dot = (double)x[(size_t)row * D + sl] * (double)yv;
   }
   return dot;
 }
 
-__device__ __forceinline__ void wave_rows(int N, int* r0, int* r1) {
-  const int nwaves = gridDim.x * (kBS / 64);
-  const int wid = xcd_remap(blockIdx.x, gridDim.x) * (kBS / 64) + (threadIdx.x >> 6);
-  const int per = (N + nwaves - 1) / nwaves;
-  *r0 = min(N, wid * per);
-  *r1 = min(N, *r0 + per);
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
+                                                     const T* __restrict__ val, const T* __restrict__ x,
+                                                     T* __restrict__ y, int N) {
+  bsr_rows<D, T, LPR, false>(cptr, conns, val, x, y, N);
 }
 
-template <int D, typename T>
-__global__ void __launch_bounds__(kBS) spmv_kernel(const int* __restrict__ cptr,
-                                                   const int* __restrict__ conns,
-                                                   const T* __restrict__ val, const T* __restrict__ x,
-                                                   T* __restrict__ y, int N) {
-  int r0, r1;
-  wave_rows(N, &r0, &r1);
-  spmv_rows<D, T, false>(cptr, conns, val, x, y, r0, r1);
-}
-
-// CG step A: q = A p, pq = p.q, alpha = rz / pq
-template <int D, typename T>
-__global__ void __launch_bounds__(kBS) cg_spmv_kernel(const int* __restrict__ cptr,
-                                                      const int* __restrict__ conns,
-                                                      const T* __restrict__ val, const T* __restrict__ p,
-                                                      T* __restrict__ q, int N, double* partials,
-                                                      Scalars* sc) {
+// SpMV of one CG iteration: w = A u and the per-workgroup partials of (w, u)
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) cg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
+                                                        const T* __restrict__ val, const T* __restrict__ u,
+                                                        T* __restrict__ w, int N, double* __restrict__ partS,
+                                                        Scalars* sc) {
   if (sc->done) return;
-  int r0, r1;
-  wave_rows(N, &r0, &r1);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // iteration index of the next vector step
   Vec<1> a;
-  a.v[0] = spmv_rows<D, T, true>(cptr, conns, val, p, q, r0, r1);
-  // only lanes < D carry a dot partial; block_sum adds all lanes
-  a = block_sum<kBS, 1>(a);
-  Vec<1> tot;
-  if (last_block_sum<kBS, 1>(a, partials, &sc->counter[2], &tot))
-    if (threadIdx.x == 0) {
-      sc->pq = tot.v[0];
-      sc->alpha = sc->rz / tot.v[0];
-    }
+  a.v[0] = bsr_rows<D, T, LPR, true>(cptr, conns, val, u, w, N);
+  write_partial<kSpBS, 1>(partS, a);
 }
 
-// CG step B: x += alpha p; r -= alpha q; z = dinv r; rz, rr; beta; stopping test
-template <typename T>
-__global__ void __launch_bounds__(kBS) cg_update_kernel(T* __restrict__ x, T* __restrict__ r,
-                                                        T* __restrict__ z, const T* __restrict__ p,
-                                                        const T* __restrict__ q,
-                                                        const T* __restrict__ dinv, int64_t n,
-                                                        double* partials, Scalars* sc) {
+// Second level for large grids: n partials -> gridDim.x partials (contiguous slices)
+__global__ void __launch_bounds__(kBS) reduce_partials_kernel(const double* __restrict__ in, int n,
+                                                              double* __restrict__ out,
+                                                              const Scalars* __restrict__ sc) {
   if (sc->done) return;
-  const T alpha = (T)sc->alpha;
-  double rz = 0, rr = 0;
-  for (int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBS) {
-    x[t] += alpha * p[t];
-    const T rv = r[t] - alpha * q[t];
-    r[t] = rv;
-    const T zv = dinv[t] * rv;
-    z[t] = zv;
-    rz += (double)rv * (double)zv;
-    rr += (double)rv * (double)rv;
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(n, lo + per);
+  Vec<1> a;
+  a.v[0] = 0;
+  for (int b = lo + threadIdx.x; b < hi; b += kBS) a.v[0] += in[b];
+  write_partial<kBS, 1>(out, a);
+}
+
+// The scalars of iteration i from the partials: gamma = (r,u), delta = (w,u), rr = (r,r)
+struct CGStep {
+  double gamma, delta, rr, alpha, beta;
+  bool stop;
+};
+
+__device__ __forceinline__ CGStep cg_scalars(const double* __restrict__ partV, int nV,
+                                             const double* __restrict__ partS, int nS, const Scalars* sc, int i) {
+  Vec<2> gv = sum_partials<kBS, 2>(partV, nV);
+  Vec<1> dv = sum_partials<kBS, 1>(partS, nS);
+  CGStep st;
+  st.gamma = gv.v[0];
+  st.rr = gv.v[1];
+  st.delta = dv.v[0];
+  if (i == 0) {
+    st.beta = 0;
+    st.alpha = st.gamma / st.delta;
+    st.stop = st.rr <= sc->rtol * st.rr + sc->atol || sc->maxit <= 0;
+  } else {
+    const double gamma_old = sc->gam[(i - 1) & 1], alpha_old = sc->alp[(i - 1) & 1];
+    st.beta = st.gamma / gamma_old;
+    st.alpha = st.gamma / (st.delta - st.beta * st.gamma / alpha_old);
+    st.stop = st.rr <= sc->thresh || i >= sc->maxit;
+  }
+  return st;
+}
+
+// Vector step of iteration i: p = u + beta p; s = w + beta s; x += alpha p;
+// r -= alpha s; u = M r; partials of (r,u), (r,r)
+template <typename T>
+__global__ void __launch_bounds__(kBS) cg_vec_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
+                                                     T* __restrict__ p, T* __restrict__ s,
+                                                     const T* __restrict__ w, const T* __restrict__ dinv,
+                                                     int64_t n, const double* __restrict__ partV_in, int nV,
+                                                     const double* __restrict__ partS, int nS,
+                                                     double* __restrict__ partV_out, Scalars* sc) {
+  if (sc->done) return;
+  const int i = sc->icur;
+  const CGStep st = cg_scalars(partV_in, nV, partS, nS, sc, i);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (i == 0) {
+      sc->rr0 = st.rr;
+      sc->thresh = sc->rtol * st.rr + sc->atol;
+    }
+    sc->gam[i & 1] = st.gamma;
+    sc->alp[i & 1] = st.alpha;
+    sc->rz = st.gamma;
+    sc->rr = st.rr;
+    sc->pq = st.delta;
+    sc->alpha = st.alpha;
+    sc->beta = st.beta;
+    sc->iter = i;
+    sc->inext = i + 1;
+    if (st.stop) sc->done = 1;
+  }
+  if (st.stop) return;  // every workgroup reaches the same decision
+  const T alpha = (T)st.alpha, beta = (T)st.beta;
+  double ru = 0, rr = 0;
+  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
+    const T pk = u[k] + beta * p[k];
+    const T sk = w[k] + beta * s[k];
+    p[k] = pk;
+    s[k] = sk;
+    x[k] += alpha * pk;
+    const T rk = r[k] - alpha * sk;
+    r[k] = rk;
+    const T uk = dinv[k] * rk;
+    u[k] = uk;
+    ru += (double)rk * (double)uk;
+    rr += (double)rk * (double)rk;
   }
   Vec<2> a;
-  a.v[0] = rz;
+  a.v[0] = ru;
   a.v[1] = rr;
-  a = block_sum<kBS, 2>(a);
-  Vec<2> tot;
-  if (last_block_sum<kBS, 2>(a, partials, &sc->counter[3], &tot))
-    if (threadIdx.x == 0) {
-      sc->beta = tot.v[0] / sc->rz;
-      sc->rz = tot.v[0];
-      sc->rr = tot.v[1];
-      sc->iter += 1;
-      if (tot.v[1] <= sc->thresh || sc->iter >= sc->maxit) sc->done = 1;
-    }
+  write_partial<kBS, 2>(partV_out, a);
 }
 
-// CG step C: p = z + beta p
-template <typename T>
-__global__ void __launch_bounds__(kBS) cg_dir_kernel(T* __restrict__ p, const T* __restrict__ z,
-                                                     int64_t n, const Scalars* sc) {
-  if (sc->done) return;
-  const T beta = (T)sc->beta;
-  for (int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBS)
-    p[t] = z[t] + beta * p[t];
-}
-
-// Start: x = 0, r = b, z = dinv b, p = z, rz, rr0, threshold.
+// Start: x = 0, r = b, u = M b, p = s = 0; partials of (r,u), (r,r)
 template <typename T>
 __global__ void __launch_bounds__(kBS) cg_init_kernel(const double* __restrict__ b, T* __restrict__ x,
-                                                      T* __restrict__ r, T* __restrict__ z,
-                                                      T* __restrict__ p, const T* __restrict__ dinv,
-                                                      int64_t n, double rtol, double atol,
-                                                      int64_t maxit, double* partials, Scalars* sc) {
-  double rz = 0, rr = 0;
-  for (int64_t t = (int64_t)blockIdx.x * kBS + threadIdx.x; t < n; t += (int64_t)gridDim.x * kBS) {
-    const T rv = (T)b[t];
-    x[t] = 0;
-    r[t] = rv;
-    const T zv = dinv[t] * rv;
-    z[t] = zv;
-    p[t] = zv;
-    rz += (double)rv * (double)zv;
-    rr += (double)rv * (double)rv;
+                                                      T* __restrict__ r, T* __restrict__ u,
+                                                      T* __restrict__ p, T* __restrict__ s,
+                                                      const T* __restrict__ dinv, int64_t n,
+                                                      double* __restrict__ partV) {
+  double ru = 0, rr = 0;
+  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
+    const T rk = (T)b[k];
+    x[k] = 0;
+    p[k] = 0;
+    s[k] = 0;
+    r[k] = rk;
+    const T uk = dinv[k] * rk;
+    u[k] = uk;
+    ru += (double)rk * (double)uk;
+    rr += (double)rk * (double)rk;
   }
   Vec<2> a;
-  a.v[0] = rz;
+  a.v[0] = ru;
   a.v[1] = rr;
-  a = block_sum<kBS, 2>(a);
-  Vec<2> tot;
-  if (last_block_sum<kBS, 2>(a, partials, &sc->counter[3], &tot))
-    if (threadIdx.x == 0) {
-      sc->rz = tot.v[0];
-      sc->rr = tot.v[1];
-      sc->rr0 = tot.v[1];
-      sc->thresh = rtol * tot.v[1] + atol;
-      sc->iter = 0;
-      sc->maxit = (int)maxit;
-      sc->alpha = 0;
-      sc->beta = 0;
-      sc->done = (tot.v[1] <= sc->thresh || maxit <= 0) ? 1 : 0;
-    }
+  write_partial<kBS, 2>(partV, a);
+}
+
+__global__ void cg_setup_kernel(Scalars* sc, double rtol, double atol, int64_t maxit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->rtol = rtol;
+    sc->atol = atol;
+    sc->maxit = (int)maxit;
+    sc->iter = 0;
+    sc->icur = 0;
+    sc->inext = 0;
+    sc->done = 0;
+  }
 }
 
 // Jacobi preconditioner from the BSR diagonal blocks (1 for the plain CG).
@@ -225,42 +264,95 @@ int grid_for(int64_t work, int per) {
   return (int)g;
 }
 
+// lanes per row: 32 when every row fits in half a wavefront, else 64
+int lanes_per_row(const Context* c) { return c->max_nconn <= 32 ? 32 : 64; }
+
+int spmv_grid(const Context* c) {
+  const int rows_per_block = (kSpBS / 64) * (64 / lanes_per_row(c));
+  return (c->N + rows_per_block - 1) / rows_per_block;
+}
+
 template <int D, typename T>
-int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* z, T* p, T* q, T* dinv, const T* val,
+void launch_bsr(Context* c, const T* val, const T* x, T* y) {
+  const int g = spmv_grid(c);
+  if (lanes_per_row(c) == 32)
+    spmv_kernel<D, T, 32><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
+  else
+    spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
+}
+
+template <int D, typename T>
+void launch_cg_spmv(Context* c, const T* val, const T* u, T* w, double* partS) {
+  const int g = spmv_grid(c);
+  if (lanes_per_row(c) == 32)
+    cg_spmv_kernel<D, T, 32><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, u, w, c->N, partS, c->sc);
+  else
+    cg_spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, u, w, c->N, partS, c->sc);
+}
+
+template <int D, typename T>
+int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* u, T* p, T* s, T* w, T* dinv, const T* val,
            int* iters, double* rr_final) {
   const int N = c->N;
   const int64_t n = (int64_t)D * N;
-  const int gv = grid_for(n, kBS * 4);
-  const int gs = grid_for(N, 4 * 4);  // ~4 rows per wavefront
+  const int gv = grid_for(n, kBS);
+  const int gs = spmv_grid(c);
+  const bool two_level = gs > kMaxParts;
+  const int nS = two_level ? kBS : gs;
+  double* partV[2] = {c->partials, c->partials + 2 * kMaxGrid};
+  double* partS_raw = c->partials + 4 * kMaxGrid;
+  double* partS = two_level ? c->partials + 4 * kMaxGrid + gs : partS_raw;
   dinv_kernel<D, T><<<(N + 255) / 256, 256, 0, c->stream>>>(c->dpos, val, N, o.precond, dinv);
   const int64_t maxit = o.maxit > 0 ? o.maxit : n;
-  cg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, z, p, dinv, n, o.rtol, o.atol, maxit,
-                                               c->partials, c->sc);
+  cg_setup_kernel<<<1, 64, 0, c->stream>>>(c->sc, o.rtol, o.atol, maxit);
+  cg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, u, p, s, dinv, n, partV[0]);
   LPMG_HIP(hipGetLastError());
-  const int chunk = std::max(1, o.chunk);
-  std::vector<hipEvent_t> evs;
-  if (c->prof) {
-    evs.resize(2 * chunk);
-    for (auto& e : evs) LPMG_HIP(hipEventCreate(&e));
-  }
-  int rc = 0;
-  int iter_before = 0;
-  for (;;) {
-    for (int k = 0; k < chunk; ++k) {
-      if (c->prof) LPMG_HIP(hipEventRecord(evs[2 * k], c->stream));
-      cg_spmv_kernel<D, T><<<gs, kBS, 0, c->stream>>>(c->cptr, c->conns, val, p, q, N, c->partials, c->sc);
-      if (c->prof) LPMG_HIP(hipEventRecord(evs[2 * k + 1], c->stream));
-      cg_update_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, z, p, q, dinv, n, c->partials, c->sc);
-      cg_dir_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, c->sc);
+  c->sc_host_iter_launched = 0;
+  // One chunk = kChunk iterations; a chunk always starts at an even iteration,
+  // so the parity of the vector-step partial buffers is fixed per launch slot.
+  auto enqueue_chunk = [&](std::vector<hipEvent_t>* evs) -> int {
+    for (int k = 0; k < kChunk; ++k) {
+      if (evs) LPMG_HIP(hipEventRecord((*evs)[2 * k], c->stream));
+      launch_cg_spmv<D, T>(c, val, u, w, partS_raw);
+      if (evs) LPMG_HIP(hipEventRecord((*evs)[2 * k + 1], c->stream));
+      if (two_level) reduce_partials_kernel<<<kBS, kBS, 0, c->stream>>>(partS_raw, gs, partS, c->sc);
+      cg_vec_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, u, p, s, w, dinv, n, partV[k & 1], gv, partS, nS,
+                                                 partV[(k + 1) & 1], c->sc);
     }
     LPMG_HIP(hipGetLastError());
+    return 0;
+  };
+  std::vector<hipEvent_t> evs;
+  hipGraphExec_t* gexec = &c->cg_graph[sizeof(T) == 4 ? 1 : 0];
+  if (c->prof) {
+    evs.resize(2 * kChunk);
+    for (auto& e : evs) LPMG_HIP(hipEventCreate(&e));
+  } else if (!*gexec) {
+    // capture one chunk once per handle and precision; every pointer, size and
+    // launch shape in it is fixed for the handle's lifetime
+    hipGraph_t graph = nullptr;
+    LPMG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_chunk(nullptr);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (rc) return rc;
+    LPMG_HIP(e2);
+    LPMG_HIP(hipGraphInstantiate(gexec, graph, nullptr, nullptr, 0));
+    LPMG_HIP(hipGraphDestroy(graph));
+  }
+  for (;;) {
+    const int i0 = c->sc_host_iter_launched;
+    if (c->prof) {
+      if (int rc = enqueue_chunk(&evs)) return rc;
+    } else {
+      LPMG_HIP(hipGraphLaunch(*gexec, c->stream));
+    }
+    c->sc_host_iter_launched += kChunk;
     // the flag is read after the chunk; iterations past convergence were no-ops
     LPMG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     LPMG_HIP(hipStreamSynchronize(c->stream));
     if (c->prof) {
-      // only the first (iter - iter_before) SpMV launches of the chunk did work;
-      // the ones after convergence returned at once
-      const int worked = std::min(chunk, c->sc_host->iter - iter_before);
+      // SpMV launches S_i0 .. S_stop did work; the ones after the stop returned at once
+      const int worked = c->sc_host->done ? std::min(kChunk, c->sc_host->iter - i0 + 1) : kChunk;
       for (int k = 0; k < worked; ++k) {
         float ms = 0;
         LPMG_HIP(hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
@@ -268,14 +360,13 @@ int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* z, T* p, T* q, T* dinv
         c->prof_spmv_n += 1;
       }
     }
-    iter_before = c->sc_host->iter;
     if (c->sc_host->done) break;
   }
   if (c->prof)
     for (auto& e : evs) (void)hipEventDestroy(e);
   *iters = c->sc_host->iter;
   *rr_final = c->sc_host->rr;
-  return rc;
+  return 0;
 }
 
 }  // namespace
@@ -287,25 +378,16 @@ double spmv_bytes(const Context* c, int precision) {
 }
 
 int launch_spmv(Context* c, int precision, const void* x, void* y) {
-  const int gs = grid_for(c->N, 16);
   if (precision == 0) {
-    if (c->dim == 3)
-      spmv_kernel<3, double><<<gs, kBS, 0, c->stream>>>(c->cptr, c->conns, c->Kval, (const double*)x,
-                                                        (double*)y, c->N);
-    else
-      spmv_kernel<2, double><<<gs, kBS, 0, c->stream>>>(c->cptr, c->conns, c->Kval, (const double*)x,
-                                                        (double*)y, c->N);
+    if (c->dim == 3) launch_bsr<3, double>(c, c->Kval, (const double*)x, (double*)y);
+    else launch_bsr<2, double>(c, c->Kval, (const double*)x, (double*)y);
   } else {
     if (!c->Kval32) {
       set_error("FP32 stiffness not assembled");
       return -1;
     }
-    if (c->dim == 3)
-      spmv_kernel<3, float><<<gs, kBS, 0, c->stream>>>(c->cptr, c->conns, c->Kval32, (const float*)x,
-                                                       (float*)y, c->N);
-    else
-      spmv_kernel<2, float><<<gs, kBS, 0, c->stream>>>(c->cptr, c->conns, c->Kval32, (const float*)x,
-                                                       (float*)y, c->N);
+    if (c->dim == 3) launch_bsr<3, float>(c, c->Kval32, (const float*)x, (float*)y);
+    else launch_bsr<2, float>(c, c->Kval32, (const float*)x, (float*)y);
   }
   LPMG_HIP(hipGetLastError());
   return 0;
@@ -319,9 +401,9 @@ int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
       set_error("stiffness not assembled");
       return -1;
     }
-    rc = c->dim == 3 ? run_cg<3, double>(c, o, c->du, c->cg_r, c->cg_z, c->cg_p, c->cg_q, c->dinv,
+    rc = c->dim == 3 ? run_cg<3, double>(c, o, c->du, c->cg_r, c->cg_z, c->cg_p, c->cg_q, c->cg_w, c->dinv,
                                          c->Kval, iters, rr_final)
-                     : run_cg<2, double>(c, o, c->du, c->cg_r, c->cg_z, c->cg_p, c->cg_q, c->dinv,
+                     : run_cg<2, double>(c, o, c->du, c->cg_r, c->cg_z, c->cg_p, c->cg_q, c->cg_w, c->dinv,
                                          c->Kval, iters, rr_final);
     if (rc) return rc;
   } else {
@@ -329,9 +411,9 @@ int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
       set_error("FP32 stiffness not assembled");
       return -1;
     }
-    rc = c->dim == 3 ? run_cg<3, float>(c, o, c->f_x, c->f_r, c->f_z, c->f_p, c->f_q, c->f_dinv,
+    rc = c->dim == 3 ? run_cg<3, float>(c, o, c->f_x, c->f_r, c->f_z, c->f_p, c->f_q, c->f_w, c->f_dinv,
                                         c->Kval32, iters, rr_final)
-                     : run_cg<2, float>(c, o, c->f_x, c->f_r, c->f_z, c->f_p, c->f_q, c->f_dinv,
+                     : run_cg<2, float>(c, o, c->f_x, c->f_r, c->f_z, c->f_p, c->f_q, c->f_w, c->f_dinv,
                                         c->Kval32, iters, rr_final);
     if (rc) return rc;
     f32_to_f64_kernel<<<grid_for(n, 1024), 256, 0, c->stream>>>(c->f_x, c->du, n);

@@ -609,10 +609,13 @@ void calc_stiffness(Ctx* c) {
   c->IK[(size_t)d * N] = c->Kp[N] + 1;
 }
 
+// cblas_dnrm2: the Euclidean norm.  Accumulated in long double so the result is
+// the correctly rounded norm for any realistic n, whatever order another
+// implementation sums in (MKL's own order is closed).
 double dnrm2(const double* x, int64_t n) {
-  double s = 0;
-  for (int64_t t = 0; t < n; ++t) s += x[t] * x[t];
-  return std::sqrt(s);
+  long double s = 0;
+  for (int64_t t = 0; t < n; ++t) s += (long double)x[t] * x[t];
+  return (double)std::sqrt(s);
 }
 
 // Stiffness::updateStiffnessDispBC (stiffness.h:363-438)
