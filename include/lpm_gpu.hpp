@@ -256,7 +256,7 @@ class Solver {
          const int& p_niter, const double& p_tol)
       : max_NR_iter(p_niter), tol_NR_iter(p_tol), sol_mode(p_sol_mode), dumpFile(p_dumpFile), ass(p_ass), stiffness(p_ass) {}
 
-  void setMixedPrecision(bool on, double inner_rtol_sq = 1e-10) {
+  void setMixedPrecision(bool on, double inner_rtol_sq = 1e-4) {
     opts[4] = on ? 1 : 0;
     stiffness.precision = on ? 1 : 0;
     if (on) opts[1] = inner_rtol_sq;

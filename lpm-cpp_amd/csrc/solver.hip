@@ -38,9 +38,9 @@ namespace lpmg {
 namespace {
 
 constexpr int kBS = 256;         // vector kernels
-constexpr int kSpBS = 512;       // SpMV kernels: 8 wavefronts per workgroup
+constexpr int kSpBS = 1024;      // SpMV kernels: 16 wavefronts per workgroup
 constexpr int kMaxGrid = 1024;   // workgroups of the grid-strided vector kernels
-constexpr int kMaxParts = 8192;  // SpMV partials summed directly by the consumer
+constexpr int kMaxParts = 4096;  // SpMV partials summed directly by the consumer
 constexpr int kChunk = 16;       // CG iterations per launch batch (even)
 
 // y = A x for the rows of this wavefront (64 / LPR rows, LPR lanes each).  With
@@ -295,7 +295,7 @@ int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* u, T* p, T* s, T* w, T
            int* iters, double* rr_final) {
   const int N = c->N;
   const int64_t n = (int64_t)D * N;
-  const int gv = grid_for(n, kBS);
+  const int gv = grid_for(n, kBS * 4);
   const int gs = spmv_grid(c);
   const bool two_level = gs > kMaxParts;
   const int nS = two_level ? kBS : gs;
