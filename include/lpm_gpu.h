@@ -208,6 +208,121 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out);
 
 int lpmg_synchronize(lpmg_handle h);
 
+/* ---- Serving runtime ---------------------------------------------------
+ * No counterpart in the reference: the reference runs one Assembly per
+ * process (Particle::_ID is a process-wide counter, particle.h:17,79, and
+ * Stiffness indexes by it, stiffness.h:57-65).  A server runs many
+ * load-stepped quasi-static solves (SolverStatic::solveProblem,
+ * solver_static.h:64-95) at once on the current GPU:
+ *   - requests wait in a FIFO and are admitted while a worker slot is free
+ *     and the request's device-memory estimate fits the budget next to the
+ *     admitted ones; a request larger than the whole budget fails at submit
+ *     with -3 instead of waiting for ever;
+ *   - each admitted request owns a handle (its particle, bond, stiffness and
+ *     solver arrays stay in HBM for the whole solve) and a stream; its
+ *     forward pass, assembly and state updates run there, overlapping the
+ *     other requests' work;
+ *   - with batching on, the CG iterations of every request that is in its
+ *     linear solve run as fused launches: one SpMV, one vector and (large
+ *     systems) one reduction launch per iteration cover all of them, grouped
+ *     by dimension, precision and row width.  A request's arithmetic is the
+ *     same alone and in any mix: the result is bit-identical.
+ * Requests join or leave the fused launches every 16 iterations. */
+typedef struct lpmg_server_s* lpmg_server;
+
+/* One boundary condition of a load step: DispBC (boundary.h:9-25) or
+ * ForceBC (boundary.h:27-44) on one of the request's groups. */
+typedef struct lpmg_bc {
+  int kind;         /* 0 displacement, 1 force */
+  int group;        /* index into the request's groups */
+  int mode;         /* LPMG_RELATIVE or LPMG_ABSOLUTE */
+  int flag;         /* displacement: 'x', 'y' or 'z', as lpmg_disp_bc */
+  double value[3];  /* displacement: value[0] per step; force: fx, fy, fz */
+} lpmg_bc;
+
+/* A field of the final state copied into caller memory (lpmg_get_field) */
+typedef struct lpmg_out_field {
+  int field;         /* LPMG_F_* */
+  int64_t capacity;  /* doubles available at data; too small fails the request */
+  double* data;
+} lpmg_out_field;
+
+/* One quasi-static solve.  Every input array is copied by
+ * lpmg_server_submit; the output arrays must stay valid until
+ * lpmg_server_wait reports the request finished. */
+typedef struct lpmg_request {
+  int lattice;                 /* LPMG_SQUARE2D ... */
+  double radius;
+  int ptype;                   /* LPMG_ELASTIC ... */
+  int n;                       /* particles */
+  const double* xyz;           /* n*3 */
+  int64_t nbond;               /* > 0: bonds given (bond-file order); 0: bond search */
+  const int32_t* p1;
+  const int32_t* p2;
+  const int32_t* layer;
+  const int32_t* type;         /* n, or NULL for all 0 */
+  int model;                   /* material of every particle, as lpmg_set_material */
+  int n_params;
+  const double* params;
+  double nonlocal_ratio;       /* > 0: searchNonlocalNeighbors(ratio) */
+  int prepare_state;           /* 1: updateStateVar after the first updateForceState */
+  int n_groups;
+  const int32_t* group_size;   /* n_groups sizes */
+  const int32_t* group_idx;    /* the groups' particle indices, concatenated */
+  int n_bc;                    /* the boundary conditions of one load step ... */
+  const lpmg_bc* bc;
+  int n_steps;                 /* ... applied in each of n_steps steps */
+  int undamaged;               /* undamaged particle type */
+  int max_iter;                /* Newton iterations per pass */
+  double tol;                  /* Newton tolerance */
+  double opts[8];              /* solver options as lpmg_solve_linear */
+  int max_halvings;            /* step halvings allowed before the request fails */
+  int n_out;
+  const lpmg_out_field* out;
+  double* step_stats;          /* per converged step: Newton iterations, CG
+                                  iterations, damage passes; or NULL */
+  int step_stats_cap;          /* steps of room at step_stats */
+} lpmg_request;
+
+/* Statistics of a finished request, filled by lpmg_server_wait */
+enum {
+  LPMG_RS_STATUS = 0,    /* 0 or the error code */
+  LPMG_RS_STEPS = 1,     /* converged load steps, halved ones included */
+  LPMG_RS_HALVINGS = 2,
+  LPMG_RS_NEWTON = 3,    /* Newton iterations of all steps */
+  LPMG_RS_CG = 4,        /* CG iterations of all steps */
+  LPMG_RS_PASSES = 5,    /* damage passes of all steps */
+  LPMG_RS_QUEUE_MS = 6,  /* submit -> admission */
+  LPMG_RS_RUN_MS = 7,    /* admission -> finished */
+  LPMG_RS_TOTAL_MS = 8,  /* submit -> finished (latency) */
+  LPMG_RS_BYTES_EST = 9, /* the admission estimate */
+  LPMG_RS_BYTES = 10,    /* device memory the request's handle held */
+  LPMG_RS_N = 11
+};
+
+/* Device-memory estimate used for admission (an upper bound of what the
+ * request's handle allocates); < 0 on a malformed request. */
+int64_t lpmg_request_bytes(const lpmg_request* r);
+
+/* max_active: worker slots (requests in flight); mem_budget: bytes (<= 0:
+ * 90% of the free device memory at creation); batching: 1 fused CG launches,
+ * 0 every request runs its CG on its own stream. */
+int lpmg_server_create(int max_active, int64_t mem_budget, int batching, lpmg_server* out);
+/* Queue a request; *ticket identifies it.  -1 malformed, -3 larger than the
+ * budget (never admissible). */
+int lpmg_server_submit(lpmg_server s, const lpmg_request* r, int64_t* ticket);
+/* Wait up to timeout_s seconds (< 0: no limit) for a request: 0 finished and
+ * succeeded, 1 still running (timeout), < 0 finished with that error code
+ * (message in lpmg_last_error) or unknown ticket.  stats[LPMG_RS_N] is filled
+ * when finished; a finished ticket is released. */
+int lpmg_server_wait(lpmg_server s, int64_t ticket, double timeout_s, double* stats);
+/* out[0] queued, [1] running, [2] reserved bytes, [3] budget, [4] peak
+ * running, [5] peak reserved bytes, [6] finished, [7] fused CG chunks,
+ * [8] system-chunks in them, [9] largest batch (systems in one chunk). */
+int lpmg_server_info(lpmg_server s, double* out);
+/* Finish every queued and running request, then free the server. */
+void lpmg_server_destroy(lpmg_server s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,10 @@ struct lpmg_context {
   std::vector<double> Kn, Tv, crit, nlL, mE, mMu, mSy, mXi, mH, mA, k0, k1, dthr, ctr;
 };
 
+namespace lpmg {
+Context* handle_context(lpmg_context* h) { return &h->c; }
+}  // namespace lpmg
+
 namespace {
 
 #define API_HIP(call)                                  \
@@ -154,7 +158,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   Context* c = &h->c;
   const size_t N = c->N, d = c->dim;
   API_HIP(hipGetDevice(&c->device));
-  API_HIP(hipStreamCreate(&c->stream));
+  API_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   int rc = 0;
   rc |= dalloc(c, &c->xyz, N);
   rc |= dalloc(c, &c->xyz0, N);
@@ -236,9 +240,11 @@ int free_context(lpmg_context* h) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (hipGraphExec_t& g : c->cg_graph)
     if (g) (void)hipGraphExecDestroy(g);
-  for (auto& g : c->groups) (void)hipFree(g.idx);
-  for (void* p : c->allocs) (void)hipFree(p);
+  for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->sc_host) (void)hipHostFree(c->sc_host);
+  if (c->cg_ready) (void)hipEventDestroy(c->cg_ready);
+  if (c->cg_released) (void)hipEventDestroy(c->cg_released);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   return 0;
 }
@@ -446,18 +452,18 @@ int lpmg_sizes(lpmg_handle h, int64_t* out) {
 int lpmg_export_bonds(lpmg_handle h, int32_t* bptr, int32_t* nb0, int32_t* p2) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (bptr) API_HIP(hipMemcpy(bptr, c->bptr, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
-  if (nb0) API_HIP(hipMemcpy(nb0, c->nb0, sizeof(int) * c->N, hipMemcpyDeviceToHost));
-  if (p2) API_HIP(hipMemcpy(p2, c->p2, sizeof(int) * c->NB, hipMemcpyDeviceToHost));
+  if (bptr) API_HIP(copy_sync(c, bptr, c->bptr, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
+  if (nb0) API_HIP(copy_sync(c, nb0, c->nb0, sizeof(int) * c->N, hipMemcpyDeviceToHost));
+  if (p2) API_HIP(copy_sync(c, p2, c->p2, sizeof(int) * c->NB, hipMemcpyDeviceToHost));
   return 0;
 }
 
 int lpmg_export_conns(lpmg_handle h, int32_t* cptr, int32_t* conns, int32_t* ncl) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (cptr) API_HIP(hipMemcpy(cptr, c->cptr, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
-  if (conns) API_HIP(hipMemcpy(conns, c->conns, sizeof(int) * c->nnzb, hipMemcpyDeviceToHost));
-  if (ncl) API_HIP(hipMemcpy(ncl, c->ncl, sizeof(int) * c->N, hipMemcpyDeviceToHost));
+  if (cptr) API_HIP(copy_sync(c, cptr, c->cptr, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
+  if (conns) API_HIP(copy_sync(c, conns, c->conns, sizeof(int) * c->nnzb, hipMemcpyDeviceToHost));
+  if (ncl) API_HIP(copy_sync(c, ncl, c->ncl, sizeof(int) * c->N, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -465,30 +471,30 @@ int lpmg_export_csr(lpmg_handle h, int64_t* Kp, int64_t* IK, int64_t* JK, double
   CHECK_HANDLE(h);
   Context* c = &h->c;
   API_HIP(hipStreamSynchronize(c->stream));
-  if (Kp) API_HIP(hipMemcpy(Kp, c->Kp, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost));
+  if (Kp) API_HIP(copy_sync(c, Kp, c->Kp, sizeof(int64_t) * (c->N + 1), hipMemcpyDeviceToHost));
   if (!IK && !JK && !K) return 0;
   int64_t *dIK = nullptr, *dJK = nullptr;
   double* dK = nullptr;
   const size_t nIK = (size_t)c->dim * c->N + 1, nnz = (size_t)c->nnz_upper;
-  API_HIP(hipMalloc(&dIK, sizeof(int64_t) * nIK));
-  API_HIP(hipMalloc(&dJK, sizeof(int64_t) * nnz));
-  API_HIP(hipMalloc(&dK, sizeof(double) * nnz));
+  API_HIP(hipMallocAsync((void**)&dIK, sizeof(int64_t) * nIK, c->stream));
+  API_HIP(hipMallocAsync((void**)&dJK, sizeof(int64_t) * nnz, c->stream));
+  API_HIP(hipMallocAsync((void**)&dK, sizeof(double) * nnz, c->stream));
   int rc = launch_export_csr(c, nullptr, dIK, dJK, dK);
   if (!rc) {
     API_HIP(hipStreamSynchronize(c->stream));
-    if (IK) API_HIP(hipMemcpy(IK, dIK, sizeof(int64_t) * nIK, hipMemcpyDeviceToHost));
-    if (JK) API_HIP(hipMemcpy(JK, dJK, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost));
-    if (K) API_HIP(hipMemcpy(K, dK, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    if (IK) API_HIP(copy_sync(c, IK, dIK, sizeof(int64_t) * nIK, hipMemcpyDeviceToHost));
+    if (JK) API_HIP(copy_sync(c, JK, dJK, sizeof(int64_t) * nnz, hipMemcpyDeviceToHost));
+    if (K) API_HIP(copy_sync(c, K, dK, sizeof(double) * nnz, hipMemcpyDeviceToHost));
   }
-  (void)hipFree(dIK);
-  (void)hipFree(dJK);
-  (void)hipFree(dK);
+  (void)hipFreeAsync(dIK, c->stream);
+  (void)hipFreeAsync(dJK, c->stream);
+  (void)hipFreeAsync(dK, c->stream);
   return rc;
 }
 
 int lpmg_set_types(lpmg_handle h, const int32_t* type) {
   CHECK_HANDLE(h);
-  API_HIP(hipMemcpy(h->c.type, type, sizeof(int) * h->c.N, hipMemcpyHostToDevice));
+  API_HIP(copy_sync(&h->c, h->c.type, type, sizeof(int) * h->c.N, hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -570,16 +576,16 @@ int lpmg_spmv(lpmg_handle h, int precision, const double* x, double* y) {
   if (precision == 1 && !c->assembled32) { set_error("FP32 stiffness not assembled"); return -1; }
   double *dx = nullptr, *dy = nullptr;
   float *fx = nullptr, *fy = nullptr;
-  API_HIP(hipMalloc(&dx, sizeof(double) * n));
-  API_HIP(hipMalloc(&dy, sizeof(double) * n));
+  API_HIP(hipMallocAsync((void**)&dx, sizeof(double) * n, c->stream));
+  API_HIP(hipMallocAsync((void**)&dy, sizeof(double) * n, c->stream));
   API_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   int rc;
   const int nb = (int)((n + 255) / 256);
   if (precision == 0) {
     rc = launch_spmv(c, 0, dx, dy);
   } else {
-    API_HIP(hipMalloc(&fx, sizeof(float) * n));
-    API_HIP(hipMalloc(&fy, sizeof(float) * n));
+    API_HIP(hipMallocAsync((void**)&fx, sizeof(float) * n, c->stream));
+    API_HIP(hipMallocAsync((void**)&fy, sizeof(float) * n, c->stream));
     d2f_kernel<<<nb, 256, 0, c->stream>>>(dx, fx, n);
     rc = launch_spmv(c, 1, fx, fy);
     f2d_kernel<<<nb, 256, 0, c->stream>>>(fy, dy, n);
@@ -588,9 +594,9 @@ int lpmg_spmv(lpmg_handle h, int precision, const double* x, double* y) {
     API_HIP(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     API_HIP(hipStreamSynchronize(c->stream));
   }
-  (void)hipFree(dx); (void)hipFree(dy);
-  if (fx) (void)hipFree(fx);
-  if (fy) (void)hipFree(fy);
+  (void)hipFreeAsync(dx, c->stream); (void)hipFreeAsync(dy, c->stream);
+  if (fx) (void)hipFreeAsync(fx, c->stream);
+  if (fy) (void)hipFreeAsync(fy, c->stream);
   return rc;
 }
 
@@ -643,8 +649,8 @@ int lpmg_define_group(lpmg_handle h, int n, const int32_t* idx) {
     if (idx[t] < 0 || idx[t] >= c->N) { set_error("group index out of range"); return -1; }
   Group g;
   g.n = n;
-  API_HIP(hipMalloc(&g.idx, sizeof(int) * std::max(n, 1)));
-  if (n) API_HIP(hipMemcpy(g.idx, idx, sizeof(int) * n, hipMemcpyHostToDevice));
+  if (dalloc(c, &g.idx, (size_t)std::max(n, 1))) return -3;
+  if (n) API_HIP(copy_sync(c, g.idx, idx, sizeof(int) * n, hipMemcpyHostToDevice));
   c->groups.push_back(g);
   return (int)c->groups.size() - 1;
 }
@@ -798,7 +804,7 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   if (which == 1 && !c->assembled32) { set_error("FP32 stiffness not assembled"); return -1; }
   const int64_t ncopy = (int64_t)c->nnzb * c->dim * c->dim;
   double* copy_dst = nullptr;
-  if (which == 4) API_HIP(hipMalloc(&copy_dst, sizeof(double) * ncopy));
+  if (which == 4) API_HIP(hipMallocAsync((void**)&copy_dst, sizeof(double) * ncopy, c->stream));
   hipEvent_t e0, e1;
   API_HIP(hipEventCreate(&e0));
   API_HIP(hipEventCreate(&e1));
@@ -817,7 +823,7 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   API_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (copy_dst) (void)hipFree(copy_dst);
+  if (copy_dst) (void)hipFreeAsync(copy_dst, c->stream);
   out[0] = ms / reps;
   out[1] = which == 0 ? spmv_bytes(c, 0) : which == 1 ? spmv_bytes(c, 1)
            : which == 2 ? assembly_bytes(c) : which == 3 ? forward_bytes(c) : 16.0 * (double)ncopy;

@@ -52,6 +52,37 @@ struct Group {
   int n = 0;
 };
 
+// One system of a CG launch: its operator, vectors, reduction buffers, scalars
+// and launch shape.  Device-visible (the fused launches read a table of them).
+struct CgSys {
+  const int* cptr;
+  const int* conns;
+  const void* val;
+  const void* dinv;
+  void *x, *r, *u, *p, *s, *w;
+  double *partV0, *partV1, *partS_raw, *partS;
+  Scalars* sc;
+  int64_t n;                      // dim * N unknowns
+  int N, gs, gv, nS, two_level;   // block rows, SpMV / vector workgroups, SpMV partials
+  int sp_begin, red_begin, vec_begin;  // first workgroup in each fused grid
+  int cls;                        // batch class: cg_class(dim, fp32, lanes per row)
+};
+
+// The systems of one batch class in a fused launch (table in device memory)
+struct CgBatchClass {
+  int cls = 0;
+  const CgSys* tab = nullptr;
+  int nsys = 0, sp_blocks = 0, red_blocks = 0, vec_blocks = 0;
+};
+
+constexpr int kCgChunk = 16;     // CG iterations per launch batch (even)
+constexpr int kCgClasses = 8;
+
+struct Context;
+// Runs the iterations of a CG whose start cg_begin has enqueued on c->stream;
+// returns after ordering c->stream behind the system's last iteration.
+typedef int (*CgExec)(void* arg, Context* c, const CgSys& q);
+
 struct Context {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -112,6 +143,11 @@ struct Context {
   // captured CG chunk per precision (FP64, FP32)
   hipGraphExec_t cg_graph[2] = {nullptr, nullptr};
   int sc_host_iter_launched = 0;
+  // CG executor (serving runtime); null: the context's own stream
+  CgExec cg_exec = nullptr;
+  void* cg_exec_arg = nullptr;
+  hipEvent_t cg_ready = nullptr, cg_released = nullptr;  // CG start / end of its last fused launch
+  size_t bytes = 0;  // device memory held by the context
 };
 
 // ---- helpers (api.hip)
@@ -123,16 +159,31 @@ int hip_fail(hipError_t e, const char* what);
     if (_e != hipSuccess) return hip_fail(_e, #call);    \
   } while (0)
 
+// Stream-ordered allocation on the context's stream: freeing it later (also
+// stream-ordered) does not synchronize the device, so handles come and go
+// while other handles' work runs.
 template <typename T>
 int dalloc(Context* c, T** p, size_t n) {
   if (n == 0) n = 1;
   void* q = nullptr;
-  hipError_t e = hipMalloc(&q, n * sizeof(T));
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  hipError_t e = hipMallocAsync(&q, n * sizeof(T), c->stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
   c->allocs.push_back(q);
+  c->bytes += n * sizeof(T);
   *p = static_cast<T*>(q);
   return 0;
 }
+
+// Copy on the context's stream and wait for it.  Never the null stream: that
+// would also wait for every other handle's work on the device.
+inline hipError_t copy_sync(Context* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c->stream);
+}
+
+// the context behind a C-ABI handle (api.hip)
+Context* handle_context(struct lpmg_context* h);
 
 // ---- setup.hip
 int build_bonds_from_search(Context* c, const double* host_xyz);
@@ -163,6 +214,10 @@ struct SolveOpts {
   int chunk = 16;       // unused: the chunk length is fixed (graph capture)
 };
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
+int cg_class(int dim, int fp32, int lpr);
+void cg_batch_layout(CgSys* tab, int nsys, CgBatchClass* b);
+int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int nclass);
+int cg_batch_flags(hipStream_t st, const CgSys* tab, int nsys, int* out);
 int launch_spmv(Context* c, int precision, const void* x, void* y);
 double spmv_bytes(const Context* c, int precision);
 

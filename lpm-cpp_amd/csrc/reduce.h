@@ -46,14 +46,19 @@ __device__ inline Vec<NV> block_sum(Vec<NV> a) {
   return t;
 }
 
-// This workgroup's partial goes to part[NV*blockIdx.x ...].  Every thread of the
+// Workgroup `blk`'s partial goes to part[NV*blk ...].  Every thread of the
 // workgroup must call it.
 template <int BS, int NV>
-__device__ inline void write_partial(double* __restrict__ part, Vec<NV> a) {
+__device__ inline void write_partial_at(double* __restrict__ part, int blk, Vec<NV> a) {
   a = block_sum<BS, NV>(a);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int k = 0; k < NV; ++k) part[(size_t)NV * blockIdx.x + k] = a.v[k];
+    for (int k = 0; k < NV; ++k) part[(size_t)NV * blk + k] = a.v[k];
+}
+
+template <int BS, int NV>
+__device__ inline void write_partial(double* __restrict__ part, Vec<NV> a) {
+  write_partial_at<BS, NV>(part, blockIdx.x, a);
 }
 
 // Sum of n partials written by an earlier kernel, in a fixed order; the same

@@ -432,7 +432,7 @@ int check_err(Context* c, const char* what) {
   LPMG_HIP(hipStreamSynchronize(c->stream));
   if (h) {
     set_error(std::string(what) + " failed (device flag " + std::to_string(h) + ")");
-    LPMG_HIP(hipMemset(c->err_flag, 0, sizeof(int)));
+    LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
     return -1;
   }
   return 0;
@@ -556,8 +556,8 @@ int build_topology(Context* c) {
   LPMG_HIP(hipStreamSynchronize(c->stream));
   // maxima (host copies are small)
   std::vector<int> hb(N + 1), hc(N + 1);
-  LPMG_HIP(hipMemcpy(hb.data(), c->bptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
-  LPMG_HIP(hipMemcpy(hc.data(), c->cptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
+  LPMG_HIP(copy_sync(c, hb.data(), c->bptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
+  LPMG_HIP(copy_sync(c, hc.data(), c->cptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
   c->max_nb = c->max_nconn = 0;
   for (int i = 0; i < N; ++i) {
     c->max_nb = std::max(c->max_nb, hb[i + 1] - hb[i]);
@@ -611,9 +611,9 @@ int search_nonlocal(Context* c, double ratio) {
   std::vector<double> hx(3 * (size_t)N), hl(N);
   {
     std::vector<double4> h4(N);
-    LPMG_HIP(hipMemcpy(h4.data(), c->xyz, sizeof(double4) * N, hipMemcpyDeviceToHost));
+    LPMG_HIP(copy_sync(c, h4.data(), c->xyz, sizeof(double4) * N, hipMemcpyDeviceToHost));
     for (int i = 0; i < N; ++i) { hx[3 * i] = h4[i].x; hx[3 * i + 1] = h4[i].y; hx[3 * i + 2] = h4[i].z; }
-    LPMG_HIP(hipMemcpy(hl.data(), c->nlL, sizeof(double) * N, hipMemcpyDeviceToHost));
+    LPMG_HIP(copy_sync(c, hl.data(), c->nlL, sizeof(double) * N, hipMemcpyDeviceToHost));
   }
   double maxL = 0;
   for (double v : hl) maxL = std::max(maxL, v);
@@ -631,7 +631,7 @@ int search_nonlocal(Context* c, double ratio) {
   int total = 0;
   if (int rc = exclusive_scan_i32(c, cnt, c->nlptr, N, &total)) return rc;
   if (c->nlidx) {
-    LPMG_HIP(hipFree(c->nlidx));
+    LPMG_HIP(hipFreeAsync(c->nlidx, c->stream));
     c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), (void*)c->nlidx));
     c->nlidx = nullptr;
   }

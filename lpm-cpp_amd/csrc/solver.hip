@@ -27,6 +27,13 @@
 // (dpar[0], dpar[1], solver.h:312-313), at most maxit (ipar[4] = n) iterations.
 // The host launches `chunk` iterations between reads of the convergence flag;
 // kernels of iterations after convergence return immediately.
+//
+// The same iteration bodies also run as fused launches over many systems (the
+// serving runtime): one launch per kernel kind covers every system of a batch
+// class (dimension, precision, lanes per row); workgroup g finds its system by
+// a binary search over the systems' first workgroups and then computes exactly
+// what that system's own launch would compute in its workgroup g - first, so a
+// result does not depend on the batch it ran in.
 #include <algorithm>
 #include <vector>
 
@@ -41,18 +48,18 @@ constexpr int kBS = 256;         // vector kernels
 constexpr int kSpBS = 1024;      // SpMV kernels: 16 wavefronts per workgroup
 constexpr int kMaxGrid = 1024;   // workgroups of the grid-strided vector kernels
 constexpr int kMaxParts = 4096;  // SpMV partials summed directly by the consumer
-constexpr int kChunk = 16;       // CG iterations per launch batch (even)
 
-// y = A x for the rows of this wavefront (64 / LPR rows, LPR lanes each).  With
-// DOT, returns this lane's share of x.y (nonzero only in the row's first b lanes).
+// y = A x for the rows of wavefronts of workgroup `blk` (64 / LPR rows per
+// wavefront, LPR lanes each).  With DOT, returns this lane's share of x.y
+// (nonzero only in the row's first b lanes).
 template <int D, typename T, int LPR, bool DOT>
-__device__ __forceinline__ double bsr_rows(const int* __restrict__ cptr, const int* __restrict__ conns,
+__device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            T* __restrict__ y, int N) {
   constexpr int DD = D * D;
   constexpr int RPW = 64 / LPR;
   const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
-  const int wave = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
   const int row = wave * RPW + sub;
   const bool live = row < N;
   const int rw = live ? row : N - 1;  // dead rows shadow the last row: all lanes stay in the shuffles
@@ -99,33 +106,37 @@ template <int D, typename T, int LPR>
 __global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
                                                      const T* __restrict__ val, const T* __restrict__ x,
                                                      T* __restrict__ y, int N) {
-  bsr_rows<D, T, LPR, false>(cptr, conns, val, x, y, N);
+  bsr_rows<D, T, LPR, false>(blockIdx.x, cptr, conns, val, x, y, N);
 }
 
-// SpMV of one CG iteration: w = A u and the per-workgroup partials of (w, u)
+// ---- one CG iteration, written for workgroup `blk` of `nblk` of one system.
+// The same bodies run in the per-system launches and in the fused launches
+// over several systems, so a system's arithmetic does not depend on which
+// other systems share its launches.
+
+// SpMV: w = A u and the per-workgroup partials of (w, u).  `done` is written
+// only by the vector step, never during this launch.
 template <int D, typename T, int LPR>
-__global__ void __launch_bounds__(kSpBS) cg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
-                                                        const T* __restrict__ val, const T* __restrict__ u,
-                                                        T* __restrict__ w, int N, double* __restrict__ partS,
-                                                        Scalars* sc) {
+__device__ __forceinline__ void cg_spmv_body(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
+                                             const T* __restrict__ val, const T* __restrict__ u,
+                                             T* __restrict__ w, int N, double* __restrict__ partS, Scalars* sc) {
   if (sc->done) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // iteration index of the next vector step
+  if (blk == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // iteration index of the next vector step
   Vec<1> a;
-  a.v[0] = bsr_rows<D, T, LPR, true>(cptr, conns, val, u, w, N);
-  write_partial<kSpBS, 1>(partS, a);
+  a.v[0] = bsr_rows<D, T, LPR, true>(blk, cptr, conns, val, u, w, N);
+  write_partial_at<kSpBS, 1>(partS, blk, a);
 }
 
-// Second level for large grids: n partials -> gridDim.x partials (contiguous slices)
-__global__ void __launch_bounds__(kBS) reduce_partials_kernel(const double* __restrict__ in, int n,
-                                                              double* __restrict__ out,
-                                                              const Scalars* __restrict__ sc) {
+// Second level for large grids: n partials -> nblk partials (contiguous slices)
+__device__ __forceinline__ void reduce_partials_body(int blk, int nblk, const double* __restrict__ in, int n,
+                                                     double* __restrict__ out, const Scalars* __restrict__ sc) {
   if (sc->done) return;
-  const int per = (n + gridDim.x - 1) / gridDim.x;
-  const int lo = blockIdx.x * per, hi = min(n, lo + per);
+  const int per = (n + nblk - 1) / nblk;
+  const int lo = blk * per, hi = min(n, lo + per);
   Vec<1> a;
   a.v[0] = 0;
   for (int b = lo + threadIdx.x; b < hi; b += kBS) a.v[0] += in[b];
-  write_partial<kBS, 1>(out, a);
+  write_partial_at<kBS, 1>(out, blk, a);
 }
 
 // The scalars of iteration i from the partials: gamma = (r,u), delta = (w,u), rr = (r,r)
@@ -156,18 +167,25 @@ __device__ __forceinline__ CGStep cg_scalars(const double* __restrict__ partV, i
 }
 
 // Vector step of iteration i: p = u + beta p; s = w + beta s; x += alpha p;
-// r -= alpha s; u = M r; partials of (r,u), (r,r)
+// r -= alpha s; u = M r; partials of (r,u), (r,r).  Workgroup 0 sets `done`
+// during this launch, so each workgroup reads the flag once, through LDS: all
+// wavefronts of a workgroup take the same branch into the barriers below.  A
+// workgroup that still sees 0 reaches the same stop decision from the same
+// partials and returns without writing.
 template <typename T>
-__global__ void __launch_bounds__(kBS) cg_vec_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
-                                                     T* __restrict__ p, T* __restrict__ s,
-                                                     const T* __restrict__ w, const T* __restrict__ dinv,
-                                                     int64_t n, const double* __restrict__ partV_in, int nV,
-                                                     const double* __restrict__ partS, int nS,
-                                                     double* __restrict__ partV_out, Scalars* sc) {
-  if (sc->done) return;
+__device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x, T* __restrict__ r,
+                                            T* __restrict__ u, T* __restrict__ p, T* __restrict__ s,
+                                            const T* __restrict__ w, const T* __restrict__ dinv, int64_t n,
+                                            const double* __restrict__ partV_in, int nV,
+                                            const double* __restrict__ partS, int nS,
+                                            double* __restrict__ partV_out, Scalars* sc) {
+  __shared__ int done_s;
+  if (threadIdx.x == 0) done_s = sc->done;
+  __syncthreads();
+  if (done_s) return;
   const int i = sc->icur;
   const CGStep st = cg_scalars(partV_in, nV, partS, nS, sc, i);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     if (i == 0) {
       sc->rr0 = st.rr;
       sc->thresh = sc->rtol * st.rr + sc->atol;
@@ -186,7 +204,7 @@ __global__ void __launch_bounds__(kBS) cg_vec_kernel(T* __restrict__ x, T* __res
   if (st.stop) return;  // every workgroup reaches the same decision
   const T alpha = (T)st.alpha, beta = (T)st.beta;
   double ru = 0, rr = 0;
-  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
+  for (int64_t k = (int64_t)blk * kBS + threadIdx.x; k < n; k += (int64_t)nblk * kBS) {
     const T pk = u[k] + beta * p[k];
     const T sk = w[k] + beta * s[k];
     p[k] = pk;
@@ -202,7 +220,75 @@ __global__ void __launch_bounds__(kBS) cg_vec_kernel(T* __restrict__ x, T* __res
   Vec<2> a;
   a.v[0] = ru;
   a.v[1] = rr;
-  write_partial<kBS, 2>(partV_out, a);
+  write_partial_at<kBS, 2>(partV_out, blk, a);
+}
+
+// ---- per-system launches
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) cg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
+                                                        const T* __restrict__ val, const T* __restrict__ u,
+                                                        T* __restrict__ w, int N, double* __restrict__ partS,
+                                                        Scalars* sc) {
+  cg_spmv_body<D, T, LPR>(blockIdx.x, cptr, conns, val, u, w, N, partS, sc);
+}
+
+__global__ void __launch_bounds__(kBS) reduce_partials_kernel(const double* __restrict__ in, int n,
+                                                              double* __restrict__ out,
+                                                              const Scalars* __restrict__ sc) {
+  reduce_partials_body(blockIdx.x, gridDim.x, in, n, out, sc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBS) cg_vec_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
+                                                     T* __restrict__ p, T* __restrict__ s,
+                                                     const T* __restrict__ w, const T* __restrict__ dinv,
+                                                     int64_t n, const double* __restrict__ partV_in, int nV,
+                                                     const double* __restrict__ partS, int nS,
+                                                     double* __restrict__ partV_out, Scalars* sc) {
+  cg_vec_body<T>(blockIdx.x, gridDim.x, x, r, u, p, s, w, dinv, n, partV_in, nV, partS, nS, partV_out, sc);
+}
+
+// ---- fused launches over the systems of a CgSys table.  Workgroup g belongs
+// to the last system whose first workgroup (of this kernel's kind) is <= g.
+template <int KIND>
+__device__ __forceinline__ int cg_sys_of(const CgSys* __restrict__ tab, int nsys, int g) {
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int b = KIND == 0 ? tab[mid].sp_begin : KIND == 1 ? tab[mid].red_begin : tab[mid].vec_begin;
+    if (b <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) fused_cg_spmv_kernel(const CgSys* __restrict__ tab, int nsys) {
+  const CgSys& q = tab[cg_sys_of<0>(tab, nsys, blockIdx.x)];
+  cg_spmv_body<D, T, LPR>(blockIdx.x - q.sp_begin, q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
+                          q.partS_raw, q.sc);
+}
+
+__global__ void __launch_bounds__(kBS) fused_reduce_partials_kernel(const CgSys* __restrict__ tab, int nsys) {
+  const CgSys& q = tab[cg_sys_of<1>(tab, nsys, blockIdx.x)];
+  if (!q.two_level) return;  // systems without a second level own no workgroups here
+  reduce_partials_body(blockIdx.x - q.red_begin, kBS, q.partS_raw, q.gs, q.partS, q.sc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBS) fused_cg_vec_kernel(const CgSys* __restrict__ tab, int nsys, int slot) {
+  const CgSys& q = tab[cg_sys_of<2>(tab, nsys, blockIdx.x)];
+  cg_vec_body<T>(blockIdx.x - q.vec_begin, q.gv, (T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
+                 (const T*)q.dinv, q.n, slot ? q.partV1 : q.partV0, q.gv, q.partS, q.nS,
+                 slot ? q.partV0 : q.partV1, q.sc);
+}
+
+// done flag and iteration count of every system of the table
+__global__ void cg_flags_kernel(const CgSys* __restrict__ tab, int nsys, int* __restrict__ out) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < nsys; k += gridDim.x * blockDim.x) {
+    out[2 * k] = tab[k].sc->done;
+    out[2 * k + 1] = tab[k].sc->iter;
+  }
 }
 
 // Start: x = 0, r = b, u = M b, p = s = 0; partials of (r,u), (r,r)
@@ -282,42 +368,75 @@ void launch_bsr(Context* c, const T* val, const T* x, T* y) {
 }
 
 template <int D, typename T>
-void launch_cg_spmv(Context* c, const T* val, const T* u, T* w, double* partS) {
-  const int g = spmv_grid(c);
-  if (lanes_per_row(c) == 32)
-    cg_spmv_kernel<D, T, 32><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, u, w, c->N, partS, c->sc);
+void launch_cg_spmv(hipStream_t st, const CgSys& q, int lpr) {
+  if (lpr == 32)
+    cg_spmv_kernel<D, T, 32><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
+                                                     q.partS_raw, q.sc);
   else
-    cg_spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, u, w, c->N, partS, c->sc);
+    cg_spmv_kernel<D, T, 64><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
+                                                     q.partS_raw, q.sc);
 }
 
+// Vectors of the CG in the given precision (FP32: the mixed-precision copies)
+template <typename T>
+void cg_vectors(Context* c, CgSys* q);
+template <>
+void cg_vectors<double>(Context* c, CgSys* q) {
+  q->val = c->Kval; q->dinv = c->dinv;
+  q->x = c->du; q->r = c->cg_r; q->u = c->cg_z; q->p = c->cg_p; q->s = c->cg_q; q->w = c->cg_w;
+}
+template <>
+void cg_vectors<float>(Context* c, CgSys* q) {
+  q->val = c->Kval32; q->dinv = c->f_dinv;
+  q->x = c->f_x; q->r = c->f_r; q->u = c->f_z; q->p = c->f_p; q->s = c->f_q; q->w = c->f_w;
+}
+
+// Fill the system's descriptor and enqueue the start of its solve (Jacobi
+// diagonal, scalars, x = 0, r = b and the first partials) on its stream.
 template <int D, typename T>
-int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* u, T* p, T* s, T* w, T* dinv, const T* val,
-           int* iters, double* rr_final) {
-  const int N = c->N;
-  const int64_t n = (int64_t)D * N;
-  const int gv = grid_for(n, kBS * 4);
-  const int gs = spmv_grid(c);
-  const bool two_level = gs > kMaxParts;
-  const int nS = two_level ? kBS : gs;
-  double* partV[2] = {c->partials, c->partials + 2 * kMaxGrid};
-  double* partS_raw = c->partials + 4 * kMaxGrid;
-  double* partS = two_level ? c->partials + 4 * kMaxGrid + gs : partS_raw;
-  dinv_kernel<D, T><<<(N + 255) / 256, 256, 0, c->stream>>>(c->dpos, val, N, o.precond, dinv);
-  const int64_t maxit = o.maxit > 0 ? o.maxit : n;
+int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
+  *q = CgSys{};
+  cg_vectors<T>(c, q);
+  q->cptr = c->cptr;
+  q->conns = c->conns;
+  q->N = c->N;
+  q->n = (int64_t)D * c->N;
+  q->gv = grid_for(q->n, kBS * 4);
+  q->gs = spmv_grid(c);
+  q->two_level = q->gs > kMaxParts;
+  q->nS = q->two_level ? kBS : q->gs;
+  q->partV0 = c->partials;
+  q->partV1 = c->partials + 2 * kMaxGrid;
+  q->partS_raw = c->partials + 4 * kMaxGrid;
+  q->partS = q->two_level ? q->partS_raw + q->gs : q->partS_raw;
+  q->sc = c->sc;
+  q->cls = cg_class(D, sizeof(T) == 4, lanes_per_row(c));
+  dinv_kernel<D, T><<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->dpos, (const T*)q->val, c->N, o.precond,
+                                                              (T*)q->dinv);
+  const int64_t maxit = o.maxit > 0 ? o.maxit : q->n;
   cg_setup_kernel<<<1, 64, 0, c->stream>>>(c->sc, o.rtol, o.atol, maxit);
-  cg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, u, p, s, dinv, n, partV[0]);
+  cg_init_kernel<T><<<q->gv, kBS, 0, c->stream>>>(c->residual, (T*)q->x, (T*)q->r, (T*)q->u, (T*)q->p, (T*)q->s,
+                                                  (const T*)q->dinv, q->n, q->partV0);
   LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+// One system, its own stream: chunks of kCgChunk iterations between reads of
+// the convergence flag; a chunk always starts at an even iteration, so the
+// parity of the vector-step partial buffers is fixed per launch slot.
+template <int D, typename T>
+int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
+  const int lpr = lanes_per_row(c);
   c->sc_host_iter_launched = 0;
-  // One chunk = kChunk iterations; a chunk always starts at an even iteration,
-  // so the parity of the vector-step partial buffers is fixed per launch slot.
   auto enqueue_chunk = [&](std::vector<hipEvent_t>* evs) -> int {
-    for (int k = 0; k < kChunk; ++k) {
+    for (int k = 0; k < kCgChunk; ++k) {
       if (evs) LPMG_HIP(hipEventRecord((*evs)[2 * k], c->stream));
-      launch_cg_spmv<D, T>(c, val, u, w, partS_raw);
+      launch_cg_spmv<D, T>(c->stream, q, lpr);
       if (evs) LPMG_HIP(hipEventRecord((*evs)[2 * k + 1], c->stream));
-      if (two_level) reduce_partials_kernel<<<kBS, kBS, 0, c->stream>>>(partS_raw, gs, partS, c->sc);
-      cg_vec_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, u, p, s, w, dinv, n, partV[k & 1], gv, partS, nS,
-                                                 partV[(k + 1) & 1], c->sc);
+      if (q.two_level) reduce_partials_kernel<<<kBS, kBS, 0, c->stream>>>(q.partS_raw, q.gs, q.partS, q.sc);
+      cg_vec_kernel<T><<<q.gv, kBS, 0, c->stream>>>((T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
+                                                    (const T*)q.dinv, q.n, (k & 1) ? q.partV1 : q.partV0, q.gv,
+                                                    q.partS, q.nS, (k & 1) ? q.partV0 : q.partV1, q.sc);
     }
     LPMG_HIP(hipGetLastError());
     return 0;
@@ -325,7 +444,7 @@ int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* u, T* p, T* s, T* w, T
   std::vector<hipEvent_t> evs;
   hipGraphExec_t* gexec = &c->cg_graph[sizeof(T) == 4 ? 1 : 0];
   if (c->prof) {
-    evs.resize(2 * kChunk);
+    evs.resize(2 * kCgChunk);
     for (auto& e : evs) LPMG_HIP(hipEventCreate(&e));
   } else if (!*gexec) {
     // capture one chunk once per handle and precision; every pointer, size and
@@ -346,13 +465,13 @@ int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* u, T* p, T* s, T* w, T
     } else {
       LPMG_HIP(hipGraphLaunch(*gexec, c->stream));
     }
-    c->sc_host_iter_launched += kChunk;
+    c->sc_host_iter_launched += kCgChunk;
     // the flag is read after the chunk; iterations past convergence were no-ops
     LPMG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     LPMG_HIP(hipStreamSynchronize(c->stream));
     if (c->prof) {
       // SpMV launches S_i0 .. S_stop did work; the ones after the stop returned at once
-      const int worked = c->sc_host->done ? std::min(kChunk, c->sc_host->iter - i0 + 1) : kChunk;
+      const int worked = c->sc_host->done ? std::min(kCgChunk, c->sc_host->iter - i0 + 1) : kCgChunk;
       for (int k = 0; k < worked; ++k) {
         float ms = 0;
         LPMG_HIP(hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
@@ -369,7 +488,61 @@ int run_cg(Context* c, const SolveOpts& o, T* x, T* r, T* u, T* p, T* s, T* w, T
   return 0;
 }
 
+template <int D, typename T, int LPR>
+void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
+  fused_cg_spmv_kernel<D, T, LPR><<<b.sp_blocks, kSpBS, 0, st>>>(b.tab, b.nsys);
+  if (b.red_blocks) fused_reduce_partials_kernel<<<b.red_blocks, kBS, 0, st>>>(b.tab, b.nsys);
+  fused_cg_vec_kernel<T><<<b.vec_blocks, kBS, 0, st>>>(b.tab, b.nsys, k & 1);
+}
+
 }  // namespace
+
+int cg_class(int dim, int fp32, int lpr) { return (dim == 3 ? 4 : 0) + (fp32 ? 2 : 0) + (lpr == 64 ? 1 : 0); }
+
+// Workgroup offsets of each system in the fused grids; returns the totals.
+void cg_batch_layout(CgSys* tab, int nsys, CgBatchClass* b) {
+  int sp = 0, red = 0, vec = 0;
+  for (int k = 0; k < nsys; ++k) {
+    tab[k].sp_begin = sp;
+    tab[k].red_begin = red;
+    tab[k].vec_begin = vec;
+    sp += tab[k].gs;
+    red += tab[k].two_level ? kBS : 0;
+    vec += tab[k].gv;
+  }
+  b->nsys = nsys;
+  b->sp_blocks = sp;
+  b->red_blocks = red;
+  b->vec_blocks = vec;
+}
+
+int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int nclass) {
+  for (int k = 0; k < kCgChunk; ++k)
+    for (int c = 0; c < nclass; ++c) {
+      const CgBatchClass& b = classes[c];
+      if (b.nsys == 0) continue;
+      switch (b.cls) {
+        case 0: launch_fused_class<2, double, 32>(st, b, k); break;
+        case 1: launch_fused_class<2, double, 64>(st, b, k); break;
+        case 2: launch_fused_class<2, float, 32>(st, b, k); break;
+        case 3: launch_fused_class<2, float, 64>(st, b, k); break;
+        case 4: launch_fused_class<3, double, 32>(st, b, k); break;
+        case 5: launch_fused_class<3, double, 64>(st, b, k); break;
+        case 6: launch_fused_class<3, float, 32>(st, b, k); break;
+        case 7: launch_fused_class<3, float, 64>(st, b, k); break;
+        default: set_error("bad CG batch class"); return -1;
+      }
+    }
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+int cg_batch_flags(hipStream_t st, const CgSys* tab, int nsys, int* out) {
+  if (nsys == 0) return 0;
+  cg_flags_kernel<<<(nsys + 63) / 64, 64, 0, st>>>(tab, nsys, out);
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
 
 double spmv_bytes(const Context* c, int precision) {
   const double w = precision ? 4.0 : 8.0;
@@ -393,29 +566,34 @@ int launch_spmv(Context* c, int precision, const void* x, void* y) {
   return 0;
 }
 
+// The linear solve K du = residual of one Newton iteration.  With a CG
+// executor installed on the context (the serving runtime's fused launches),
+// the iterations run there; otherwise on the context's own stream.
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   const int64_t n = (int64_t)c->dim * c->N;
+  if (o.precision == 0 ? !c->assembled64 : !c->assembled32) {
+    set_error(o.precision == 0 ? "stiffness not assembled" : "FP32 stiffness not assembled");
+    return -1;
+  }
+  CgSys q;
   int rc;
-  if (o.precision == 0) {
-    if (!c->assembled64) {
-      set_error("stiffness not assembled");
-      return -1;
-    }
-    rc = c->dim == 3 ? run_cg<3, double>(c, o, c->du, c->cg_r, c->cg_z, c->cg_p, c->cg_q, c->cg_w, c->dinv,
-                                         c->Kval, iters, rr_final)
-                     : run_cg<2, double>(c, o, c->du, c->cg_r, c->cg_z, c->cg_p, c->cg_q, c->cg_w, c->dinv,
-                                         c->Kval, iters, rr_final);
-    if (rc) return rc;
+  if (o.precision == 0) rc = c->dim == 3 ? cg_begin<3, double>(c, o, &q) : cg_begin<2, double>(c, o, &q);
+  else rc = c->dim == 3 ? cg_begin<3, float>(c, o, &q) : cg_begin<2, float>(c, o, &q);
+  if (rc) return rc;
+  if (c->cg_exec && !c->prof) {
+    if ((rc = c->cg_exec(c->cg_exec_arg, c, q))) return rc;
+    // the executor has ordered this stream after the system's last fused launch
+    LPMG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    LPMG_HIP(hipStreamSynchronize(c->stream));
+    *iters = c->sc_host->iter;
+    *rr_final = c->sc_host->rr;
+  } else if (o.precision == 0) {
+    rc = c->dim == 3 ? run_cg<3, double>(c, q, iters, rr_final) : run_cg<2, double>(c, q, iters, rr_final);
   } else {
-    if (!c->assembled32) {
-      set_error("FP32 stiffness not assembled");
-      return -1;
-    }
-    rc = c->dim == 3 ? run_cg<3, float>(c, o, c->f_x, c->f_r, c->f_z, c->f_p, c->f_q, c->f_w, c->f_dinv,
-                                        c->Kval32, iters, rr_final)
-                     : run_cg<2, float>(c, o, c->f_x, c->f_r, c->f_z, c->f_p, c->f_q, c->f_w, c->f_dinv,
-                                        c->Kval32, iters, rr_final);
-    if (rc) return rc;
+    rc = c->dim == 3 ? run_cg<3, float>(c, q, iters, rr_final) : run_cg<2, float>(c, q, iters, rr_final);
+  }
+  if (rc) return rc;
+  if (o.precision != 0) {
     f32_to_f64_kernel<<<grid_for(n, 1024), 256, 0, c->stream>>>(c->f_x, c->du, n);
     LPMG_HIP(hipGetLastError());
   }

@@ -32,7 +32,8 @@ EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "
            "lpmg_update_static_damage", "lpmg_store_state", "lpmg_reset_state", "lpmg_search_nonlocal",
            "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_restart", "lpmg_field_len",
            "lpmg_get_field", "lpmg_set_field", "lpmg_profile", "lpmg_profile_read", "lpmg_time_kernel",
-           "lpmg_synchronize")
+           "lpmg_synchronize", "lpmg_request_bytes", "lpmg_server_create", "lpmg_server_submit",
+           "lpmg_server_wait", "lpmg_server_info", "lpmg_server_destroy")
 
 
 class LpmgError(RuntimeError):
@@ -55,6 +56,8 @@ def lib():
         for name in EXPORTS:
             getattr(L, name).restype = ctypes.c_int
         L.lpmg_last_error.restype = ctypes.c_char_p
+        L.lpmg_request_bytes.restype = ctypes.c_int64
+        L.lpmg_server_destroy.restype = None
         L.lpmg_destroy.restype = None
         L.lpmg_destroy.argtypes = [vp]
         L.lpmg_field_len.restype = ctypes.c_int64

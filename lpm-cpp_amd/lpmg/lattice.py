@@ -92,3 +92,19 @@ def bcc_block(nx, ny, nz, radius):
     out[0::2] = corner
     out[1::2] = centre
     return out
+
+
+def unit_cell_info(lattice_type, radius):
+    """Neighbour counts and shell cutoffs of UnitCell (include/unit_cell.h:29-122)."""
+    r = float(radius)
+    if lattice_type == SQUARE2D:
+        return dict(dim=2, nneighbors=8, cut=(2.0 * r, 2.0 * math.sqrt(2.0) * r))
+    if lattice_type == HEXAGON2D:
+        return dict(dim=2, nneighbors=12, cut=(2.0 * r, 2.0 * math.sqrt(3.0) * r))
+    if lattice_type == SC3D:
+        return dict(dim=3, nneighbors=18, cut=(2.0 * r, 2.0 * math.sqrt(2.0) * r))
+    if lattice_type == FCC3D:
+        return dict(dim=3, nneighbors=18, cut=(2.0 * r, 2.0 * math.sqrt(2.0) * r))
+    if lattice_type == BCC3D:
+        return dict(dim=3, nneighbors=14, cut=(2.0 * r, 4.0 / math.sqrt(3.0) * r))
+    raise ValueError(f"unknown lattice {lattice_type}")

@@ -95,7 +95,11 @@ def bcc_j2_bar(nx=30, ny=30, nz=60, radius=0.1, steps=45, strain_step=2.5e-4, E=
 
 def build(backend, prob):
     """Create the system with `backend(lattice, radius, ptype, xyz)` and set it up."""
-    s = backend(prob.lattice, prob.radius, prob.ptype, prob.xyz)
+    bonds = prob.extra.get("bonds")
+    if bonds is not None:
+        s = backend(prob.lattice, prob.radius, prob.ptype, prob.xyz, bonds=bonds)
+    else:
+        s = backend(prob.lattice, prob.radius, prob.ptype, prob.xyz)
     s.set_material(*prob.material)
     if prob.types is not None:
         s.set_types(prob.types)
@@ -144,3 +148,38 @@ def prepare(s):
     """The examples' pre-solve pass: updateGeometry + updateForceState
     (e.g. examples/ex1_isoelasticity_3d_sc.cpp:104-105)."""
     s.forward()
+
+
+def ct_plate(n_target=20000, a_over_w=0.35, steps=30, opening=2.5e-4, W=50.0, E=200e3, nu=0.3,
+             crit=1e-3, max_iter=30, tol=1e-5):
+    """2D hexagonal CT specimen (lpmg.geometry.ct_specimen) of brittle elastic
+    particles: ParticleElastic bonds break once their stretch reaches `crit`
+    (particle_elastic.h:52-68).  The pin particles (first particle ring of the
+    upper half of the top hole and the lower half of the bottom hole) move
+    +-opening*W in y per step; the back-edge particle on the crack line is
+    fixed.  Particles within 0.2 W of a pin centre are type 4, the undamaged
+    type, as the pin regions of examples/ex4_isoelasticity_2d_crack.cpp:47-69."""
+    from . import geometry as geo
+    ct = geo.ct_specimen(n_target, a_over_w, W=W)
+    xyz, r = ct["xyz"], ct["radius"]
+    x, y = xyz[:, 0], xyz[:, 1]
+    (tx, ty), (bx, by) = ct["pin_top"], ct["pin_bottom"]
+    dt = np.hypot(x - tx, y - ty)
+    db = np.hypot(x - bx, y - by)
+    R = ct["R"]
+    top = np.nonzero((dt < R + 2.0 * r) & (y >= ty))[0]
+    bottom = np.nonzero((db < R + 2.0 * r) & (y <= by))[0]
+    fix = np.array([int(np.argmin(np.hypot(x - 0.0, y - ct["yc"])))])
+    types = np.zeros(len(xyz), dtype=np.int32)
+    types[(dt < 0.2 * W) | (db < 0.2 * W)] = 4
+    d = opening * W
+    step = [("disp", "fix", RELATIVE, "x", 0.0), ("disp", "fix", RELATIVE, "y", 0.0),
+            ("disp", "top", RELATIVE, "y", d), ("disp", "bottom", RELATIVE, "y", -d)]
+    p = Problem("ct_plate", lat.HEXAGON2D, r, lat.ELASTIC, xyz, (MAT_ELASTIC, [0.0, 1, E, nu, crit]),
+                dict(fix=fix, top=top, bottom=bottom), _bc_steps(steps, step), types=types, undamaged=4,
+                max_iter=max_iter, tol=tol,
+                note=f"CT hex N={len(xyz)}, a/W={a_over_w}, W={W}, r={r:.4g}, E={E}, nu={nu} plane stress, "
+                     f"critical bond stretch {crit}, pin opening {d:.4g} per step x {steps}")
+    p.extra["bonds"] = ct["bonds"]
+    p.extra["ct"] = dict(W=W, a=ct["a"], yc=ct["yc"], radius=r)
+    return p

@@ -27,6 +27,8 @@
 //   CG / Newton               scipy sparse direct solve on small systems;
 //                             residual of the converged state
 //   J2 return / damage        textbook radial return; energy closed form
+//   CT bond exclusion         hand-placed bonds across a slit / a hole rim with
+//                             known outcome; brute force vs the grid search
 // CG iteration counts are parity unpinned (MKL's internals are closed).
 
 #include <algorithm>
@@ -1288,6 +1290,83 @@ int lpmo_set_field(void* h, int f, const double* in) {
     default: g_err = "field is read-only"; return -1;
   }
   std::copy(in, in + v->size(), v->begin());
+  return 0;
+}
+
+// searchNeighbor + isValid of the CT pre-processor
+// (geometry/geo1_CT_configuration.cpp:44-62, 78-132, 137-171), brute force over
+// all O(N^2) pairs as the reference does.  Excluded regions are `ncirc`
+// circles (cx, cy, R; inCircle) and `nnotch` notches (tip x, tip y, half
+// width; inNotch).  Writes bonds (layer, p1, p2) by p1 then ascending p2;
+// returns their count, or -1 when max_out is too small or a particle has more
+// than `nneighbors` valid neighbours (the reference's fixed-size array).
+static bool ct_in_region(const double* pt, int ncirc, const double* circ, int nnotch, const double* notch) {
+  for (int c = 0; c < ncirc; ++c) {  // inCircle, :44-49
+    const double dist_sq = std::pow(pt[0] - circ[3 * c], 2.0) + std::pow(pt[1] - circ[3 * c + 1], 2.0);
+    if (dist_sq <= circ[3 * c + 2] * circ[3 * c + 2]) return true;
+  }
+  for (int k = 0; k < nnotch; ++k) {  // inNotch, :52-62
+    const double* tip = notch + 3 * k;
+    if (std::fabs(pt[1] - tip[1]) <= tip[2])
+      if ((pt[1] - tip[1]) >= 1. / std::sqrt(3.) * (tip[0] - pt[0]) &&
+          (pt[1] - tip[1]) <= 1. / std::sqrt(3.) * (-tip[0] + pt[0]))
+        return true;
+  }
+  return false;
+}
+
+int lpmo_ct_search_neighbor(int n, const double* xyz, double cut0, double cut1, int nneighbors, int ncirc,
+                            const double* circ, int nnotch, const double* notch, int64_t max_out,
+                            int32_t* layer_out, int32_t* p1_out, int32_t* p2_out) {
+  std::vector<std::vector<int>> nbr(n), lay(n);
+  bool overflow = false;
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int i = 0; i < n; i++) {
+    const double* xi = xyz + 3 * (size_t)i;
+    for (int j = 0; j < n; j++) {
+      const double* xj = xyz + 3 * (size_t)j;
+      const double dis = std::sqrt(std::pow(xj[0] - xi[0], 2) + std::pow(xj[1] - xi[1], 2) +
+                                   std::pow(xj[2] - xi[2], 2));
+      if ((dis < 1.01 * cut1) && (j != i)) {
+        bool valid = true;
+        const int n_interval = 10;
+        const double dx = (xj[0] - xi[0]) / n_interval;
+        const double dy = (xj[1] - xi[1]) / n_interval;
+        const double dz = (xj[2] - xi[2]) / n_interval;
+        for (int k = 0; k < n_interval; ++k) {
+          const double pt1[3] = {xi[0] + k * dx, xi[1] + k * dy, xi[2] + k * dz};
+          const double pt2[3] = {xi[0] + (k + 1) * dx, xi[1] + (k + 1) * dy, xi[2] + (k + 1) * dz};
+          const double mid[3] = {(pt1[0] + pt2[0]) / 2.0, (pt1[1] + pt2[1]) / 2.0, (pt1[2] + pt2[2]) / 2.0};
+          valid = !ct_in_region(mid, ncirc, circ, nnotch, notch) && valid;
+        }
+        if (valid) {
+          nbr[i].push_back(j);
+          lay[i].push_back(dis < 1.01 * cut0 ? 0 : 1);
+        }
+      }
+    }
+    if ((int)nbr[i].size() > nneighbors) {
+#pragma omp atomic write
+      overflow = true;
+    }
+  }
+  if (overflow) { g_err = "a particle has more valid neighbours than the unit cell holds"; return -1; }
+  int64_t m = 0;
+  for (int i = 0; i < n; ++i)
+    for (size_t t = 0; t < nbr[i].size(); ++t) {
+      if (m >= max_out) { g_err = "output too small"; return -1; }
+      layer_out[m] = lay[i][t];
+      p1_out[m] = i;
+      p2_out[m] = nbr[i][t];
+      ++m;
+    }
+  return (int)m;
+}
+
+// isValid per particle for the same regions (geo1_CT_configuration.cpp:137-171, 194-196)
+int lpmo_ct_valid(int n, const double* xyz, int ncirc, const double* circ, int nnotch, const double* notch,
+                  int32_t* valid) {
+  for (int i = 0; i < n; ++i) valid[i] = !ct_in_region(xyz + 3 * (size_t)i, ncirc, circ, nnotch, notch);
   return 0;
 }
 

@@ -59,6 +59,14 @@ def lib():
         L.lpmo_search_nonlocal.argtypes = [vp, ctypes.c_double]
         L.lpmo_unit_cell.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.lpmo_set_threads.argtypes = [ctypes.c_int]
+        L.lpmo_ct_search_neighbor.restype = ctypes.c_int
+        L.lpmo_ct_search_neighbor.argtypes = [ctypes.c_int, dp, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                              ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int64,
+                                              ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_int32)]
+        L.lpmo_ct_valid.restype = ctypes.c_int
+        L.lpmo_ct_valid.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, dp,
+                                    ctypes.POINTER(ctypes.c_int32)]
         _lib = L
     return _lib
 
@@ -76,6 +84,38 @@ def unit_cell(lattice, radius):
 
 def set_threads(n):
     lib().lpmo_set_threads(int(n))
+
+
+def _regions(region):
+    circ = np.ascontiguousarray(np.array(region.circles, dtype=np.float64).reshape(-1, 3))
+    notch = np.ascontiguousarray(np.array(region.notches, dtype=np.float64).reshape(-1, 3))
+    return circ, notch
+
+
+def ct_search_neighbor(xyz, cut, nneighbors, region):
+    """geo1's searchNeighbor by brute force: (layer, p1, p2)."""
+    L = lib()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    n = xyz.shape[0]
+    circ, notch = _regions(region)
+    cap = n * nneighbors
+    out = [np.zeros(cap, dtype=np.int32) for _ in range(3)]
+    m = L.lpmo_ct_search_neighbor(n, _p(xyz, ctypes.c_double), float(cut[0]), float(cut[1]), int(nneighbors),
+                                  len(circ), _p(circ, ctypes.c_double), len(notch), _p(notch, ctypes.c_double),
+                                  cap, *(_p(o, ctypes.c_int32) for o in out))
+    if m < 0:
+        raise RuntimeError(L.lpmo_last_error().decode())
+    return tuple(o[:m] for o in out)
+
+
+def ct_valid(xyz, region):
+    """geo1's isValid per particle."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    circ, notch = _regions(region)
+    v = np.zeros(xyz.shape[0], dtype=np.int32)
+    lib().lpmo_ct_valid(xyz.shape[0], _p(xyz, ctypes.c_double), len(circ), _p(circ, ctypes.c_double),
+                        len(notch), _p(notch, ctypes.c_double), _p(v, ctypes.c_int32))
+    return v.astype(bool)
 
 
 class Oracle:
