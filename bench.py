@@ -91,6 +91,84 @@ def cpu_baseline(prob, threads):
                 step_s=t_step, converged=bool(ok))
 
 
+# CT serving stream: Poisson arrivals (50 requests/s), particle counts
+# log-uniform in [5k, 250k], a/W uniform in [0.2, 0.5], 30 load steps, critical
+# bond stretch 1e-3.  Newton tolerance 1e-9: FP64 roundoff in the absolute
+# particle coordinates puts a floor near 2e-10 of the reaction on the residual at
+# 1e5 particles, which the reference's arithmetic shares.
+SERVE_RATE = 50.0
+SERVE_SEED = 1
+SERVE_TOL = 1e-9
+SERVE_FUSE_ROWS = 50000
+SERVE_SLOTS = 16
+
+
+def ct_stream(n_req, seed=SERVE_SEED):
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / SERVE_RATE, n_req)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    sizes = np.exp(rng.uniform(np.log(5e3), np.log(2.5e5), n_req)).astype(int)
+    aw = rng.uniform(0.2, 0.5, n_req)
+    return arrivals, sizes, aw
+
+
+def replay(probs, arrivals, opts, slots, fuse):
+    """Submit each request at its arrival time; latency = submit -> finished."""
+    from lpmg import solver_opts
+    from lpmg.serve import Request, Server
+    o = solver_opts(**opts)
+    reqs = [Request(p, o, out_fields=("XYZ", "BDAMAGE"), sizes={"BDAMAGE": len(p.extra["bonds"][0])})
+            for p in probs]
+    with Server(max_active=slots, batching=fuse) as srv:
+        t0 = time.perf_counter()
+        tickets = []
+        for r, t in zip(reqs, arrivals):
+            dt = t - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(srv.submit(r))
+        results = [srv.wait(k) for k in tickets]
+        makespan = time.perf_counter() - t0
+        info = srv.info()
+    bad = [st.get("error") for rc, st in results if rc != 0]
+    if bad:
+        raise RuntimeError(f"served requests failed: {bad}")
+    lat = [st["total_ms"] for _, st in results]
+    return reqs, results, dict(value=len(reqs) / makespan, unit="solves/s", makespan_s=makespan,
+                               p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
+                               cg_iterations=float(sum(st["cg"] for _, st in results)),
+                               peak_running=info["peak_running"], max_batch=info["max_batch"],
+                               fused_chunks=info["chunks"], graphs=info["graphs"])
+
+
+def serving_bench(n_req):
+    """Config-4 stream replayed twice on the GPU: the plain FP64 path (one
+    request at a time on its own handle, Jacobi-PCG at the reference's CG
+    tolerance) and the served mixed path (16 slots, fused CG launches for
+    requests up to 50k particles, FP32 inner solves)."""
+    from lpmg import problems as P
+    arrivals, sizes, aw = ct_stream(n_req)
+    probs = [P.ct_plate(int(n), float(a), steps=30, opening=1.5e-5, crit=1e-3, tol=SERVE_TOL)
+             for n, a in zip(sizes, aw)]
+    print(f"# serving: {n_req} CT requests, particles {[len(p.xyz) for p in probs]}", file=sys.stderr, flush=True)
+    r64, _, plain = replay(probs, arrivals, FP64_OPTS, 1, False)
+    print(f"# serving plain FP64: {plain}", file=sys.stderr, flush=True)
+    rmx, res_mx, served = replay(probs, arrivals, MIXED_OPTS, SERVE_SLOTS, SERVE_FUSE_ROWS)
+    print(f"# serving mixed: {served}", file=sys.stderr, flush=True)
+    errs, flags_equal = [], 0
+    for p, a, b in zip(probs, r64, rmx):
+        u64 = a.out["XYZ"].reshape(-1, 3) - p.xyz
+        umx = b.out["XYZ"].reshape(-1, 3) - p.xyz
+        errs.append(float(np.linalg.norm(umx - u64) / np.linalg.norm(u64)))
+        flags_equal += int(np.array_equal(a.out["BDAMAGE"] > 0.5, b.out["BDAMAGE"] > 0.5))
+    return dict(workload=f"CT stream: {n_req} requests (seed {SERVE_SEED}, Poisson {SERVE_RATE:g}/s, particles "
+                         f"log-uniform 5k-250k, a/W U[0.2,0.5], 30 load steps, critical bond stretch 1e-3, "
+                         f"Newton tol {SERVE_TOL:g})",
+                particles=[len(p.xyz) for p in probs], plain_fp64=plain, served_mixed=served,
+                speedup_vs_plain_fp64=served["value"] / plain["value"],
+                max_disp_rel_err_vs_fp64=max(errs), requests_with_identical_damage_flags=flags_equal)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +180,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--serve-requests", type=int, default=6,
+                    help="requests of the CT serving stream replayed (0 skips the serving section)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every solve of this workload runs on a single MI355X "
@@ -184,6 +264,8 @@ def main():
     if not args.no_mixed:
         s.assemble(precision=1)  # the FP64-only solves above left no current FP32 copy
         sp32_ms, sp32_bytes = s.time_kernel(1, reps=50)
+    s.close()
+    serving = serving_bench(args.serve_requests) if args.serve_requests > 0 else None
     cpu = None if args.no_cpu else cpu_baseline(prob, args.cpu_threads)
 
     value = len(times) / sum(times)
@@ -221,6 +303,7 @@ def main():
         "spmv_back_to_back": {"fp64_ms": sp64_ms, "fp64_GBps": sp64_bytes / (sp64_ms * 1e-3) / 1e9,
                               "fp32_ms": sp32_ms, "fp32_GBps": sp32_bytes / (sp32_ms * 1e-3) / 1e9},
         "mixed": mixed,
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     traffic_file = os.path.join(ROOT, "profiles", "spmv_fp64_sc40_traffic.json")

@@ -222,12 +222,13 @@ int lpmg_synchronize(lpmg_handle h);
  *     solver arrays stay in HBM for the whole solve) and a stream; its
  *     forward pass, assembly and state updates run there, overlapping the
  *     other requests' work;
- *   - with batching on, the CG iterations of every request that is in its
+ *   - the CG iterations of every (small enough) request that is in its
  *     linear solve run as fused launches: one SpMV, one vector and (large
  *     systems) one reduction launch per iteration cover all of them, grouped
  *     by dimension, precision and row width.  A request's arithmetic is the
  *     same alone and in any mix: the result is bit-identical.
- * Requests join or leave the fused launches every 16 iterations. */
+ * Requests join or leave the fused launches every 16 iterations; a chunk of
+ * 16 iterations is one captured hipGraph per bucket of launch sizes. */
 typedef struct lpmg_server_s* lpmg_server;
 
 /* One boundary condition of a load step: DispBC (boundary.h:9-25) or
@@ -305,9 +306,12 @@ enum {
 int64_t lpmg_request_bytes(const lpmg_request* r);
 
 /* max_active: worker slots (requests in flight); mem_budget: bytes (<= 0:
- * 90% of the free device memory at creation); batching: 1 fused CG launches,
- * 0 every request runs its CG on its own stream. */
-int lpmg_server_create(int max_active, int64_t mem_budget, int batching, lpmg_server* out);
+ * 90% of the free device memory at creation); fuse_max_rows: requests with at
+ * most this many particles (block rows) run their CG iterations in the fused
+ * launches, larger ones on their own stream, where one system already streams
+ * HBM at full rate and its phases overlap the fused batch; < 0 fuses every
+ * request, 0 none. */
+int lpmg_server_create(int max_active, int64_t mem_budget, int64_t fuse_max_rows, lpmg_server* out);
 /* Queue a request; *ticket identifies it.  -1 malformed, -3 larger than the
  * budget (never admissible). */
 int lpmg_server_submit(lpmg_server s, const lpmg_request* r, int64_t* ticket);
@@ -318,7 +322,8 @@ int lpmg_server_submit(lpmg_server s, const lpmg_request* r, int64_t* ticket);
 int lpmg_server_wait(lpmg_server s, int64_t ticket, double timeout_s, double* stats);
 /* out[0] queued, [1] running, [2] reserved bytes, [3] budget, [4] peak
  * running, [5] peak reserved bytes, [6] finished, [7] fused CG chunks,
- * [8] system-chunks in them, [9] largest batch (systems in one chunk). */
+ * [8] system-chunks in them, [9] largest batch (systems in one chunk),
+ * [10] chunk graphs captured. */
 int lpmg_server_info(lpmg_server s, double* out);
 /* Finish every queued and running request, then free the server. */
 void lpmg_server_destroy(lpmg_server s);

@@ -68,11 +68,22 @@ struct CgSys {
   int cls;                        // batch class: cg_class(dim, fp32, lanes per row)
 };
 
-// The systems of one batch class in a fused launch (table in device memory)
+// Live size of one batch class, read by the fused kernels from device memory
+// so that a captured launch (grid sized for a bucket) serves any membership
+// that fits it: workgroups past the live counts return at once.
+struct CgBatchHdr {
+  int nsys, sp_blocks, red_blocks, vec_blocks;
+};
+constexpr size_t kCgHdrBytes = 256;  // the table follows the header
+
+// The systems of one batch class in a fused launch: header + table in device
+// memory (`hdr`, table at hdr + kCgHdrBytes), live counts, and the grids the
+// launches use (>= the live counts)
 struct CgBatchClass {
   int cls = 0;
-  const CgSys* tab = nullptr;
-  int nsys = 0, sp_blocks = 0, red_blocks = 0, vec_blocks = 0;
+  const CgBatchHdr* hdr = nullptr;
+  CgBatchHdr live{0, 0, 0, 0};
+  int sp_grid = 0, red_grid = 0, vec_grid = 0;
 };
 
 constexpr int kCgChunk = 16;     // CG iterations per launch batch (even)
@@ -215,7 +226,7 @@ struct SolveOpts {
 };
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 int cg_class(int dim, int fp32, int lpr);
-void cg_batch_layout(CgSys* tab, int nsys, CgBatchClass* b);
+void cg_batch_layout(CgSys* tab, int nsys, CgBatchHdr* h);
 int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int nclass);
 int cg_batch_flags(hipStream_t st, const CgSys* tab, int nsys, int* out);
 int launch_spmv(Context* c, int precision, const void* x, void* y);

@@ -22,7 +22,13 @@
 //            pipelined two deep, so the flags of chunk k are read while chunk
 //            k+1 runs.  A leaving system's stream waits on an event recorded
 //            after the last chunk that still lists it.
+//            A chunk is a captured hipGraph: the fused kernels read the live
+//            system count and workgroup totals from a device header uploaded
+//            before each chunk, and their grids are rounded up to a bucket
+//            (<= 25% above the live count), so one graph per bucket combination
+//            serves every membership that fits it; graphs are cached.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -61,10 +67,11 @@ class Batcher {
     device_ = device;
     cap_ = cap;
     LPMG_HIP(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking));
+    LPMG_HIP(hipStreamCreateWithFlags(&capst_, hipStreamNonBlocking));
     for (int s = 0; s < 2; ++s) {
       for (int k = 0; k < kCgClasses; ++k) {
-        LPMG_HIP(hipMalloc((void**)&dtab_[s][k], sizeof(CgSys) * cap));
-        LPMG_HIP(hipHostMalloc((void**)&htab_[s][k], sizeof(CgSys) * cap, hipHostMallocDefault));
+        LPMG_HIP(hipMalloc((void**)&dbuf_[s][k], kCgHdrBytes + sizeof(CgSys) * cap));
+        LPMG_HIP(hipHostMalloc((void**)&hbuf_[s][k], kCgHdrBytes + sizeof(CgSys) * cap, hipHostMallocDefault));
       }
       LPMG_HIP(hipMalloc((void**)&dflags_[s], sizeof(int) * 2 * kCgClasses * cap));
       LPMG_HIP(hipHostMalloc((void**)&hflags_[s], sizeof(int) * 2 * kCgClasses * cap, hipHostMallocDefault));
@@ -86,17 +93,20 @@ class Batcher {
       started_ = false;
     }
     if (st_) (void)hipStreamSynchronize(st_);
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+    graphs_.clear();
     for (int s = 0; s < 2; ++s) {
       for (int k = 0; k < kCgClasses; ++k) {
-        if (dtab_[s][k]) (void)hipFree(dtab_[s][k]);
-        if (htab_[s][k]) (void)hipHostFree(htab_[s][k]);
+        if (dbuf_[s][k]) (void)hipFree(dbuf_[s][k]);
+        if (hbuf_[s][k]) (void)hipHostFree(hbuf_[s][k]);
       }
       if (dflags_[s]) (void)hipFree(dflags_[s]);
       if (hflags_[s]) (void)hipHostFree(hflags_[s]);
       if (cev_[s]) (void)hipEventDestroy(cev_[s]);
     }
     if (st_) (void)hipStreamDestroy(st_);
-    st_ = nullptr;
+    if (capst_) (void)hipStreamDestroy(capst_);
+    st_ = capst_ = nullptr;
   }
 
   // Worker side: the system's start is enqueued on c->stream.  Returns once
@@ -129,11 +139,12 @@ class Batcher {
     return 0;
   }
 
-  void info(double* chunks, double* sys_chunks, double* max_batch) {
+  void info(double* chunks, double* sys_chunks, double* max_batch, double* graphs) {
     std::lock_guard<std::mutex> lk(m_);
     *chunks = (double)chunks_;
     *sys_chunks = (double)sys_chunks_;
     *max_batch = (double)max_batch_;
+    *graphs = (double)graphs_made_;
   }
 
  private:
@@ -141,6 +152,57 @@ class Batcher {
     bool valid = false;
     std::vector<std::shared_ptr<CgJob>> members;
   };
+
+  // launch grid for a live workgroup count: rounded up to a multiple of a
+  // quarter of its leading power of two (at least 16)
+  static int bucket(int x) {
+    if (x <= 0) return 0;
+    int p = 1;
+    while (p * 2 <= x) p *= 2;
+    const int g = std::max(16, p / 4);
+    return (x + g - 1) / g * g;
+  }
+
+  // the captured chunk for this slot and these grids (captured on first use)
+  int chunk_graph(int slot, const CgBatchClass* cls, hipGraphExec_t* out) {
+    std::array<int, 1 + 3 * kCgClasses> key{};
+    key[0] = slot;
+    for (int k = 0; k < kCgClasses; ++k) {
+      key[1 + 3 * k] = cls[k].sp_grid;
+      key[2 + 3 * k] = cls[k].red_grid;
+      key[3 + 3 * k] = cls[k].vec_grid;
+    }
+    auto it = graphs_.find(key);
+    if (it != graphs_.end()) {
+      *out = it->second;
+      return 0;
+    }
+    if (graphs_.size() >= 1024) {  // bound the cache; every graph is idle once the stream drains
+      LPMG_HIP(hipStreamSynchronize(st_));
+      for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
+      graphs_.clear();
+    }
+    // captured on a stream that never runs work: the chunk stream itself is
+    // never in capture mode, so events recorded on it stay plain events for
+    // the workers' streams to wait on
+    hipGraph_t graph = nullptr;
+    LPMG_HIP(hipStreamBeginCapture(capst_, hipStreamCaptureModeThreadLocal));
+    const int rc = cg_batch_enqueue_chunk(capst_, cls, kCgClasses);
+    const hipError_t e = hipStreamEndCapture(capst_, &graph);
+    if (rc) return rc;
+    LPMG_HIP(e);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    LPMG_HIP(e2);
+    graphs_[key] = exec;
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      ++graphs_made_;
+    }
+    *out = exec;
+    return 0;
+  }
 
   int n_active() const {
     int n = 0;
@@ -219,23 +281,31 @@ class Batcher {
         const auto& a = active_[k];
         cls[k].cls = k;
         if (a.empty()) continue;
-        for (size_t t = 0; t < a.size(); ++t) htab_[slot][k][t] = a[t]->q;
-        cg_batch_layout(htab_[slot][k], (int)a.size(), &cls[k]);
-        cls[k].tab = dtab_[slot][k];
-        if (hipMemcpyAsync(dtab_[slot][k], htab_[slot][k], sizeof(CgSys) * a.size(), hipMemcpyHostToDevice, st_) !=
-            hipSuccess) {
+        CgBatchHdr* hh = reinterpret_cast<CgBatchHdr*>(hbuf_[slot][k]);
+        CgSys* ht = reinterpret_cast<CgSys*>(hbuf_[slot][k] + kCgHdrBytes);
+        for (size_t t = 0; t < a.size(); ++t) ht[t] = a[t]->q;
+        cg_batch_layout(ht, (int)a.size(), hh);
+        cls[k].hdr = reinterpret_cast<const CgBatchHdr*>(dbuf_[slot][k]);
+        cls[k].live = *hh;
+        cls[k].sp_grid = bucket(hh->sp_blocks);
+        cls[k].red_grid = bucket(hh->red_blocks);
+        cls[k].vec_grid = bucket(hh->vec_blocks);
+        if (hipMemcpyAsync(dbuf_[slot][k], hbuf_[slot][k], kCgHdrBytes + sizeof(CgSys) * a.size(),
+                           hipMemcpyHostToDevice, st_) != hipSuccess) {
           fail_all("table upload failed");
           return false;
         }
       }
-      if (cg_batch_enqueue_chunk(st_, cls, kCgClasses)) {
-        fail_all("fused launch failed");
+      hipGraphExec_t g = nullptr;
+      if (chunk_graph(slot, cls, &g) || hipGraphLaunch(g, st_) != hipSuccess) {
+        fail_all("fused chunk launch failed");
         return false;
       }
       for (int k = 0; k < kCgClasses; ++k) {
         const auto& a = active_[k];
         if (a.empty()) continue;
-        if (cg_batch_flags(st_, dtab_[slot][k], (int)a.size(), dflags_[slot] + 2 * nflag)) {
+        const CgSys* dt = reinterpret_cast<const CgSys*>(dbuf_[slot][k] + kCgHdrBytes);
+        if (cg_batch_flags(st_, dt, (int)a.size(), dflags_[slot] + 2 * nflag)) {
           fail_all("flag gather failed");
           return false;
         }
@@ -288,9 +358,11 @@ class Batcher {
   }
 
   int device_ = 0, cap_ = 0;
-  hipStream_t st_ = nullptr;
-  CgSys* dtab_[2][kCgClasses] = {};
-  CgSys* htab_[2][kCgClasses] = {};
+  hipStream_t st_ = nullptr, capst_ = nullptr;  // chunk stream, capture-only stream
+  char* dbuf_[2][kCgClasses] = {};  // header + table per chunk slot and class
+  char* hbuf_[2][kCgClasses] = {};
+  std::map<std::array<int, 1 + 3 * kCgClasses>, hipGraphExec_t> graphs_;
+  int64_t graphs_made_ = 0;
   int* dflags_[2] = {};
   int* hflags_[2] = {};
   hipEvent_t cev_[2] = {};
@@ -391,6 +463,7 @@ using namespace lpmg;
 
 struct lpmg_server_s {
   int device = 0, max_active = 1, batching = 1;
+  int64_t fuse_max_rows = -1;  // systems with more block rows run their CG on their own stream
   int64_t budget = 0;
   std::mutex m;
   std::condition_variable cv_work, cv_done;
@@ -431,7 +504,7 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
     off += r.group_size[g];
   }
   if (r.nonlocal_ratio > 0 && (rc = lpmg_search_nonlocal(h, r.nonlocal_ratio))) return rc;
-  if (S->batching) {
+  if (S->batching && (S->fuse_max_rows < 0 || (int64_t)c->N <= S->fuse_max_rows)) {
     c->cg_exec = batcher_exec;
     c->cg_exec_arg = &S->batcher;
   }
@@ -550,13 +623,14 @@ int64_t lpmg_request_bytes(const lpmg_request* r) {
   return estimate_bytes(r);
 }
 
-int lpmg_server_create(int max_active, int64_t mem_budget, int batching, lpmg_server* out) {
+int lpmg_server_create(int max_active, int64_t mem_budget, int64_t fuse_max_rows, lpmg_server* out) {
   if (!out) { set_error("null output server"); return -1; }
   *out = nullptr;
   if (max_active <= 0 || max_active > 256) { set_error("max_active must be in 1..256"); return -1; }
   auto* S = new lpmg_server_s;
   S->max_active = max_active;
-  S->batching = batching ? 1 : 0;
+  S->batching = fuse_max_rows != 0 ? 1 : 0;
+  S->fuse_max_rows = fuse_max_rows;
   hipError_t e = hipGetDevice(&S->device);
   size_t fr = 0, tot = 0;
   if (e == hipSuccess) e = hipMemGetInfo(&fr, &tot);
@@ -658,8 +732,8 @@ int lpmg_server_info(lpmg_server S, double* out) {
     out[5] = (double)S->peak_reserved;
     out[6] = (double)S->finished;
   }
-  out[7] = out[8] = out[9] = 0;
-  if (S->batching) S->batcher.info(&out[7], &out[8], &out[9]);
+  out[7] = out[8] = out[9] = out[10] = 0;
+  if (S->batching) S->batcher.info(&out[7], &out[8], &out[9], &out[10]);
   return 0;
 }
 

@@ -262,22 +262,30 @@ __device__ __forceinline__ int cg_sys_of(const CgSys* __restrict__ tab, int nsys
   return lo;
 }
 
+// The launches are sized for a bucket >= the live workgroup counts in `hdr`;
+// workgroups past them return before any barrier.
 template <int D, typename T, int LPR>
-__global__ void __launch_bounds__(kSpBS) fused_cg_spmv_kernel(const CgSys* __restrict__ tab, int nsys) {
-  const CgSys& q = tab[cg_sys_of<0>(tab, nsys, blockIdx.x)];
+__global__ void __launch_bounds__(kSpBS) fused_cg_spmv_kernel(const CgBatchHdr* __restrict__ hdr,
+                                                              const CgSys* __restrict__ tab) {
+  if ((int)blockIdx.x >= hdr->sp_blocks) return;
+  const CgSys& q = tab[cg_sys_of<0>(tab, hdr->nsys, blockIdx.x)];
   cg_spmv_body<D, T, LPR>(blockIdx.x - q.sp_begin, q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
                           q.partS_raw, q.sc);
 }
 
-__global__ void __launch_bounds__(kBS) fused_reduce_partials_kernel(const CgSys* __restrict__ tab, int nsys) {
-  const CgSys& q = tab[cg_sys_of<1>(tab, nsys, blockIdx.x)];
+__global__ void __launch_bounds__(kBS) fused_reduce_partials_kernel(const CgBatchHdr* __restrict__ hdr,
+                                                                    const CgSys* __restrict__ tab) {
+  if ((int)blockIdx.x >= hdr->red_blocks) return;
+  const CgSys& q = tab[cg_sys_of<1>(tab, hdr->nsys, blockIdx.x)];
   if (!q.two_level) return;  // systems without a second level own no workgroups here
   reduce_partials_body(blockIdx.x - q.red_begin, kBS, q.partS_raw, q.gs, q.partS, q.sc);
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBS) fused_cg_vec_kernel(const CgSys* __restrict__ tab, int nsys, int slot) {
-  const CgSys& q = tab[cg_sys_of<2>(tab, nsys, blockIdx.x)];
+__global__ void __launch_bounds__(kBS) fused_cg_vec_kernel(const CgBatchHdr* __restrict__ hdr,
+                                                           const CgSys* __restrict__ tab, int slot) {
+  if ((int)blockIdx.x >= hdr->vec_blocks) return;
+  const CgSys& q = tab[cg_sys_of<2>(tab, hdr->nsys, blockIdx.x)];
   cg_vec_body<T>(blockIdx.x - q.vec_begin, q.gv, (T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
                  (const T*)q.dinv, q.n, slot ? q.partV1 : q.partV0, q.gv, q.partS, q.nS,
                  slot ? q.partV0 : q.partV1, q.sc);
@@ -490,17 +498,18 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
 
 template <int D, typename T, int LPR>
 void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
-  fused_cg_spmv_kernel<D, T, LPR><<<b.sp_blocks, kSpBS, 0, st>>>(b.tab, b.nsys);
-  if (b.red_blocks) fused_reduce_partials_kernel<<<b.red_blocks, kBS, 0, st>>>(b.tab, b.nsys);
-  fused_cg_vec_kernel<T><<<b.vec_blocks, kBS, 0, st>>>(b.tab, b.nsys, k & 1);
+  const CgSys* tab = reinterpret_cast<const CgSys*>(reinterpret_cast<const char*>(b.hdr) + kCgHdrBytes);
+  fused_cg_spmv_kernel<D, T, LPR><<<b.sp_grid, kSpBS, 0, st>>>(b.hdr, tab);
+  if (b.red_grid) fused_reduce_partials_kernel<<<b.red_grid, kBS, 0, st>>>(b.hdr, tab);
+  fused_cg_vec_kernel<T><<<b.vec_grid, kBS, 0, st>>>(b.hdr, tab, k & 1);
 }
 
 }  // namespace
 
 int cg_class(int dim, int fp32, int lpr) { return (dim == 3 ? 4 : 0) + (fp32 ? 2 : 0) + (lpr == 64 ? 1 : 0); }
 
-// Workgroup offsets of each system in the fused grids; returns the totals.
-void cg_batch_layout(CgSys* tab, int nsys, CgBatchClass* b) {
+// Workgroup offsets of each system in the fused grids; the live totals.
+void cg_batch_layout(CgSys* tab, int nsys, CgBatchHdr* h) {
   int sp = 0, red = 0, vec = 0;
   for (int k = 0; k < nsys; ++k) {
     tab[k].sp_begin = sp;
@@ -510,17 +519,17 @@ void cg_batch_layout(CgSys* tab, int nsys, CgBatchClass* b) {
     red += tab[k].two_level ? kBS : 0;
     vec += tab[k].gv;
   }
-  b->nsys = nsys;
-  b->sp_blocks = sp;
-  b->red_blocks = red;
-  b->vec_blocks = vec;
+  h->nsys = nsys;
+  h->sp_blocks = sp;
+  h->red_blocks = red;
+  h->vec_blocks = vec;
 }
 
 int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int nclass) {
   for (int k = 0; k < kCgChunk; ++k)
     for (int c = 0; c < nclass; ++c) {
       const CgBatchClass& b = classes[c];
-      if (b.nsys == 0) continue;
+      if (b.sp_grid == 0) continue;
       switch (b.cls) {
         case 0: launch_fused_class<2, double, 32>(st, b, k); break;
         case 1: launch_fused_class<2, double, 64>(st, b, k); break;

@@ -54,7 +54,7 @@ def _bind():
     L.lpmg_request_bytes.restype = ctypes.c_int64
     L.lpmg_request_bytes.argtypes = [ctypes.POINTER(RequestStruct)]
     L.lpmg_server_create.restype = ctypes.c_int
-    L.lpmg_server_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int, ctypes.POINTER(vp)]
+    L.lpmg_server_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(vp)]
     L.lpmg_server_submit.restype = ctypes.c_int
     L.lpmg_server_submit.argtypes = [vp, ctypes.POINTER(RequestStruct), ctypes.POINTER(ctypes.c_int64)]
     L.lpmg_server_wait.restype = ctypes.c_int
@@ -162,12 +162,16 @@ def _field_len(name, n, nbond, dim):
 
 
 class Server:
-    """lpmg_server: worker slots, a device-memory budget and fused CG launches."""
+    """lpmg_server: worker slots, a device-memory budget and fused CG launches.
+
+    batching: True fuses every request's CG, False none; an int fuses the
+    requests of at most that many particles (larger ones use their own stream)."""
 
     def __init__(self, max_active=8, mem_budget=0, batching=True):
         L = _bind()
         h = ctypes.c_void_p()
-        rc = L.lpmg_server_create(int(max_active), int(mem_budget), int(bool(batching)), ctypes.byref(h))
+        fuse = -1 if batching is True else 0 if batching is False else int(batching)
+        rc = L.lpmg_server_create(int(max_active), int(mem_budget), fuse, ctypes.byref(h))
         if rc != 0:
             raise LpmgError(f"lpmg_server_create failed ({rc}): {_err()}")
         self.h = h
@@ -194,11 +198,11 @@ class Server:
         return rc, stats
 
     def info(self):
-        out = np.zeros(10, dtype=np.float64)
+        out = np.zeros(11, dtype=np.float64)
         if _bind().lpmg_server_info(self.h, _p(out, ctypes.c_double)) != 0:
             raise LpmgError(_err())
         keys = ("queued", "running", "reserved", "budget", "peak_running", "peak_reserved", "finished",
-                "chunks", "system_chunks", "max_batch")
+                "chunks", "system_chunks", "max_batch", "graphs")
         return dict(zip(keys, out.tolist()))
 
     def close(self):
