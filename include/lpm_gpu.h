@@ -69,17 +69,21 @@ enum {
   LPMG_F_DLE = 21,
   LPMG_F_DAMAGE_LAST = 22,
   LPMG_F_TYPE = 23,        /* n     Particle::type         */
-  LPMG_F_TDLE_TOTAL = 24   /* n*2   Particle::TdLe_total   */
+  LPMG_F_TDLE_TOTAL = 24,  /* n*2   Particle::TdLe_total   */
+  LPMG_F_FLOATING = 25     /* n     1: particle of a fragment no intact bond ties to a
+                                    constrained particle (held in place; no
+                                    reference counterpart), read-only */
 };
 
 /* Solver options, opts[8]:
- *   [0] preconditioner: 0 none (the reference's MKL CG, solver.h:311), 1 Jacobi
+ *   [0] preconditioner: 0 none (the reference's MKL CG, solver.h:311), 1 Jacobi,
+ *       2 aggregation multigrid (one V-cycle; built on first use per handle)
  *   [1] rtol, [2] atol: stop when ||r||^2 <= rtol*||r0||^2 + atol
  *       (the reference's dpar[0], dpar[1], solver.h:312-313)
  *   [3] maximum CG iterations (<= 0: the problem size, as ipar[4], solver.h:308)
  *   [4] precision: 0 FP64 stiffness and CG; 1 FP32 stiffness and CG inside the
  *       FP64 Newton (residual, refinement update and state stay FP64)
- *   [5] CG iterations launched between convergence checks (<= 0: 16)
+ *   [5] unused (16 CG iterations run between convergence checks)
  * Solver statistics, stats[4]: [0] CG iterations, [1] final ||r||^2 (CG) or
  * ||r|| (Newton), [2] Newton tolerance multiplier, [3] Newton iterations. */
 
@@ -206,6 +210,12 @@ int lpmg_profile_read(lpmg_handle h, double* out);
  * out[1] = algorithmic bytes per launch. */
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out);
 
+/* Unknowns per level of the handle's multigrid hierarchy (finest first), built
+ * on its first multigrid solve (opts[0] = 2); returns the number of levels,
+ * 0 before.  No reference counterpart (the reference solves with MKL CG or
+ * PARDISO). */
+int lpmg_multigrid_levels(lpmg_handle h, int32_t* out, int cap);
+
 int lpmg_synchronize(lpmg_handle h);
 
 /* ---- Serving runtime ---------------------------------------------------
@@ -276,7 +286,9 @@ typedef struct lpmg_request {
   int undamaged;               /* undamaged particle type */
   int max_iter;                /* Newton iterations per pass */
   double tol;                  /* Newton tolerance */
-  double opts[8];              /* solver options as lpmg_solve_linear */
+  double opts[8];              /* solver options as lpmg_solve_linear; opts[0] < 0:
+                                  the server picks fused Jacobi for requests it
+                                  fuses, multigrid for the others */
   int max_halvings;            /* step halvings allowed before the request fails */
   int n_out;
   const lpmg_out_field* out;

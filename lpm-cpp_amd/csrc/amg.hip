@@ -1,0 +1,840 @@
+// amg.hip -- aggregation multigrid preconditioner for the linear solve of the
+// Newton step (opts precond = 2), a replacement for the Jacobi-PCG that keeps
+// the solve's stopping test (||r||^2 <= rtol ||r0||^2 + atol, solver.h:312-313).
+//
+// Hierarchy (built once per handle from the particle positions and the
+// block-CSR pattern, on the host):
+//   level 0      the handle's BSR stiffness (b = dim)
+//   level l+1    aggregates of level-l nodes: the nonempty cells of a uniform
+//                grid (cell edge 3 lattice spacings at level 1, x2.5 per level)
+//                over the nodes' positions (particles, then aggregate centroids)
+//   prolongation piecewise constant per aggregate and component (the
+//                translations, unsmoothed), so the Galerkin coarse operator is
+//                a plain block sum: A_{l+1}[a,b] = sum over level-l blocks (i,j)
+//                with agg(i) = a, agg(j) = b.  Each coarse block owns the list of
+//                its level-l blocks, summed in a fixed order (deterministic).
+//   coarsest     <= 128 unknowns, inverted in LDS by one workgroup
+//                (Gauss-Jordan, no pivoting: SPD, or semidefinite when a crack
+//                has cut a piece loose -- vanishing pivots are then dropped and
+//                the inverse is taken on the range)
+//   values       summed in FP64 from the FP64 stiffness, rounded afterwards
+//                for the FP32 inner solve
+// The values are rebuilt on the device whenever the stiffness was assembled
+// again (every modified-Newton pass).
+//
+// Preconditioner: one V-cycle with one damped-Jacobi sweep (omega = 2/3) before
+// and after the coarse correction at every level.  Per level: a fused
+// residual kernel r = b - A (omega D^-1 b), the restriction (sum over each
+// aggregate's members), the prolongation fused with the pre-smoothed value,
+// and the post-smoothing sweep; the finest sweep also writes the partials of
+// (r, z) and zeroes z on the DOFs the Dirichlet elimination decoupled (the
+// coarse correction would otherwise move them: the preconditioner then acts
+// on the free DOFs only, where it stays symmetric positive definite).
+//
+// PCG (standard form, two reductions per iteration): w = A p with partials of
+// (p,w); x += alpha p, r -= alpha w with partials of (r,r); z = M r (V-cycle)
+// with partials of (r,z); stop test, p = z + beta p.  As in the Jacobi CG, no
+// kernel waits on other workgroups of its own launch; the iteration index a
+// kernel reads is written by an earlier launch, and scalars live across
+// iterations in slots indexed by its parity.  The host replays a captured graph
+// of 16 iterations between reads of the convergence flag.
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "bsr.h"
+
+namespace lpmg {
+
+namespace {
+
+constexpr int kCoarsestMax = 128;  // unknowns of the dense coarsest level (fits LDS)
+constexpr double kOmega = 2.0 / 3.0;
+
+// ---- device kernels ------------------------------------------------------
+
+// Row gather of A x for this lane's row, reduced over the row's lanes.  XM 0:
+// x_j = x[j]; XM 1: x_j = om * dinv[j] * b[j] (one Jacobi sweep from zero).
+template <int D, typename T, int LPR, int XM>
+__device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
+                                           const T* __restrict__ val, const T* __restrict__ x,
+                                           const T* __restrict__ dinv, const T* __restrict__ b, T om, int N,
+                                           int* row_out, bool* live_out, int* sl_out, T acc[D]) {
+  constexpr int DD = D * D;
+  constexpr int RPW = 64 / LPR;
+  const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
+  const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
+  const int row = wave * RPW + sub;
+  const bool live = row < N;
+  const int rw = live ? row : N - 1;
+  const int lo = cptr[rw], hi = cptr[rw + 1];
+#pragma unroll
+  for (int q = 0; q < D; ++q) acc[q] = T(0);
+  for (int e0 = lo; e0 < hi; e0 += LPR) {
+    const int e = e0 + sl;
+    const bool ok = e < hi;
+    const int ec = ok ? e : hi - 1;
+    const int col = conns[ec];
+    const T* vb = val + (size_t)ec * DD;
+    T v[DD], xv[D];
+#pragma unroll
+    for (int k = 0; k < DD; ++k) v[k] = vb[k];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const size_t q = (size_t)col * D + k;
+      xv[k] = XM == 0 ? x[q] : om * dinv[q] * b[q];
+    }
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      T t = v[q * D] * xv[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) t += v[q * D + k] * xv[k];
+      acc[q] += ok ? t : T(0);
+    }
+  }
+#pragma unroll
+  for (int o = LPR / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int q = 0; q < D; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
+  *row_out = row;
+  *live_out = live;
+  *sl_out = sl;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ T pick(const T acc[D], int sl) {
+  T v = acc[0];
+#pragma unroll
+  for (int q = 1; q < D; ++q)
+    if (sl == q) v = acc[q];
+  return v;
+}
+
+// r = b - A (om D^-1 b): the residual after one Jacobi sweep from zero
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* __restrict__ cptr,
+                                                                    const int* __restrict__ conns,
+                                                                    const T* __restrict__ val, int N,
+                                                                    const T* __restrict__ b,
+                                                                    const T* __restrict__ dinv, T om,
+                                                                    T* __restrict__ r, const Scalars* sc) {
+  if (sc->done) return;
+  int row, sl;
+  bool live;
+  T acc[D];
+  row_gather<D, T, LPR, 1>(blockIdx.x, cptr, conns, val, nullptr, dinv, b, om, N, &row, &live, &sl, acc);
+  if (live && sl < D) {
+    const size_t q = (size_t)row * D + sl;
+    r[q] = b[q] - pick<D, T>(acc, sl);
+  }
+}
+
+// z = x + om D^-1 (b - A x); with `mask` (finest level), z = 0 on the
+// eliminated DOFs, so the CG never moves them however inexact the solve; with
+// `part`, the workgroup's partial of (b, z)
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) amg_postsmooth_kernel(const int* __restrict__ cptr,
+                                                               const int* __restrict__ conns,
+                                                               const T* __restrict__ val, int N,
+                                                               const T* __restrict__ b, const T* __restrict__ x,
+                                                               const T* __restrict__ dinv, T om,
+                                                               const T* __restrict__ mask,
+                                                               T* __restrict__ z, double* __restrict__ part,
+                                                               const Scalars* sc) {
+  if (sc->done) return;
+  int row, sl;
+  bool live;
+  T acc[D];
+  row_gather<D, T, LPR, 0>(blockIdx.x, cptr, conns, val, x, nullptr, nullptr, om, N, &row, &live, &sl, acc);
+  double dot = 0;
+  if (live && sl < D) {
+    const size_t q = (size_t)row * D + sl;
+    T zq = x[q] + om * dinv[q] * (b[q] - pick<D, T>(acc, sl));
+    if (mask) zq *= mask[q];
+    z[q] = zq;
+    dot = (double)b[q] * (double)zq;
+  }
+  if (part) {
+    Vec<1> a;
+    a.v[0] = dot;
+    write_partial_at<kSpBS, 1>(part, blockIdx.x, a);
+  }
+}
+
+// b_c[a] = sum of r over the aggregate's members: one wavefront per
+// aggregate, lanes over members, then a fixed butterfly (deterministic)
+template <int D, typename T>
+__global__ void __launch_bounds__(256) amg_restrict_kernel(const int* __restrict__ mptr, const int* __restrict__ midx,
+                                                           int Nc, const T* __restrict__ r, T* __restrict__ bc,
+                                                           const Scalars* sc) {
+  if (sc->done) return;
+  const int a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (a >= Nc) return;  // whole wavefronts leave together
+  double s[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) s[k] = 0;
+  for (int q = mptr[a] + lane; q < mptr[a + 1]; q += 64) {
+    const size_t i = (size_t)midx[q] * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s[k] += (double)r[i + k];
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < D; ++k) s[k] += __shfl_xor(s[k], o, 64);
+  if (lane < D) {
+    double v = s[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k)
+      if (lane == k) v = s[k];
+    bc[(size_t)a * D + lane] = (T)v;
+  }
+}
+
+// x = om D^-1 b + x_c[agg]: the pre-smoothed value plus the coarse correction
+template <int D, typename T>
+__global__ void amg_prolong_kernel(const int* __restrict__ agg, int N, const T* __restrict__ b,
+                                   const T* __restrict__ dinv, T om, const T* __restrict__ xc,
+                                   T* __restrict__ x, const Scalars* sc) {
+  if (sc->done) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)N * D) return;
+  const int i = (int)(t / D), k = (int)(t - (int64_t)i * D);
+  x[t] = om * dinv[t] * b[t] + xc[(size_t)agg[i] * D + k];
+}
+
+// x = Ainv b on the coarsest level (n <= kCoarsestMax)
+template <typename T>
+__global__ void amg_coarsest_kernel(const T* __restrict__ inv, int n, const T* __restrict__ b, T* __restrict__ x,
+                                    double* __restrict__ part, const Scalars* sc) {
+  if (sc->done) return;
+  const int r = threadIdx.x;
+  double s = 0;
+  if (r < n) {
+    for (int j = 0; j < n; ++j) s += (double)inv[(size_t)r * n + j] * (double)b[j];
+    x[r] = (T)s;
+  }
+  if (part) {  // the whole system is the coarsest level: partial of (b, x)
+    Vec<1> a;
+    a.v[0] = r < n ? (double)b[r] * (double)(T)s : 0.0;
+    write_partial_at<kBS, 1>(part, 0, a);
+  }
+}
+
+// coarse values: block c = sum of its fine blocks (fixed order), in FP64
+template <typename Tf, typename Tc, int DD>
+__global__ void amg_gather_kernel(const int64_t* __restrict__ gptr, const int* __restrict__ gidx, int64_t nb,
+                                  const Tf* __restrict__ fine, Tc* __restrict__ coarse) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * DD) return;
+  const int64_t c = t / DD;
+  const int k = (int)(t - c * DD);
+  double s = 0;
+  for (int64_t q = gptr[c]; q < gptr[c + 1]; ++q) s += (double)fine[(size_t)gidx[q] * DD + k];
+  coarse[t] = (Tc)s;
+}
+
+template <typename T, int D>
+__global__ void amg_dinv_kernel(const int* __restrict__ dpos, const T* __restrict__ val, int N, T* __restrict__ dinv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+#pragma unroll
+  for (int r = 0; r < D; ++r) dinv[(size_t)D * i + r] = T(1) / val[(size_t)dpos[i] * D * D + r * D + r];
+}
+
+// Inverse of the coarsest matrix (n <= kCoarsestMax, SPD) by in-place
+// Gauss-Jordan in LDS, one workgroup; the BSR level is expanded to dense first.
+template <typename Tv, typename Ti, int D>
+__global__ void __launch_bounds__(1024) amg_invert_kernel(const int* __restrict__ cptr,
+                                                          const int* __restrict__ conns,
+                                                          const Tv* __restrict__ val, int N,
+                                                          Ti* __restrict__ inv) {
+  __shared__ double A[kCoarsestMax * kCoarsestMax];
+  __shared__ double fcol[kCoarsestMax];
+  __shared__ int skip[kCoarsestMax];
+  __shared__ double dmax;
+  const int n = N * D;
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) A[t] = 0;
+  __syncthreads();
+  for (int t = threadIdx.x; t < N * D; t += blockDim.x) {  // thread per (row node, component)
+    const int i = t / D, ri = t - i * D;
+    for (int e = cptr[i]; e < cptr[i + 1]; ++e) {
+      const int j = conns[e];
+      for (int cj = 0; cj < D; ++cj) A[(i * D + ri) * n + j * D + cj] = (double)val[(size_t)e * D * D + ri * D + cj];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0;
+    for (int k = 0; k < n; ++k) m = fmax(m, fabs(A[k * n + k]));
+    dmax = m;
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) skip[k] = 0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    const double p = A[k * n + k];
+    __syncthreads();
+    // a vanishing pivot of the (semidefinite) Schur complement: a rigid motion
+    // the constraints leave free (a specimen split by a crack); its row and
+    // column are zero, and the inverse is taken on the range only
+    if (!(p > 1e-10 * dmax)) {
+      if (threadIdx.x == 0) skip[k] = 1;
+      continue;
+    }
+    for (int j = threadIdx.x; j < n; j += blockDim.x) A[k * n + j] = (j == k ? 1.0 : A[k * n + j]) / p;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) fcol[i] = i == k ? 0.0 : A[i * n + k];
+    __syncthreads();
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+      const int i = t / n, j = t - i * n;
+      if (i == k) continue;
+      A[t] = (j == k ? 0.0 : A[t]) - fcol[i] * A[k * n + j];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < n * n; t += blockDim.x) {
+    const int i = t / n, j = t - i * n;
+    inv[t] = (skip[i] || skip[j]) ? Ti(0) : (Ti)A[t];
+  }
+}
+
+// ---- PCG kernels (one system, its own stream) ----
+
+template <int D, typename T, int LPR>
+__global__ void __launch_bounds__(kSpBS) pcg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
+                                                         const T* __restrict__ val, const T* __restrict__ p,
+                                                         T* __restrict__ w, int N, double* __restrict__ part,
+                                                         Scalars* sc) {
+  if (sc->done) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // read by the next kernels
+  Vec<1> a;
+  a.v[0] = bsr_rows<D, T, LPR, true>(blockIdx.x, cptr, conns, val, p, w, N);
+  write_partial<kSpBS, 1>(part, a);
+}
+
+__global__ void __launch_bounds__(kBS) pcg_reduce_kernel(const double* __restrict__ in, int n,
+                                                         double* __restrict__ out, const Scalars* __restrict__ sc) {
+  if (sc->done) return;
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(n, lo + per);
+  Vec<1> a;
+  a.v[0] = 0;
+  for (int b = lo + threadIdx.x; b < hi; b += kBS) a.v[0] += in[b];
+  write_partial<kBS, 1>(out, a);
+}
+
+// x += alpha p; r -= alpha w; partials of (r,r).  alpha = gamma_i / (p,w).
+template <typename T>
+__global__ void __launch_bounds__(kBS) pcg_vec1_kernel(T* __restrict__ x, T* __restrict__ r,
+                                                       const T* __restrict__ p, const T* __restrict__ w, int64_t n,
+                                                       const double* __restrict__ pw_part, int npw,
+                                                       double* __restrict__ rr_part, Scalars* sc) {
+  if (sc->done) return;
+  const int i = sc->icur;
+  const double pw = sum_partials<kBS, 1>(pw_part, npw).v[0];
+  // (p, A p) <= 0: breakdown (a singular stiffness); no update, and the
+  // vector step that follows stops the solve
+  const double alpha = pw > 0 ? sc->gam[i & 1] / pw : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->alp[i & 1] = alpha;
+    sc->pq = pw;
+    sc->alpha = alpha;
+  }
+  const T a = (T)alpha;
+  double rr = 0;
+  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
+    x[k] += a * p[k];
+    const T rk = r[k] - a * w[k];
+    r[k] = rk;
+    rr += (double)rk * (double)rk;
+  }
+  Vec<1> v;
+  v.v[0] = rr;
+  write_partial<kBS, 1>(rr_part, v);
+}
+
+// stop test on (r,r); beta = (r,z)_{i+1} / (r,z)_i; p = z + beta p.  Workgroup 0
+// sets `done`, so each workgroup reads it once through LDS.
+template <typename T>
+__global__ void __launch_bounds__(kBS) pcg_vec2_kernel(T* __restrict__ p, const T* __restrict__ z, int64_t n,
+                                                       const double* __restrict__ rz_part, int nrz,
+                                                       const double* __restrict__ rr_part, int nrr, Scalars* sc) {
+  __shared__ int done_s;
+  if (threadIdx.x == 0) done_s = sc->done;
+  __syncthreads();
+  if (done_s) return;
+  const int i = sc->icur;
+  const double rz = sum_partials<kBS, 1>(rz_part, nrz).v[0];
+  const double rr = sum_partials<kBS, 1>(rr_part, nrr).v[0];
+  const bool stop = rr <= sc->thresh || i + 1 >= sc->maxit || !(sc->pq > 0) || !(rz > 0);
+  const double beta = rz / sc->gam[i & 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rr = rr;
+    sc->rz = rz;
+    sc->iter = i + 1;
+    sc->inext = i + 1;
+    sc->gam[(i + 1) & 1] = rz;
+    sc->beta = beta;
+    if (stop) sc->done = 1;
+  }
+  if (stop) return;
+  const T b = (T)beta;
+  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) p[k] = z[k] + b * p[k];
+}
+
+// start: x = 0, r = b (rhs); partials of (r,r)
+template <typename T>
+__global__ void __launch_bounds__(kBS) pcg_init_kernel(const double* __restrict__ rhs, T* __restrict__ x,
+                                                       T* __restrict__ r, int64_t n, double* __restrict__ rr_part) {
+  double rr = 0;
+  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
+    const T rk = (T)rhs[k];
+    x[k] = 0;
+    r[k] = rk;
+    rr += (double)rk * (double)rk;
+  }
+  Vec<1> v;
+  v.v[0] = rr;
+  write_partial<kBS, 1>(rr_part, v);
+}
+
+// after z = M r: gamma_0 = (r,z), the stop threshold, p = z
+template <typename T>
+__global__ void __launch_bounds__(kBS) pcg_init2_kernel(T* __restrict__ p, const T* __restrict__ z, int64_t n,
+                                                        const double* __restrict__ rz_part, int nrz,
+                                                        const double* __restrict__ rr_part, int nrr, Scalars* sc) {
+  const double rz = sum_partials<kBS, 1>(rz_part, nrz).v[0];
+  const double rr = sum_partials<kBS, 1>(rr_part, nrr).v[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->gam[0] = rz;
+    sc->rz = rz;
+    sc->rr = rr;
+    sc->rr0 = rr;
+    sc->thresh = sc->rtol * rr + sc->atol;
+    sc->iter = 0;
+    sc->icur = sc->inext = 0;
+    if (rr <= sc->thresh || sc->maxit <= 0) sc->done = 1;
+  }
+  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) p[k] = z[k];
+}
+
+__global__ void amg_setup_scalars_kernel(Scalars* sc, double rtol, double atol, int64_t maxit) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    sc->rtol = rtol;
+    sc->atol = atol;
+    sc->maxit = (int)maxit;
+    sc->iter = 0;
+    sc->icur = sc->inext = 0;
+    sc->done = 0;
+  }
+}
+
+// finest level: point-Jacobi diagonal and the free-DOF mask (1 free, 0 eliminated)
+template <int D, typename T>
+__global__ void amg_dinv0_kernel(const int* __restrict__ dpos, const T* __restrict__ val, const int* __restrict__ cons,
+                                 const double* __restrict__ dvis, const int* __restrict__ floating, int N,
+                                 T* __restrict__ dinv, T* __restrict__ mask) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    dinv[(size_t)D * i + r] = T(1) / val[(size_t)dpos[i] * D * D + r * D + r];
+    mask[(size_t)D * i + r] = dof_fixed(cons, dvis, floating, i, r) ? T(0) : T(1);
+  }
+}
+
+int grid_n(int64_t work, int per) {
+  int64_t g = (work + per - 1) / per;
+  g = std::max<int64_t>(8, std::min<int64_t>(g, kMaxGrid));
+  return (int)g;
+}
+
+int rows_grid(int N, int lpr) {
+  const int rows_per_block = (kSpBS / 64) * (64 / lpr);
+  return (N + rows_per_block - 1) / rows_per_block;
+}
+
+}  // namespace
+
+// ---- hierarchy ------------------------------------------------------------
+
+struct AmgLevel {  // level l >= 1; its nodes are the aggregates of level l-1
+  int N = 0, lpr = 32;
+  int64_t nnzb = 0;
+  int *cptr = nullptr, *conns = nullptr, *dpos = nullptr;
+  int64_t* gptr = nullptr;   // nnzb + 1: this level's blocks -> ranges of gidx
+  int* gidx = nullptr;       // level l-1 block indices, grouped by coarse block
+  int *mptr = nullptr, *midx = nullptr;  // members (level l-1 nodes) of each aggregate
+  int* agg = nullptr;        // aggregate of each level l-1 node
+  void* val[2] = {nullptr, nullptr};   // FP64 / FP32 values
+  void* dinv[2] = {nullptr, nullptr};
+  void* vec[2][4] = {{nullptr}};       // b, r, x, y per precision
+};
+
+struct Amg {
+  bool ok = false;             // false: the hierarchy could not be built (falls back to Jacobi)
+  std::vector<AmgLevel> lv;    // levels 1..L
+  int nc = 0;                  // unknowns of the coarsest level
+  void* inv[2] = {nullptr, nullptr};
+  void* mask[2] = {nullptr, nullptr};  // finest-level free-DOF mask per precision
+  int64_t version[2] = {-1, -1};
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+template <typename T>
+int amg_alloc(Context* c, T** p, size_t n) { return dalloc(c, p, n); }
+
+// Build the aggregation hierarchy on the host from the initial positions and
+// the level-0 block pattern.
+int amg_build(Context* c) {
+  Amg* A = c->amg;
+  const int D = c->dim;
+  int N = c->N;
+  std::vector<int> cptr(N + 1), conns(c->nnzb);
+  std::vector<double4> x4(N);
+  LPMG_HIP(copy_sync(c, cptr.data(), c->cptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
+  LPMG_HIP(copy_sync(c, conns.data(), c->conns, sizeof(int) * c->nnzb, hipMemcpyDeviceToHost));
+  LPMG_HIP(copy_sync(c, x4.data(), c->xyz0, sizeof(double4) * N, hipMemcpyDeviceToHost));
+  std::vector<double> pos(3 * (size_t)N);
+  for (int i = 0; i < N; ++i) {
+    pos[3 * i] = x4[i].x;
+    pos[3 * i + 1] = x4[i].y;
+    pos[3 * i + 2] = x4[i].z;
+  }
+  double h = 3.0 * 2.0 * c->cell.radius;
+  while ((int64_t)N * D > kCoarsestMax) {
+    // aggregates: nonempty cells of a grid of edge h, numbered in cell order
+    double lo[3] = {1e300, 1e300, 1e300};
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], pos[3 * i + k]);
+    std::vector<int64_t> key(N);
+    int64_t dims[3] = {1, 1, 1};
+    std::vector<int64_t> cc(3 * (size_t)N);
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < 3; ++k) {
+        cc[3 * i + k] = (int64_t)std::floor((pos[3 * i + k] - lo[k]) / h);
+        dims[k] = std::max(dims[k], cc[3 * i + k] + 1);
+      }
+    for (int i = 0; i < N; ++i) key[i] = (cc[3 * i + 2] * dims[1] + cc[3 * i + 1]) * dims[0] + cc[3 * i];
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+    std::vector<int> agg(N), mptr(1, 0), midx(N);
+    int Nc = 0;
+    for (int t = 0; t < N; ++t) {
+      if (t > 0 && key[order[t]] != key[order[t - 1]]) {
+        mptr.push_back(t);
+        ++Nc;
+      }
+      agg[order[t]] = Nc;
+      midx[t] = order[t];
+    }
+    ++Nc;
+    mptr.push_back(N);
+    if (Nc > (int)(0.8 * N)) {  // too little coarsening at this cell size
+      h *= 2.0;
+      if (h > 1e6 * c->cell.radius) break;
+      continue;
+    }
+    // coarse pattern and gather lists: per aggregate row, the (column, fine
+    // block) pairs sorted by column then fine block; each column run is one
+    // coarse block and its gather list
+    std::vector<int> ccptr(Nc + 1, 0), cconns, dpos(Nc, -1), gidx;
+    std::vector<int64_t> gptr(1, 0);
+    std::vector<std::pair<int, int>> pairs;
+    for (int a = 0; a < Nc; ++a) {
+      pairs.clear();
+      for (int q = mptr[a]; q < mptr[a + 1]; ++q) {
+        const int i = midx[q];
+        for (int e = cptr[i]; e < cptr[i + 1]; ++e) pairs.emplace_back(agg[conns[e]], e);
+      }
+      std::sort(pairs.begin(), pairs.end());
+      const size_t np = pairs.size();
+      for (size_t t = 0; t < np; ++t) {
+        if (t == 0 || pairs[t].first != pairs[t - 1].first) {
+          if (pairs[t].first == a) dpos[a] = (int)cconns.size();
+          cconns.push_back(pairs[t].first);
+        }
+        gidx.push_back(pairs[t].second);
+        if (t + 1 == np || pairs[t + 1].first != pairs[t].first) gptr.push_back((int64_t)gidx.size());
+      }
+      ccptr[a + 1] = (int)cconns.size();
+      if (dpos[a] < 0) {
+        set_error("multigrid: aggregate without a diagonal block");
+        return -1;
+      }
+    }
+    AmgLevel L;
+    L.N = Nc;
+    L.nnzb = (int64_t)cconns.size();
+    int maxrow = 0;
+    for (int a = 0; a < Nc; ++a) maxrow = std::max(maxrow, ccptr[a + 1] - ccptr[a]);
+    L.lpr = maxrow <= 32 ? 32 : 64;
+    int rc = 0;
+    rc |= amg_alloc(c, &L.cptr, Nc + 1);
+    rc |= amg_alloc(c, &L.conns, cconns.size());
+    rc |= amg_alloc(c, &L.dpos, Nc);
+    rc |= amg_alloc(c, &L.gptr, gptr.size());
+    rc |= amg_alloc(c, &L.gidx, gidx.size());
+    rc |= amg_alloc(c, &L.mptr, Nc + 1);
+    rc |= amg_alloc(c, &L.midx, N);
+    rc |= amg_alloc(c, &L.agg, N);
+    if (rc) return -3;
+    LPMG_HIP(copy_sync(c, L.cptr, ccptr.data(), sizeof(int) * (Nc + 1), hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.conns, cconns.data(), sizeof(int) * cconns.size(), hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.dpos, dpos.data(), sizeof(int) * Nc, hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.gptr, gptr.data(), sizeof(int64_t) * gptr.size(), hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.gidx, gidx.data(), sizeof(int) * gidx.size(), hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.mptr, mptr.data(), sizeof(int) * (Nc + 1), hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.midx, midx.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, L.agg, agg.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    A->lv.push_back(L);
+    // next level: the aggregates' centroids and pattern
+    std::vector<double> cen(3 * (size_t)Nc, 0.0);
+    for (int a = 0; a < Nc; ++a) {
+      const int m = mptr[a + 1] - mptr[a];
+      for (int q = mptr[a]; q < mptr[a + 1]; ++q)
+        for (int k = 0; k < 3; ++k) cen[3 * a + k] += pos[3 * midx[q] + k];
+      for (int k = 0; k < 3; ++k) cen[3 * a + k] /= m;
+    }
+    pos.swap(cen);
+    cptr.swap(ccptr);
+    conns.swap(cconns);
+    N = Nc;
+    h *= 2.5;
+  }
+  if ((int64_t)N * D > kCoarsestMax) {
+    set_error("multigrid: the coarsest level is too large");
+    return -1;
+  }
+  A->nc = N * D;
+  A->ok = true;
+  return 0;
+}
+
+template <typename T>
+constexpr int prec_of() { return sizeof(T) == 4 ? 1 : 0; }
+
+template <typename T>
+int amg_alloc_prec(Context* c) {
+  Amg* A = c->amg;
+  const int P = prec_of<T>();
+  const int D = c->dim;
+  if (P == 1)  // the FP32 levels are rounded from FP64 ones
+    if (int rc = amg_alloc_prec<double>(c)) return rc;
+  for (auto& L : A->lv) {
+    if (L.val[P]) continue;
+    T *v = nullptr, *di = nullptr, *w[4] = {nullptr, nullptr, nullptr, nullptr};
+    int rc = amg_alloc(c, &v, (size_t)L.nnzb * D * D) | amg_alloc(c, &di, (size_t)L.N * D);
+    for (auto& q : w) rc |= amg_alloc(c, &q, (size_t)L.N * D);
+    if (rc) return -3;
+    L.val[P] = v;
+    L.dinv[P] = di;
+    for (int k = 0; k < 4; ++k) L.vec[P][k] = w[k];
+  }
+  if (!A->inv[P]) {
+    T *iv = nullptr, *mk = nullptr;
+    if (amg_alloc(c, &iv, (size_t)A->nc * A->nc) || amg_alloc(c, &mk, (size_t)c->N * D)) return -3;
+    A->inv[P] = iv;
+    A->mask[P] = mk;
+  }
+  return 0;
+}
+
+// Level values from the current FP64 stiffness.  The coarse operators are
+// block sums whose entries cancel more at every level (their rows add up to
+// nearly nothing for the rigid motions), so they are always summed from the
+// FP64 values and only then rounded for the FP32 solve.
+template <typename T>
+__global__ void amg_round_kernel(const double* __restrict__ a, T* __restrict__ b, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) b[t] = (T)a[t];
+}
+
+template <int D, typename T>
+int amg_values(Context* c) {
+  Amg* A = c->amg;
+  const int P = prec_of<T>();
+  const double* fine = c->Kval;
+  for (auto& L : A->lv) {
+    const int64_t nt = L.nnzb * D * D;
+    const unsigned gb = (unsigned)((nt + 255) / 256);
+    amg_gather_kernel<double, double, D * D><<<gb, 256, 0, c->stream>>>(L.gptr, L.gidx, L.nnzb, fine,
+                                                                       (double*)L.val[0]);
+    if (P) amg_round_kernel<T><<<gb, 256, 0, c->stream>>>((const double*)L.val[0], (T*)L.val[P], nt);
+    amg_dinv_kernel<T, D><<<(L.N + 255) / 256, 256, 0, c->stream>>>(L.dpos, (const T*)L.val[P], L.N, (T*)L.dinv[P]);
+    fine = (const double*)L.val[0];
+  }
+  if (A->lv.empty())
+    amg_invert_kernel<double, T, D><<<1, 1024, 0, c->stream>>>(c->cptr, c->conns, c->Kval, c->N, (T*)A->inv[P]);
+  else {
+    const AmgLevel& L = A->lv.back();
+    amg_invert_kernel<double, T, D><<<1, 1024, 0, c->stream>>>(L.cptr, L.conns, (const double*)L.val[0], L.N,
+                                                              (T*)A->inv[P]);
+  }
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+// Enqueue z = M b at level l (l = 0: the handle's operator, b = the CG
+// residual, z written with partials of (b, z) into rz_part).
+template <int D, typename T>
+void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r, T* scratch_x, double* rz_part) {
+  Amg* A = c->amg;
+  const int P = prec_of<T>();
+  const int L = (int)A->lv.size();
+  const T om = (T)kOmega;
+  if (l == L) {  // coarsest (the eliminated DOFs of a single-level system
+                 // decouple in the exact inverse: z = 0 there already)
+    amg_coarsest_kernel<T><<<1, kBS, 0, st>>>((const T*)A->inv[P], A->nc, b, out, l == 0 ? rz_part : nullptr, c->sc);
+    return;
+  }
+  const int N = l == 0 ? c->N : A->lv[l - 1].N;
+  const int* cptr = l == 0 ? c->cptr : A->lv[l - 1].cptr;
+  const int* conns = l == 0 ? c->conns : A->lv[l - 1].conns;
+  const T* val = l == 0 ? (const T*)(P ? (const void*)c->Kval32 : (const void*)c->Kval) : (const T*)A->lv[l - 1].val[P];
+  const T* dinv = l == 0 ? (const T*)(P ? (const void*)c->f_dinv : (const void*)c->dinv) : (const T*)A->lv[l - 1].dinv[P];
+  const int lpr = l == 0 ? (c->max_nconn <= 32 ? 32 : 64) : A->lv[l - 1].lpr;
+  T* r = l == 0 ? scratch_r : (T*)A->lv[l - 1].vec[P][1];
+  T* xp = l == 0 ? scratch_x : (T*)A->lv[l - 1].vec[P][2];
+  const AmgLevel& C = A->lv[l];  // the next level
+  T* bc = (T*)C.vec[P][0];
+  T* xc = (l + 1 == L) ? (T*)C.vec[P][2] : (T*)C.vec[P][3];
+  const int g = rows_grid(N, lpr);
+  if (lpr == 32)
+    amg_presmooth_resid_kernel<D, T, 32><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
+  else
+    amg_presmooth_resid_kernel<D, T, 64><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
+  amg_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(C.mptr, C.midx, C.N, r, bc, c->sc);
+  vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr, nullptr);
+  amg_prolong_kernel<D, T><<<(unsigned)(((int64_t)N * D + 255) / 256), 256, 0, st>>>(C.agg, N, b, dinv, om, xc, xp,
+                                                                                      c->sc);
+  const T* mask = l == 0 ? (const T*)A->mask[P] : nullptr;
+  if (lpr == 32)
+    amg_postsmooth_kernel<D, T, 32><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, xp, dinv, om, mask, out,
+                                                        l == 0 ? rz_part : nullptr, c->sc);
+  else
+    amg_postsmooth_kernel<D, T, 64><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, xp, dinv, om, mask, out,
+                                                        l == 0 ? rz_part : nullptr, c->sc);
+}
+
+}  // namespace
+
+void amg_free(Context* c) {
+  if (!c->amg) return;
+  for (auto& g : c->amg->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  delete c->amg;  // device arrays are the context's allocations
+  c->amg = nullptr;
+}
+
+// PCG with the multigrid preconditioner on the context's stream; x = du (FP64)
+// or f_x (FP32), rhs = residual.
+template <int D, typename T>
+int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
+  if (!c->amg) {
+    c->amg = new Amg;
+    if (int rc = amg_build(c)) {
+      amg_free(c);
+      return rc;
+    }
+  }
+  Amg* A = c->amg;
+  const int P = prec_of<T>();
+  if (int rc = amg_alloc_prec<T>(c)) return rc;
+  const T* val0 = (const T*)(P ? (const void*)c->Kval32 : (const void*)c->Kval);
+  T* dinv0 = (T*)(P ? (void*)c->f_dinv : (void*)c->dinv);
+  T *x, *r, *z, *p, *w, *tx;
+  if (P) {
+    x = (T*)c->f_x; r = (T*)c->f_r; z = (T*)c->f_z; p = (T*)c->f_p; w = (T*)c->f_w; tx = (T*)c->f_q;
+  } else {
+    x = (T*)c->du; r = (T*)c->cg_r; z = (T*)c->cg_z; p = (T*)c->cg_p; w = (T*)c->cg_w; tx = (T*)c->cg_q;
+  }
+  const int N = c->N;
+  const int64_t n = (int64_t)D * N;
+  amg_dinv0_kernel<D, T><<<(N + 255) / 256, 256, 0, c->stream>>>(c->dpos, val0, c->cons, c->damage_visual,
+                                                                 c->floating, N, dinv0, (T*)A->mask[P]);
+  if (A->version[P] != c->asm_version) {
+    if (int rc = amg_values<D, T>(c)) return rc;
+    A->version[P] = c->asm_version;
+  }
+  const int lpr = c->max_nconn <= 32 ? 32 : 64;
+  const int gs = rows_grid(N, lpr);
+  const int gv = grid_n(n, kBS * 4);
+  const bool two = gs > kMaxParts;
+  const bool coarse0 = A->lv.empty();  // the whole system is the coarsest level
+  // partial buffers: (r,r) [gv], (p,w) [gs (+kBS)], (r,z) [gs (+kBS)]
+  double* rr_part = c->partials;
+  double* pw_raw = c->partials + 2 * kMaxGrid;
+  double* pw2 = pw_raw + gs;
+  double* rz_raw = pw2 + kBS;
+  double* rz2 = rz_raw + gs;
+  if ((int64_t)(2 * kMaxGrid + 2 * (gs + kBS)) > 2 * (int64_t)c->n_partials) {
+    set_error("partials buffer too small for the multigrid CG");
+    return -1;
+  }
+  const int nrz = coarse0 ? 1 : (two ? kBS : gs);
+  const int npw = two ? kBS : gs;
+  const int64_t maxit = o.maxit > 0 ? o.maxit : n;
+  amg_setup_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, o.rtol, o.atol, maxit);
+  pcg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, n, rr_part);
+  vcycle<D, T>(c, c->stream, 0, r, z, w, tx, rz_raw);
+  if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
+  pcg_init2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv, c->sc);
+  LPMG_HIP(hipGetLastError());
+  auto enqueue_chunk = [&]() -> int {
+    for (int k = 0; k < kCgChunk; ++k) {
+      if (lpr == 32)
+        pcg_spmv_kernel<D, T, 32><<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
+      else
+        pcg_spmv_kernel<D, T, 64><<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
+      if (two) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(pw_raw, gs, pw2, c->sc);
+      pcg_vec1_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, p, w, n, two ? pw2 : pw_raw, npw, rr_part, c->sc);
+      vcycle<D, T>(c, c->stream, 0, r, z, w, tx, rz_raw);
+      if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
+      pcg_vec2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv,
+                                                   c->sc);
+    }
+    LPMG_HIP(hipGetLastError());
+    return 0;
+  };
+  hipGraphExec_t* gexec = &A->graph[P];
+  if (!*gexec) {
+    hipGraph_t graph = nullptr;
+    LPMG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_chunk();
+    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (rc) return rc;
+    LPMG_HIP(e2);
+    LPMG_HIP(hipGraphInstantiate(gexec, graph, nullptr, nullptr, 0));
+    LPMG_HIP(hipGraphDestroy(graph));
+  }
+  for (;;) {
+    LPMG_HIP(hipGraphLaunch(*gexec, c->stream));
+    LPMG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
+    LPMG_HIP(hipStreamSynchronize(c->stream));
+    if (c->sc_host->done) break;
+  }
+  *iters = c->sc_host->iter;
+  *rr_final = c->sc_host->rr;
+  return 0;
+}
+
+template int run_pcg_amg<2, double>(Context*, const SolveOpts&, int*, double*);
+template int run_pcg_amg<3, double>(Context*, const SolveOpts&, int*, double*);
+template int run_pcg_amg<2, float>(Context*, const SolveOpts&, int*, double*);
+template int run_pcg_amg<3, float>(Context*, const SolveOpts&, int*, double*);
+
+int amg_levels(Context* c, int* out, int cap) {
+  if (!c->amg || !c->amg->ok) return 0;
+  int k = 0;
+  if (k < cap) out[k++] = c->N * c->dim;
+  for (auto& L : c->amg->lv)
+    if (k < cap) out[k++] = L.N * c->dim;
+  return k;
+}
+
+}  // namespace lpmg

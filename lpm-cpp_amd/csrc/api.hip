@@ -177,6 +177,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   rc |= dalloc(c, &c->cs_sum, 6 * N);
   rc |= dalloc(c, &c->Kn, 2 * N);
   rc |= dalloc(c, &c->type, N);
+  rc |= dalloc(c, &c->floating, N);
   rc |= dalloc(c, &c->cons, 3 * N);
   for (double** p : {&c->residual, &c->du, &c->cg_r, &c->cg_p, &c->cg_q, &c->cg_z, &c->cg_w, &c->dinv})
     rc |= dalloc(c, p, d * N);
@@ -201,6 +202,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   API_HIP(hipMemsetAsync(c->sv, 0, sizeof(double) * 7 * N, c->stream));
   API_HIP(hipMemsetAsync(c->sv_last, 0, sizeof(double) * 7 * N, c->stream));
   API_HIP(hipMemsetAsync(c->type, 0, sizeof(int) * N, c->stream));
+  API_HIP(hipMemsetAsync(c->floating, 0, sizeof(int) * N, c->stream));
   API_HIP(hipMemsetAsync(c->cons, 0, sizeof(int) * 3 * N, c->stream));
   for (double* p : {c->residual, c->du, c->cg_r, c->cg_p, c->cg_q, c->cg_z, c->cg_w, c->dinv})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * d * N, c->stream));
@@ -240,6 +242,7 @@ int free_context(lpmg_context* h) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (hipGraphExec_t& g : c->cg_graph)
     if (g) (void)hipGraphExecDestroy(g);
+  amg_free(c);
   for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->sc_host) (void)hipHostFree(c->sc_host);
@@ -314,6 +317,7 @@ int do_newton(lpmg_context* h, int max_iter, double tol, const SolveOpts& so, do
   double cgit = 0;
   while (nr > tol * mult) {
     if (++ni > max_iter) { ni = max_iter; break; }
+    if (!std::isfinite(nr)) break;  // reported below
     double st[2] = {0, 0};
     int rc = do_solve_linear(h, so, st);
     if (rc && rc != -4) return rc;  // a CG that stops at maxit keeps its partial update, as MKL's
@@ -323,6 +327,12 @@ int do_newton(lpmg_context* h, int max_iter, double tol, const SolveOpts& so, do
     if ((rc = do_update_rr(h, &nr, &dummy))) return rc;
   }
   if (stats) { stats[0] = cgit; stats[1] = nr; stats[2] = mult; stats[3] = ni; }
+  // A NaN residual would pass the loop test above as converged (the
+  // reference's test, solver.h:74, has the same blind spot): report it.
+  if (!std::isfinite(nr) || !std::isfinite(mult)) {
+    set_error("Newton residual is not finite");
+    return -4;
+  }
   return ni;
 }
 
@@ -333,7 +343,7 @@ size_t field_len(const Context* c, int f) {
     case LPMG_F_CONSTRAINT: return 3 * N;
     case LPMG_F_STRESS: case LPMG_F_CS_SUM: return 6 * N;
     case LPMG_F_DAMAGE: case LPMG_F_DAMAGE_VISUAL: case LPMG_F_DDOT_LOCAL: case LPMG_F_DDOT_NONLOCAL:
-    case LPMG_F_DAMAGE_LAST: case LPMG_F_TYPE: return N;
+    case LPMG_F_DAMAGE_LAST: case LPMG_F_TYPE: case LPMG_F_FLOATING: return N;
     case LPMG_F_STATE_VAR: return 7 * N;
     case LPMG_F_RESIDUAL: return (size_t)c->dim * N;
     case LPMG_F_BDAMAGE: case LPMG_F_DLP: case LPMG_F_BFORCE: case LPMG_F_DIS: case LPMG_F_BSTRAIN:
@@ -719,10 +729,11 @@ int lpmg_get_field(lpmg_handle h, int f, double* out) {
     unpack_xyz_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(p, tmp, c->N);
     API_HIP(hipMemcpyAsync(out, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     API_HIP(hipFreeAsync(tmp, c->stream));
-  } else if (f == LPMG_F_CONSTRAINT || f == LPMG_F_TYPE) {
+  } else if (f == LPMG_F_CONSTRAINT || f == LPMG_F_TYPE || f == LPMG_F_FLOATING) {
     double* tmp = nullptr;
     API_HIP(hipMallocAsync((void**)&tmp, sizeof(double) * n, c->stream));
-    i2d_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(f == LPMG_F_TYPE ? c->type : c->cons, tmp, n);
+    const int* src = f == LPMG_F_TYPE ? c->type : f == LPMG_F_FLOATING ? c->floating : c->cons;
+    i2d_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(src, tmp, n);
     API_HIP(hipMemcpyAsync(out, tmp, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     API_HIP(hipFreeAsync(tmp, c->stream));
   } else {
@@ -734,6 +745,7 @@ int lpmg_get_field(lpmg_handle h, int f, double* out) {
 
 int lpmg_set_field(lpmg_handle h, int f, const double* in) {
   CHECK_HANDLE(h);
+  h->c.floating_dirty = true;
   Context* c = &h->c;
   const size_t n = field_len(c, f);
   const bool writable = f == LPMG_F_XYZ || f == LPMG_F_XYZ_LAST || f == LPMG_F_PEX || f == LPMG_F_DAMAGE ||
@@ -773,6 +785,8 @@ int lpmg_restart(lpmg_handle h) {
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * NB, c->stream));
   API_HIP(hipMemcpyAsync(c->dis, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));
   API_HIP(hipMemcpyAsync(c->dis_last, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));
+  API_HIP(hipMemsetAsync(c->floating, 0, sizeof(int) * N, c->stream));
+  c->floating_dirty = true;
   c->reac_sumsq = 0;
   c->assembled64 = c->assembled32 = false;
   API_HIP(hipStreamSynchronize(c->stream));
@@ -828,6 +842,12 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   out[1] = which == 0 ? spmv_bytes(c, 0) : which == 1 ? spmv_bytes(c, 1)
            : which == 2 ? assembly_bytes(c) : which == 3 ? forward_bytes(c) : 16.0 * (double)ncopy;
   return rc;
+}
+
+int lpmg_multigrid_levels(lpmg_handle h, int32_t* out, int cap) {
+  CHECK_HANDLE(h);
+  if (cap < 0 || (cap > 0 && !out)) { set_error("bad output"); return -1; }
+  return amg_levels(&h->c, out, cap);
 }
 
 int lpmg_synchronize(lpmg_handle h) {

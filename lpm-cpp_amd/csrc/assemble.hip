@@ -162,15 +162,12 @@ __global__ void __launch_bounds__(256) norm_diag_finish_kernel(const double* __r
   if (threadIdx.x == 0) sc->norm_diag = sqrt(tot.v[0]);
 }
 
-__device__ __forceinline__ bool dof_fixed(const int* cons, const double* dvis, int i, int k) {
-  return cons[3 * i + k] == 1 || fabs(dvis[i] - 1.0) < kEPS;
-}
-
 template <int D>
 __global__ void __launch_bounds__(256)
 symmetrize_bc_kernel(const int* __restrict__ brow, const int* __restrict__ conns,
                      const int* __restrict__ tpos, int64_t nnzb, const int* __restrict__ cons,
-                     const double* __restrict__ dvis, const Scalars* __restrict__ sc,
+                     const double* __restrict__ dvis, const int* __restrict__ floating,
+                     const Scalars* __restrict__ sc,
                      double* __restrict__ Kval, float* __restrict__ Kval32) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nnzb) return;
@@ -180,8 +177,8 @@ symmetrize_bc_kernel(const int* __restrict__ brow, const int* __restrict__ conns
   bool fi[D], fj[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    fi[k] = dof_fixed(cons, dvis, i, k);
-    fj[k] = dof_fixed(cons, dvis, j, k);
+    fi[k] = dof_fixed(cons, dvis, floating, i, k);
+    fj[k] = dof_fixed(cons, dvis, floating, j, k);
   }
   double* Ke = Kval + (size_t)e * D * D;
   if (i == j) {
@@ -270,6 +267,8 @@ int launch_assemble(Context* c, int want32) {
     return -1;
   }
   if (want32 && !c->Kval32 && dalloc(c, &c->Kval32, (size_t)c->nnzb * c->dim * c->dim)) return -3;
+  if (c->floating_dirty)
+    if (int rc = update_floating(c)) return rc;
   float* k32 = want32 ? c->Kval32 : nullptr;
   if (c->dim == 3) {
     tangent_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
@@ -278,7 +277,7 @@ int launch_assemble(Context* c, int want32) {
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     symmetrize_bc_kernel<3><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
-                                                      c->damage_visual, c->sc, c->Kval, k32);
+                                                      c->damage_visual, c->floating, c->sc, c->Kval, k32);
   } else {
     tangent_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                          c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
@@ -286,11 +285,12 @@ int launch_assemble(Context* c, int want32) {
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     symmetrize_bc_kernel<2><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
-                                                      c->damage_visual, c->sc, c->Kval, k32);
+                                                      c->damage_visual, c->floating, c->sc, c->Kval, k32);
   }
   LPMG_HIP(hipGetLastError());
   c->assembled64 = true;
   c->assembled32 = want32 != 0;
+  c->asm_version += 1;
   return 0;
 }
 

@@ -125,15 +125,17 @@ particle_force_kernel(const int* __restrict__ bptr, const int* __restrict__ rev,
 // only ever read through its norm).
 __global__ void __launch_bounds__(256)
 update_rr_kernel(const double* __restrict__ Pex, const double* __restrict__ Pin,
-                 const int* __restrict__ cons, int N, int d, double* __restrict__ residual,
-                 double* __restrict__ partials, Scalars* sc) {
+                 const int* __restrict__ cons, const int* __restrict__ floating, int N, int d,
+                 double* __restrict__ residual, double* __restrict__ partials, Scalars* sc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double rr = 0, reac = 0;
   if (i < N) {
+    const int fl = floating[i];
     for (int k = 0; k < d; ++k) {
       const int cn = cons[3 * i + k];
       const double pin = Pin[3 * i + k];
-      const double r = (1 - cn) * (Pex[3 * i + k] - pin);
+      // a floating fragment's particles are held where they are (no residual)
+      const double r = fl ? 0.0 : (1 - cn) * (Pex[3 * i + k] - pin);
       residual[(size_t)d * i + k] = r;
       rr += r * r;
       if (cn == 1) reac += pin * pin;
@@ -217,7 +219,7 @@ int launch_update_rr(Context* c, double* norm_res, double* reac_sq) {
     set_error("partials buffer too small");
     return -1;
   }
-  update_rr_kernel<<<nb, tb, 0, c->stream>>>(c->Pex, c->Pin, c->cons, N, c->dim, c->residual,
+  update_rr_kernel<<<nb, tb, 0, c->stream>>>(c->Pex, c->Pin, c->cons, c->floating, N, c->dim, c->residual,
                                              c->partials, c->sc);
   update_rr_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nb, c->sc);
   LPMG_HIP(hipGetLastError());
@@ -230,6 +232,7 @@ int launch_update_rr(Context* c, double* norm_res, double* reac_sq) {
 
 int launch_disp_bc(Context* c, const Group& g, int mode, int k, double step) {
   if (g.n == 0) return 0;
+  c->floating_dirty = true;  // the constrained set may grow
   disp_bc_kernel<<<(g.n + 255) / 256, 256, 0, c->stream>>>(c->xyz, c->cons, g.idx, g.n, mode, k, step);
   LPMG_HIP(hipGetLastError());
   return 0;

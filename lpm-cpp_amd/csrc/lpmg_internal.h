@@ -90,6 +90,7 @@ constexpr int kCgChunk = 16;     // CG iterations per launch batch (even)
 constexpr int kCgClasses = 8;
 
 struct Context;
+struct Amg;  // multigrid hierarchy (amg.hip)
 // Runs the iterations of a CG whose start cg_begin has enqueued on c->stream;
 // returns after ordering c->stream behind the system's last iteration.
 typedef int (*CgExec)(void* arg, Context* c, const CgSys& q);
@@ -159,6 +160,14 @@ struct Context {
   void* cg_exec_arg = nullptr;
   hipEvent_t cg_ready = nullptr, cg_released = nullptr;  // CG start / end of its last fused launch
   size_t bytes = 0;  // device memory held by the context
+  int64_t asm_version = 0;  // number of assemblies so far (multigrid values follow it)
+  // fragments: component label per particle (intact bonds), 1 for particles of
+  // components without a displacement constraint; recomputed at the next
+  // assembly after bonds broke or constraints changed
+  int *comp = nullptr, *compfix = nullptr, *floating = nullptr, *changed = nullptr;
+  bool floating_dirty = true;
+  int n_floating = 0;
+  Amg* amg = nullptr;
 };
 
 // ---- helpers (api.hip)
@@ -183,6 +192,15 @@ int dalloc(Context* c, T** p, size_t n) {
   c->bytes += n * sizeof(T);
   *p = static_cast<T*>(q);
   return 0;
+}
+
+// A DOF eliminated from the linear solve: a displacement constraint or a
+// fully damaged particle, as Stiffness::updateStiffnessDispBC
+// (stiffness.h:368-410), or a particle of a fragment that intact bonds no
+// longer tie to any constrained particle (`floating`, no counterpart in the
+// reference, whose stiffness is singular then).
+__device__ __forceinline__ bool dof_fixed(const int* cons, const double* dvis, const int* floating, int i, int k) {
+  return cons[3 * i + k] == 1 || fabs(dvis[i] - 1.0) < kEPS || floating[i] != 0;
 }
 
 // Copy on the context's stream and wait for it.  Never the null stream: that
@@ -218,13 +236,18 @@ int launch_export_csr(Context* c, int64_t* dKp, int64_t* dIK, int64_t* dJK, doub
 
 // ---- solver.hip
 struct SolveOpts {
-  int precond = 1;      // 0 plain CG (the reference's MKL call), 1 Jacobi PCG
+  int precond = 1;      // 0 plain CG (the reference's MKL call), 1 Jacobi PCG, 2 multigrid PCG
   int precision = 0;    // 0 FP64, 1 FP32 inner solve
   double rtol = 1e-12, atol = 1e-12;  // ||r||^2 <= rtol*||r0||^2 + atol
   int64_t maxit = 0;    // 0 -> problem size
   int chunk = 16;       // unused: the chunk length is fixed (graph capture)
 };
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
+// ---- amg.hip
+template <int D, typename T>
+int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final);
+void amg_free(Context* c);
+int amg_levels(Context* c, int* out, int cap);
 int cg_class(int dim, int fp32, int lpr);
 void cg_batch_layout(CgSys* tab, int nsys, CgBatchHdr* h);
 int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int nclass);
@@ -235,6 +258,7 @@ double spmv_bytes(const Context* c, int precision);
 // ---- state.hip
 int launch_update_state_var(Context* c);
 int launch_static_damage(Context* c, int undamaged, int* any);
+int update_floating(Context* c);
 int store_state(Context* c);
 int reset_state(Context* c, int reset_xyz);
 

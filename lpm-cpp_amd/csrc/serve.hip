@@ -504,7 +504,15 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
     off += r.group_size[g];
   }
   if (r.nonlocal_ratio > 0 && (rc = lpmg_search_nonlocal(h, r.nonlocal_ratio))) return rc;
-  if (S->batching && (S->fuse_max_rows < 0 || (int64_t)c->N <= S->fuse_max_rows)) {
+  // the fused launches run the Jacobi CG; multigrid solves use the handle's
+  // stream.  opts[0] < 0 lets the server choose: fused Jacobi for the systems
+  // it fuses, multigrid for the larger ones (where Jacobi needs thousands of
+  // iterations per solve).
+  const bool fusable = S->batching && (S->fuse_max_rows < 0 || (int64_t)c->N <= S->fuse_max_rows);
+  double opts[8];
+  std::memcpy(opts, r.opts, sizeof(opts));
+  if (opts[0] < 0) opts[0] = fusable ? 1 : 2;
+  if (fusable && opts[0] <= 1) {
     c->cg_exec = batcher_exec;
     c->cg_exec_arg = &S->batcher;
   }
@@ -526,7 +534,7 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
       if (x.kind == 0 && (rc = lpmg_disp_bc(h, gid[x.group], x.mode, x.flag, x.value[0] * sc))) return rc;
     }
     double st[3] = {0, 0, 0};
-    rc = lpmg_solve_step(h, r.undamaged, r.max_iter, r.tol, r.opts, st);
+    rc = lpmg_solve_step(h, r.undamaged, r.max_iter, r.tol, opts, st);
     if (rc < 0) return rc;
     if (rc == 0) {  // not converged: restore the last converged state, halve the step
       if (halvings >= r.max_halvings) {

@@ -171,6 +171,107 @@ int launch_static_damage(Context* c, int undamaged, int* any) {
   LPMG_HIP(hipStreamSynchronize(c->stream));
   LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
   *any = h;
+  if (h) c->floating_dirty = true;
+  return 0;
+}
+
+namespace {
+
+// ---- fragments: connected components of the intact-bond graph
+// Labels start as the particle index; each round hooks the larger of two
+// labels met across an intact bond onto the smaller (atomicMin on the label's
+// own entry) and then shortens every particle's label chain to its root.
+// Labels only decrease, so the rounds end; at the end every component carries
+// its smallest particle index, whatever the order of the atomics.
+__global__ void comp_init_kernel(int* __restrict__ comp, int* __restrict__ compfix, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) {
+    comp[i] = i;
+    compfix[i] = 0;
+  }
+}
+
+__global__ void comp_hook_kernel(const int* __restrict__ bptr, const int* __restrict__ p2,
+                                 const double* __restrict__ bdamage, int N, int* comp, int* __restrict__ changed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  for (int b = bptr[i]; b < bptr[i + 1]; ++b) {
+    if (fabs(bdamage[b] - 1.0) < kEPS) continue;  // broken bond
+    const int li = __atomic_load_n(&comp[i], __ATOMIC_RELAXED);
+    const int lj = __atomic_load_n(&comp[p2[b]], __ATOMIC_RELAXED);
+    if (li == lj) continue;
+    const int lo = li < lj ? li : lj, hi = li < lj ? lj : li;
+    atomicMin(&comp[hi], lo);
+    *changed = 1;
+  }
+}
+
+__global__ void comp_jump_kernel(int* comp, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int l = __atomic_load_n(&comp[i], __ATOMIC_RELAXED);
+  for (;;) {
+    const int m = __atomic_load_n(&comp[l], __ATOMIC_RELAXED);
+    if (m == l) break;
+    l = m;
+  }
+  atomicMin(&comp[i], l);
+}
+
+__global__ void comp_fixed_kernel(const int* __restrict__ comp, const int* __restrict__ cons, int N, int d,
+                                  int* __restrict__ compfix, int* __restrict__ anyfix) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  bool c = false;
+  for (int k = 0; k < d; ++k) c = c || cons[3 * i + k] == 1;
+  if (c) {
+    compfix[comp[i]] = 1;
+    *anyfix = 1;
+  }
+}
+
+// With no displacement constraint anywhere (a pure traction problem) nothing
+// is held: the stiffness is singular as the user posed it.
+__global__ void comp_floating_kernel(const int* __restrict__ comp, const int* __restrict__ compfix,
+                                     const int* __restrict__ anyfix, int N, int* __restrict__ floating,
+                                     int* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int f = (*anyfix && !compfix[comp[i]]) ? 1 : 0;
+  floating[i] = f;
+  if (f) atomicAdd(count, 1);
+}
+
+}  // namespace
+
+int update_floating(Context* c) {
+  const int N = c->N, tb = 256, nb = (N + tb - 1) / tb;
+  if (!c->comp) {
+    if (dalloc(c, &c->comp, N) || dalloc(c, &c->compfix, N) || dalloc(c, &c->changed, 3)) return -3;
+  }
+  comp_init_kernel<<<nb, tb, 0, c->stream>>>(c->comp, c->compfix, N);
+  int h[2] = {0, 0};
+  int rounds = 0;
+  for (;; ++rounds) {
+    if (rounds > 4096) {
+      set_error("fragment labelling did not settle");
+      return -1;
+    }
+    LPMG_HIP(hipMemsetAsync(c->changed, 0, sizeof(int), c->stream));
+    comp_hook_kernel<<<nb, tb, 0, c->stream>>>(c->bptr, c->p2, c->bdamage, N, c->comp, c->changed);
+    comp_jump_kernel<<<nb, tb, 0, c->stream>>>(c->comp, N);
+    LPMG_HIP(hipGetLastError());
+    LPMG_HIP(copy_sync(c, h, c->changed, sizeof(int), hipMemcpyDeviceToHost));
+    if (!h[0]) break;
+  }
+  LPMG_HIP(hipMemsetAsync(c->changed + 1, 0, 2 * sizeof(int), c->stream));
+  comp_fixed_kernel<<<nb, tb, 0, c->stream>>>(c->comp, c->cons, N, c->dim, c->compfix, c->changed + 2);
+  comp_floating_kernel<<<nb, tb, 0, c->stream>>>(c->comp, c->compfix, c->changed + 2, N, c->floating,
+                                                 c->changed + 1);
+  LPMG_HIP(hipGetLastError());
+  LPMG_HIP(copy_sync(c, h + 1, c->changed + 1, sizeof(int), hipMemcpyDeviceToHost));
+  c->n_floating = h[1];
+  c->floating_dirty = false;
   return 0;
 }
 
@@ -192,6 +293,7 @@ int store_state(Context* c) {
 
 int reset_state(Context* c, int reset_xyz) {
   const size_t N = c->N, NB = c->NB;
+  c->floating_dirty = true;
   auto cp = [&](void* d, const void* s, size_t bytes) {
     return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream);
   };

@@ -22,7 +22,7 @@ _lib = None
 FIELDS = dict(XYZ=0, XYZ_INITIAL=1, XYZ_LAST=2, PIN=3, PEX=4, STRESS=5, DAMAGE=6,
               DAMAGE_VISUAL=7, STATE_VAR=8, RESIDUAL=9, BDAMAGE=10, DLP=11, BFORCE=12,
               DIS=13, CS=14, CONSTRAINT=15, DDOT_LOCAL=16, DDOT_NONLOCAL=17, DLE_TOTAL=18,
-              CS_SUM=19, BSTRAIN=20, DLE=21, DAMAGE_LAST=22, TYPE=23, TDLE_TOTAL=24)
+              CS_SUM=19, BSTRAIN=20, DLE=21, DAMAGE_LAST=22, TYPE=23, TDLE_TOTAL=24, FLOATING=25)
 
 # every symbol include/lpm_gpu.h declares
 EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "lpmg_create_with_bonds",
@@ -33,7 +33,7 @@ EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "
            "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_restart", "lpmg_field_len",
            "lpmg_get_field", "lpmg_set_field", "lpmg_profile", "lpmg_profile_read", "lpmg_time_kernel",
            "lpmg_synchronize", "lpmg_request_bytes", "lpmg_server_create", "lpmg_server_submit",
-           "lpmg_server_wait", "lpmg_server_info", "lpmg_server_destroy")
+           "lpmg_server_wait", "lpmg_server_info", "lpmg_server_destroy", "lpmg_multigrid_levels")
 
 
 class LpmgError(RuntimeError):
@@ -94,6 +94,7 @@ def lib():
         L.lpmg_profile.argtypes = [vp, ctypes.c_int]
         L.lpmg_profile_read.argtypes = [vp, dp]
         L.lpmg_time_kernel.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+        L.lpmg_multigrid_levels.argtypes = [vp, i32p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -273,6 +274,11 @@ class System:
         self._chk(lib().lpmg_profile_read(self.h, _p(out, ctypes.c_double)))
         return dict(spmv_ms=out[0], spmv_launches=int(out[1]), spmv_bytes_fp64=out[2],
                     spmv_bytes_fp32=out[3])
+
+    def multigrid_levels(self):
+        out = np.zeros(32, dtype=np.int32)
+        k = self._chk(lib().lpmg_multigrid_levels(self.h, _p(out, ctypes.c_int32), 32))
+        return out[:k].tolist()
 
     def time_kernel(self, which, reps=20):
         out = np.zeros(2)

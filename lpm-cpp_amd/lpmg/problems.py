@@ -156,9 +156,13 @@ def ct_plate(n_target=20000, a_over_w=0.35, steps=30, opening=2.5e-4, W=50.0, E=
     particles: ParticleElastic bonds break once their stretch reaches `crit`
     (particle_elastic.h:52-68).  The pin particles (first particle ring of the
     upper half of the top hole and the lower half of the bottom hole) move
-    +-opening*W in y per step; the back-edge particle on the crack line is
-    fixed.  Particles within 0.2 W of a pin centre are type 4, the undamaged
-    type, as the pin regions of examples/ex4_isoelasticity_2d_crack.cpp:47-69."""
+    +-opening*W in y per step and are held in x; the back-edge particle on the
+    crack line is fixed.  Particles within 0.2 W of a pin centre are type 4, the
+    undamaged type, as the pin regions of
+    examples/ex4_isoelasticity_2d_crack.cpp:47-69.  Holding the pins in x keeps
+    each half fully constrained once the crack has cut the ligament (the
+    example's pin particles are held in y only, which leaves a separated half
+    a free rigid motion and the stiffness singular)."""
     from . import geometry as geo
     ct = geo.ct_specimen(n_target, a_over_w, W=W)
     xyz, r = ct["xyz"], ct["radius"]
@@ -174,7 +178,8 @@ def ct_plate(n_target=20000, a_over_w=0.35, steps=30, opening=2.5e-4, W=50.0, E=
     types[(dt < 0.2 * W) | (db < 0.2 * W)] = 4
     d = opening * W
     step = [("disp", "fix", RELATIVE, "x", 0.0), ("disp", "fix", RELATIVE, "y", 0.0),
-            ("disp", "top", RELATIVE, "y", d), ("disp", "bottom", RELATIVE, "y", -d)]
+            ("disp", "top", RELATIVE, "x", 0.0), ("disp", "top", RELATIVE, "y", d),
+            ("disp", "bottom", RELATIVE, "x", 0.0), ("disp", "bottom", RELATIVE, "y", -d)]
     p = Problem("ct_plate", lat.HEXAGON2D, r, lat.ELASTIC, xyz, (MAT_ELASTIC, [0.0, 1, E, nu, crit]),
                 dict(fix=fix, top=top, bottom=bottom), _bc_steps(steps, step), types=types, undamaged=4,
                 max_iter=max_iter, tol=tol,

@@ -161,3 +161,21 @@ def test_two_level_reduction_in_fused_launch():
         res = [srv.wait(t) for t in [srv.submit(q) for q in reqs]]
     for q, rr, (ref, rs) in zip(reqs, res, refs):
         _same(q, rr, ref, rs)
+
+
+def test_server_picks_solver_by_size():
+    """precond < 0: requests up to the fusion threshold run the fused Jacobi CG,
+    larger ones the multigrid CG on their own stream -- each bit-identical to
+    the same solve on its own handle with that preconditioner."""
+    small = P.ct_plate(2000, 0.30, steps=12, opening=3e-5)
+    large = P.hex_plate(60, 60, steps=3, total=0.003)
+    auto = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1)
+    ref_small = _direct(small, dict(auto, precond=1))
+    ref_large = _direct(large, dict(auto, precond=2))
+    reqs = [_request(small, auto), _request(large, auto)]
+    with Server(max_active=2, batching=3000) as srv:
+        res = [srv.wait(t) for t in [srv.submit(q) for q in reqs]]
+        info = srv.info()
+    _same(reqs[0], res[0], *ref_small)
+    _same(reqs[1], res[1], *ref_large)
+    assert info["chunks"] > 0
