@@ -288,7 +288,8 @@ typedef struct lpmg_request {
   double tol;                  /* Newton tolerance */
   double opts[8];              /* solver options as lpmg_solve_linear; opts[0] < 0:
                                   the server picks fused Jacobi for requests it
-                                  fuses, multigrid for the others */
+                                  fuses, multigrid for larger 2D requests and
+                                  Jacobi for larger 3D ones */
   int max_halvings;            /* step halvings allowed before the request fails */
   int n_out;
   const lpmg_out_field* out;

@@ -506,12 +506,14 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
   if (r.nonlocal_ratio > 0 && (rc = lpmg_search_nonlocal(h, r.nonlocal_ratio))) return rc;
   // the fused launches run the Jacobi CG; multigrid solves use the handle's
   // stream.  opts[0] < 0 lets the server choose: fused Jacobi for the systems
-  // it fuses, multigrid for the larger ones (where Jacobi needs thousands of
-  // iterations per solve).
+  // it fuses; for the larger ones multigrid in 2D (Jacobi needs thousands of
+  // iterations per solve there; measured 3.5-4x faster at 1e5-2.5e5
+  // particles) and Jacobi in 3D (a V-cycle costs about two SpMVs, more than
+  // the iterations it saves on the 3D lattices measured).
   const bool fusable = S->batching && (S->fuse_max_rows < 0 || (int64_t)c->N <= S->fuse_max_rows);
   double opts[8];
   std::memcpy(opts, r.opts, sizeof(opts));
-  if (opts[0] < 0) opts[0] = fusable ? 1 : 2;
+  if (opts[0] < 0) opts[0] = (fusable || c->dim == 3) ? 1 : 2;
   if (fusable && opts[0] <= 1) {
     c->cg_exec = batcher_exec;
     c->cg_exec_arg = &S->batcher;

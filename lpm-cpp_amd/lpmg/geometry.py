@@ -201,3 +201,104 @@ def read_bond(path):
         z = np.zeros(0, dtype=np.int32)
         return z, z, z
     return a[:, 1].astype(np.int32), a[:, 2].astype(np.int32), a[:, 3].astype(np.int32)
+
+
+# ---- BCC polycrystal ---------------------------------------------------------
+
+def random_rotations(n, rng):
+    """Uniformly distributed rotations (unit quaternions, Shoemake's method)."""
+    u1, u2, u3 = rng.random(n), rng.random(n), rng.random(n)
+    q = np.stack([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)], axis=1)
+    x, y, z, w = q.T
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (y * y + z * z); R[:, 0, 1] = 2 * (x * y - z * w); R[:, 0, 2] = 2 * (x * z + y * w)
+    R[:, 1, 0] = 2 * (x * y + z * w); R[:, 1, 1] = 1 - 2 * (x * x + z * z); R[:, 1, 2] = 2 * (y * z - x * w)
+    R[:, 2, 0] = 2 * (x * z - y * w); R[:, 2, 1] = 2 * (y * z + x * w); R[:, 2, 2] = 1 - 2 * (x * x + y * y)
+    return R
+
+
+def bcc_polycrystal(n_cells=80, n_grains=200, radius=0.1, seed=7, offset=1.3, threshold=1.3, min_bonds=6):
+    """Voronoi BCC polycrystal in a cube of n_cells BCC cells per edge.
+
+    The algorithm of benchmark/BCC_Hailong/create_BCC_configuration.F90:169-333:
+    per grain, a BCC lattice (lattice constant 4r/sqrt(3), a corner and a body
+    centre per cell) is rotated by the grain's orientation and centred on its
+    seed; a point is kept when it is closer to its own seed than to every other
+    seed by less than `offset` r (the cell widened a little) and lies in the
+    box; grains are added in order, and a point closer than `threshold` r to a
+    point of an earlier grain is dropped (that earlier point is marked a
+    grain-boundary particle).  Seeds are uniform in the box, orientations
+    uniform over SO(3), both from `seed`.  The nearest-seed test uses a k-d tree
+    instead of the reference's neighbour-seed list.  Last, particles with fewer
+    than `min_bonds` neighbours inside the second-shell cutoff are removed,
+    repeatedly until none is left: a grain-boundary particle held by two or
+    three central bonds is a mechanism that makes the stiffness singular.
+    Returns xyz, the grain of each particle (n_grains for grain-boundary
+    particles), seeds, rotations."""
+    from scipy.spatial import cKDTree
+    a = 4.0 * radius / np.sqrt(3.0)
+    L = n_cells * a
+    rng = np.random.default_rng(seed)
+    seeds = rng.random((n_grains, 3)) * L
+    rots = random_rotations(n_grains, rng)
+    tree = cKDTree(seeds)
+    # the lattice block around each seed: a ball that covers its Voronoi cell
+    # (cells farther than 3 mean grain radii from their seed are not expected)
+    rg = 3.0 * (3.0 * L ** 3 / (4.0 * np.pi * n_grains)) ** (1.0 / 3.0)
+    m = int(np.ceil(rg / a)) + 1
+    g = np.arange(-m, m + 1, dtype=np.float64) * a
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    corner = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    block = np.concatenate([corner, corner + 0.5 * a], axis=0)
+    block = block[np.einsum("ij,ij->i", block, block) <= rg * rg]
+    pts, grain = [], []
+    for k in range(n_grains):
+        p = block @ rots[k].T + seeds[k]
+        inside = np.all((p >= 0.0) & (p <= L), axis=1)
+        p = p[inside]
+        d, idx = tree.query(p, k=2)
+        d_own = np.linalg.norm(p - seeds[k], axis=1)
+        other = np.where(idx[:, 0] == k, d[:, 1], d[:, 0])
+        keep = other > d_own - offset * radius
+        pts.append(p[keep])
+        grain.append(np.full(int(keep.sum()), k, dtype=np.int32))
+    xyz = np.concatenate(pts, axis=0)
+    grain = np.concatenate(grain)
+    # overlaps: grains in order; a point within threshold*r of a kept point of
+    # an earlier grain is dropped
+    pairs = cKDTree(xyz).query_pairs(threshold * radius, output_type="ndarray")
+    gi, gj = grain[pairs[:, 0]], grain[pairs[:, 1]]
+    cross = gi != gj
+    pairs = pairs[cross]
+    early = np.where(grain[pairs[:, 0]] < grain[pairs[:, 1]], pairs[:, 0], pairs[:, 1])
+    late = np.where(grain[pairs[:, 0]] < grain[pairs[:, 1]], pairs[:, 1], pairs[:, 0])
+    order = np.argsort(grain[late], kind="stable")
+    early, late = early[order], late[order]
+    kept = np.ones(len(xyz), dtype=bool)
+    boundary = np.zeros(len(xyz), dtype=bool)
+    lg = grain[late]
+    starts = np.searchsorted(lg, np.arange(n_grains + 1))
+    for k in range(n_grains):
+        e, l = early[starts[k]:starts[k + 1]], late[starts[k]:starts[k + 1]]
+        hit = kept[e]
+        kept[l[hit]] = False
+        boundary[e[hit]] = True
+    grain = np.where(boundary, n_grains, grain)
+    xyz, grain = xyz[kept], grain[kept]
+    if min_bonds > 0:
+        rc = 1.01 * lat.unit_cell_info(lat.BCC3D, radius)["cut"][1]
+        pr = cKDTree(xyz).query_pairs(rc, output_type="ndarray")
+        nb = np.bincount(pr.ravel(), minlength=len(xyz))
+        alive = np.ones(len(xyz), dtype=bool)
+        while True:
+            weak = alive & (nb < min_bonds)
+            if not weak.any():
+                break
+            alive &= ~weak
+            for a_, b_ in ((0, 1), (1, 0)):  # partners of the removed lose a bond
+                hit = weak[pr[:, a_]]
+                np.subtract.at(nb, pr[hit, b_], 1)
+            pr = pr[~(weak[pr[:, 0]] | weak[pr[:, 1]])]
+        xyz, grain = xyz[alive], grain[alive]
+    return dict(xyz=xyz, grain=grain, seeds=seeds, rotations=rots, L=L, radius=radius)

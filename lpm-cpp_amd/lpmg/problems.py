@@ -188,3 +188,29 @@ def ct_plate(n_target=20000, a_over_w=0.35, steps=30, opening=2.5e-4, W=50.0, E=
     p.extra["bonds"] = ct["bonds"]
     p.extra["ct"] = dict(W=W, a=ct["a"], yc=ct["yc"], radius=r)
     return p
+
+
+def bcc_polycrystal(n_cells=80, n_grains=200, radius=0.1, steps=10, strain_step=1e-4, C11=230e3, C12=135e3,
+                    C44=117e3, seed=7):
+    """3D BCC Voronoi polycrystal (lpmg.geometry.bcc_polycrystal) with cubic
+    elasticity C11/C12/C44 (MPa) in every grain -- the grains differ only in
+    the orientation of their lattice -- through setParticleProperty(L, C11,
+    C12, C44, critical_bstrain) (particle_elastic.h:110-140).  Bottom face
+    fixed, top face pulled `strain_step` of the height per step."""
+    from . import geometry as geo
+    pc = geo.bcc_polycrystal(n_cells, n_grains, radius, seed=seed)
+    xyz = pc["xyz"]
+    z = xyz[:, 2]
+    a = 4.0 * radius / math.sqrt(3.0)
+    bottom = np.nonzero(z < z.min() + 0.5 * a)[0]
+    top = np.nonzero(z > z.max() - 0.5 * a)[0]
+    d = strain_step * (z.max() - z.min())
+    step = [("disp", "bottom", RELATIVE, "x", 0.0), ("disp", "bottom", RELATIVE, "y", 0.0),
+            ("disp", "bottom", RELATIVE, "z", 0.0), ("disp", "top", RELATIVE, "z", d)]
+    p = Problem("bcc_polycrystal", lat.BCC3D, radius, lat.ELASTIC, xyz,
+                (MAT_ELASTIC_CUBIC, [0.0, C11, C12, C44, 1.0]), dict(bottom=bottom, top=top),
+                _bc_steps(steps, step),
+                note=f"BCC polycrystal {n_cells}^3 cells, {n_grains} grains (seed {seed}), N={len(xyz)}, "
+                     f"cubic C11/C12/C44 = {C11}/{C12}/{C44} MPa, tension {strain_step}/step x {steps}")
+    p.extra["grain"] = pc["grain"]
+    return p

@@ -82,3 +82,36 @@ def test_dump_and_bond_files_round_trip(tmp_path):
     with open(os.path.join(tmp_path, "s.bond")) as f:
         first = f.readline().split()
     assert first == ["0", str(layer[0]), str(p1[0]), str(p2[0])]
+
+
+def test_bcc_polycrystal_structure():
+    """Voronoi BCC polycrystal (create_BCC_configuration.F90's algorithm): the
+    rotations are proper, points keep the overlap threshold, the particle count
+    is the lattice's, grain interiors are rotated BCC lattices (nearest
+    neighbour at 2r), each point sits in (or within the offset of) its grain's
+    Voronoi cell, and the result is reproducible from the seed."""
+    from scipy.spatial import cKDTree
+    r = 0.1
+    pc = G.bcc_polycrystal(12, 10, radius=r, seed=3)
+    R = pc["rotations"]
+    assert np.allclose(np.einsum("kij,kil->kjl", R, R), np.eye(3)[None], atol=1e-12)
+    assert np.allclose(np.linalg.det(R), 1.0)
+    xyz, grain = pc["xyz"], pc["grain"]
+    n_ideal = 2 * 12 ** 3
+    assert 0.9 * n_ideal < len(xyz) < 1.05 * n_ideal
+    L = pc["L"]
+    assert xyz.min() >= 0.0 and xyz.max() <= L
+    d, _ = cKDTree(xyz).query(xyz, k=2)
+    assert d[:, 1].min() >= 1.3 * r - 1e-12
+    interior = grain < 10
+    seeds = pc["seeds"]
+    dist = np.linalg.norm(xyz[:, None, :] - seeds[None, :, :], axis=2)
+    own = dist[np.arange(len(xyz))[interior], grain[interior]]
+    assert np.all(own <= dist[interior].min(axis=1) + 1.3 * r + 1e-12)
+    # far from any other seed, a grain is a perfect rotated lattice
+    second = np.sort(dist, axis=1)[:, 1]
+    deep = interior & (second - dist.min(axis=1) > 6 * r)
+    assert deep.sum() > 100
+    np.testing.assert_allclose(d[deep, 1], 2 * r, rtol=1e-12)
+    pc2 = G.bcc_polycrystal(12, 10, radius=r, seed=3)
+    assert np.array_equal(pc2["xyz"], xyz)

@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import lpmg
+from lpmg import geometry
 from lpmg import lattice as lat
 from lpmg import problems as P
 from oracle import Oracle
@@ -34,7 +35,8 @@ def _jitter(xyz, amp, seed=0):
 LATTICES = [("hex", lat.HEXAGON2D, lambda: lat.hex_plate(11, 9, R)),
             ("square", lat.SQUARE2D, lambda: lat.sc_block(8, 7, 1, R)),
             ("sc", lat.SC3D, lambda: lat.sc_block(7, 6, 5, R)),
-            ("bcc", lat.BCC3D, lambda: lat.bcc_block(4, 4, 5, R))]
+            ("bcc", lat.BCC3D, lambda: lat.bcc_block(4, 4, 5, R)),
+            ("bcc_poly", lat.BCC3D, lambda: geometry.bcc_polycrystal(6, 5, radius=R, seed=11)["xyz"])]
 
 
 def _pair(L, xyz, ptype=lat.ELASTIC, model=0, params=(0, 1, 200e3, 0.3, 1.0)):
@@ -221,8 +223,9 @@ def _run_problem(prob, backend, opts, tol=1e-11):
 
 
 @pytest.mark.parametrize("prob", [P.sc_cube(n=8, steps=3), P.hex_plate(nx=14, ny=12, steps=3),
-                                  P.bcc_j2_bar(nx=3, ny=3, nz=6, steps=10, strain_step=5e-4)],
-                         ids=["sc_cube", "hex_plate", "bcc_j2_bar"])
+                                  P.bcc_j2_bar(nx=3, ny=3, nz=6, steps=10, strain_step=5e-4),
+                                  P.bcc_polycrystal(n_cells=7, n_grains=6, steps=3, strain_step=2e-4)],
+                         ids=["sc_cube", "hex_plate", "bcc_j2_bar", "bcc_polycrystal_cubic"])
 def test_quasi_static_steps_match_oracle(prob):
     g, _ = _run_problem(prob, lpmg.System, GPU_OPTS)
     o, _ = _run_problem(prob, Oracle, ORC_OPTS)
