@@ -146,7 +146,7 @@ def replay(probs, arrivals, opts, slots, fuse):
                                fused_chunks=info["chunks"], graphs=info["graphs"])
 
 
-def serving_bench(n_req):
+def serving_bench(n_req, plain=True):
     """The CT request stream replayed twice on the GPU: the plain FP64 path
     (one request at a time on its own handle, Jacobi-PCG at the reference's
     CG tolerance) and the served mixed path (16 slots, FP32 inner solves, fused
@@ -157,10 +157,12 @@ def serving_bench(n_req):
     probs = [P.ct_plate(int(n), float(a), steps=30, opening=SERVE_OPENING, crit=1e-3, tol=SERVE_TOL)
              for n, a in zip(sizes, aw)]
     print(f"# serving: {n_req} CT requests, particles {[len(p.xyz) for p in probs]}", file=sys.stderr, flush=True)
-    r64, _, plain = replay(probs, arrivals, FP64_OPTS, 1, False)
-    print(f"# serving plain FP64: {plain}", file=sys.stderr, flush=True)
     rmx, res_mx, served = replay(probs, arrivals, SERVED_OPTS, SERVE_SLOTS, SERVE_FUSE_ROWS)
     print(f"# serving mixed: {served}", file=sys.stderr, flush=True)
+    if not plain:
+        return dict(particles=[len(p.xyz) for p in probs], served_mixed=served)
+    r64, _, plain = replay(probs, arrivals, FP64_OPTS, 1, False)
+    print(f"# serving plain FP64: {plain}", file=sys.stderr, flush=True)
     errs, flags_equal = [], 0
     for p, a, b in zip(probs, r64, rmx):
         u64 = a.out["XYZ"].reshape(-1, 3) - p.xyz
@@ -188,6 +190,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--serve-requests", type=int, default=6,
                     help="requests of the CT serving stream replayed (0 skips the serving section)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the served replay of the CT stream (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every solve of this workload runs on a single MI355X "
@@ -196,6 +200,10 @@ def main():
 
     import lpmg
     from lpmg import problems as P
+
+    if args.serve_only:
+        print(json.dumps(serving_bench(max(1, args.serve_requests), plain=False)))
+        return
 
     prob = P.sc_cube(n=args.n, steps=20, total=0.05, E=70e3, nu=0.33)
     prob.steps = prob.steps[:args.load_steps]
@@ -218,9 +226,9 @@ def main():
               flush=True)
     u64 = s.get("XYZ") - s.get("XYZ_INITIAL")
     f64 = s.get("PIN")
-    # one more solve of the same workload with every CG SpMV launch bracketed by
-    # HIP events on the system's stream (the timed solves replay graphs, where
-    # per-kernel events cannot be placed)
+    # one more solve of the same workload with start/stop events attached to
+    # every CG SpMV dispatch on the system's stream (the timed solves replay
+    # graphs, where per-kernel events cannot be placed)
     s.profile(True)
     t_prof, _, _ = solve_once(s, gids, prob, FP64_OPTS, P)
     prof = s.profile_read()
@@ -301,8 +309,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "cg_spmv_kernel<3,double,64>", "avg_ms": spmv_ms,
                      "launches": prof["spmv_launches"], "algorithmic_bytes": spmv_bytes,
-                     "timing": "HIP events around every CG SpMV launch of one extra solve of the workload "
-                               f"({t_prof * 1e3:.0f} ms with events)"},
+                     "timing": "start/stop events attached to every CG SpMV dispatch (hipExtLaunchKernelGGL) "
+                               f"of one extra solve of the workload ({t_prof * 1e3:.0f} ms with events)"},
         "assembly": {"ms": asm_ms, "algorithmic_bytes": asm_bytes, "GBps": asm_bytes / (asm_ms * 1e-3) / 1e9},
         "forward": {"ms": fwd_ms, "algorithmic_bytes": fwd_bytes, "GBps": fwd_bytes / (fwd_ms * 1e-3) / 1e9},
         "stream_copy": {"ms": copy_ms, "bytes": copy_bytes, "GBps": copy_bytes / (copy_ms * 1e-3) / 1e9},

@@ -196,7 +196,7 @@ int lpmg_get_field(lpmg_handle h, int field, double* out);
 int lpmg_set_field(lpmg_handle h, int field, const double* in);
 
 /* Bench instrumentation (no reference counterpart): while enabled, every CG
- * SpMV launch is bracketed by HIP events on the handle's stream.
+ * SpMV dispatch carries start/stop HIP events (hipExtLaunchKernelGGL).
  * lpmg_profile(h, enable) resets the counters; lpmg_profile_read returns
  * out[0] = total SpMV ms, out[1] = number of SpMV launches, out[2] = algorithmic
  * bytes of one FP64 SpMV, out[3] = of one FP32 SpMV. */

@@ -37,11 +37,18 @@
 #include <algorithm>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "bsr.h"
 
 namespace lpmg {
 
 namespace {
+
+// CG vector step: one element per thread, 16 wavefronts per workgroup (the
+// workgroup also sums the previous kernel's partials, so fewer, fuller
+// workgroups cost less than many small ones)
+constexpr int kVBS = 1024;
 
 template <int D, typename T, int LPR>
 __global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
@@ -88,8 +95,8 @@ struct CGStep {
 
 __device__ __forceinline__ CGStep cg_scalars(const double* __restrict__ partV, int nV,
                                              const double* __restrict__ partS, int nS, const Scalars* sc, int i) {
-  Vec<2> gv = sum_partials<kBS, 2>(partV, nV);
-  Vec<1> dv = sum_partials<kBS, 1>(partS, nS);
+  Vec<2> gv = sum_partials<kVBS, 2>(partV, nV);
+  Vec<1> dv = sum_partials<kVBS, 1>(partS, nS);
   CGStep st;
   st.gamma = gv.v[0];
   st.rr = gv.v[1];
@@ -145,7 +152,7 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
   if (st.stop) return;  // every workgroup reaches the same decision
   const T alpha = (T)st.alpha, beta = (T)st.beta;
   double ru = 0, rr = 0;
-  for (int64_t k = (int64_t)blk * kBS + threadIdx.x; k < n; k += (int64_t)nblk * kBS) {
+  for (int64_t k = (int64_t)blk * kVBS + threadIdx.x; k < n; k += (int64_t)nblk * kVBS) {
     const T pk = u[k] + beta * p[k];
     const T sk = w[k] + beta * s[k];
     p[k] = pk;
@@ -161,7 +168,7 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
   Vec<2> a;
   a.v[0] = ru;
   a.v[1] = rr;
-  write_partial_at<kBS, 2>(partV_out, blk, a);
+  write_partial_at<kVBS, 2>(partV_out, blk, a);
 }
 
 // ---- per-system launches
@@ -180,7 +187,7 @@ __global__ void __launch_bounds__(kBS) reduce_partials_kernel(const double* __re
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBS) cg_vec_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
+__global__ void __launch_bounds__(kVBS) cg_vec_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
                                                      T* __restrict__ p, T* __restrict__ s,
                                                      const T* __restrict__ w, const T* __restrict__ dinv,
                                                      int64_t n, const double* __restrict__ partV_in, int nV,
@@ -223,7 +230,7 @@ __global__ void __launch_bounds__(kBS) fused_reduce_partials_kernel(const CgBatc
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kBS) fused_cg_vec_kernel(const CgBatchHdr* __restrict__ hdr,
+__global__ void __launch_bounds__(kVBS) fused_cg_vec_kernel(const CgBatchHdr* __restrict__ hdr,
                                                            const CgSys* __restrict__ tab, int slot) {
   if ((int)blockIdx.x >= hdr->vec_blocks) return;
   const CgSys& q = tab[cg_sys_of<2>(tab, hdr->nsys, blockIdx.x)];
@@ -242,13 +249,13 @@ __global__ void cg_flags_kernel(const CgSys* __restrict__ tab, int nsys, int* __
 
 // Start: x = 0, r = b, u = M b, p = s = 0; partials of (r,u), (r,r)
 template <typename T>
-__global__ void __launch_bounds__(kBS) cg_init_kernel(const double* __restrict__ b, T* __restrict__ x,
+__global__ void __launch_bounds__(kVBS) cg_init_kernel(const double* __restrict__ b, T* __restrict__ x,
                                                       T* __restrict__ r, T* __restrict__ u,
                                                       T* __restrict__ p, T* __restrict__ s,
                                                       const T* __restrict__ dinv, int64_t n,
                                                       double* __restrict__ partV) {
   double ru = 0, rr = 0;
-  for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
+  for (int64_t k = (int64_t)blockIdx.x * kVBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kVBS) {
     const T rk = (T)b[k];
     x[k] = 0;
     p[k] = 0;
@@ -262,7 +269,7 @@ __global__ void __launch_bounds__(kBS) cg_init_kernel(const double* __restrict__
   Vec<2> a;
   a.v[0] = ru;
   a.v[1] = rr;
-  write_partial<kBS, 2>(partV, a);
+  write_partial<kVBS, 2>(partV, a);
 }
 
 __global__ void cg_setup_kernel(Scalars* sc, double rtol, double atol, int64_t maxit) {
@@ -316,14 +323,25 @@ void launch_bsr(Context* c, const T* val, const T* x, T* y) {
     spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
 }
 
+// With events, the kernel's own start and end are recorded with its dispatch
+// (hipExtLaunchKernelGGL): the interval is the kernel, without the gaps that
+// separately enqueued event records add.
 template <int D, typename T>
-void launch_cg_spmv(hipStream_t st, const CgSys& q, int lpr) {
-  if (lpr == 32)
+void launch_cg_spmv(hipStream_t st, const CgSys& q, int lpr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  if (e0) {  // instrumented (eager) launches only; captured chunks use plain launches
+    if (lpr == 32)
+      hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 32>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
+                            (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
+    else
+      hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 64>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
+                            (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
+  } else if (lpr == 32) {
     cg_spmv_kernel<D, T, 32><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
                                                      q.partS_raw, q.sc);
-  else
+  } else {
     cg_spmv_kernel<D, T, 64><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
                                                      q.partS_raw, q.sc);
+  }
 }
 
 // Vectors of the CG in the given precision (FP32: the mixed-precision copies)
@@ -350,7 +368,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
   q->conns = c->conns;
   q->N = c->N;
   q->n = (int64_t)D * c->N;
-  q->gv = grid_for(q->n, kBS * 4);
+  q->gv = grid_for(q->n, kVBS);
   q->gs = spmv_grid(c);
   q->two_level = q->gs > kMaxParts;
   q->nS = q->two_level ? kBS : q->gs;
@@ -364,7 +382,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
                                                               (T*)q->dinv);
   const int64_t maxit = o.maxit > 0 ? o.maxit : q->n;
   cg_setup_kernel<<<1, 64, 0, c->stream>>>(c->sc, o.rtol, o.atol, maxit);
-  cg_init_kernel<T><<<q->gv, kBS, 0, c->stream>>>(c->residual, (T*)q->x, (T*)q->r, (T*)q->u, (T*)q->p, (T*)q->s,
+  cg_init_kernel<T><<<q->gv, kVBS, 0, c->stream>>>(c->residual, (T*)q->x, (T*)q->r, (T*)q->u, (T*)q->p, (T*)q->s,
                                                   (const T*)q->dinv, q->n, q->partV0);
   LPMG_HIP(hipGetLastError());
   return 0;
@@ -379,11 +397,12 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
   c->sc_host_iter_launched = 0;
   auto enqueue_chunk = [&](std::vector<hipEvent_t>* evs) -> int {
     for (int k = 0; k < kCgChunk; ++k) {
-      if (evs) LPMG_HIP(hipEventRecord((*evs)[2 * k], c->stream));
-      launch_cg_spmv<D, T>(c->stream, q, lpr);
-      if (evs) LPMG_HIP(hipEventRecord((*evs)[2 * k + 1], c->stream));
+      if (evs)
+        launch_cg_spmv<D, T>(c->stream, q, lpr, (*evs)[2 * k], (*evs)[2 * k + 1]);
+      else
+        launch_cg_spmv<D, T>(c->stream, q, lpr);
       if (q.two_level) reduce_partials_kernel<<<kBS, kBS, 0, c->stream>>>(q.partS_raw, q.gs, q.partS, q.sc);
-      cg_vec_kernel<T><<<q.gv, kBS, 0, c->stream>>>((T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
+      cg_vec_kernel<T><<<q.gv, kVBS, 0, c->stream>>>((T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
                                                     (const T*)q.dinv, q.n, (k & 1) ? q.partV1 : q.partV0, q.gv,
                                                     q.partS, q.nS, (k & 1) ? q.partV0 : q.partV1, q.sc);
     }
@@ -442,7 +461,7 @@ void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
   const CgSys* tab = reinterpret_cast<const CgSys*>(reinterpret_cast<const char*>(b.hdr) + kCgHdrBytes);
   fused_cg_spmv_kernel<D, T, LPR><<<b.sp_grid, kSpBS, 0, st>>>(b.hdr, tab);
   if (b.red_grid) fused_reduce_partials_kernel<<<b.red_grid, kBS, 0, st>>>(b.hdr, tab);
-  fused_cg_vec_kernel<T><<<b.vec_grid, kBS, 0, st>>>(b.hdr, tab, k & 1);
+  fused_cg_vec_kernel<T><<<b.vec_grid, kVBS, 0, st>>>(b.hdr, tab, k & 1);
 }
 
 }  // namespace
