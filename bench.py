@@ -101,7 +101,7 @@ SERVE_SEED = 1
 SERVE_TOL = 1e-9
 SERVE_FUSE_ROWS = 50000
 SERVE_SLOTS = 16
-SERVE_OPENING = 1e-5   # pin opening per load step, in units of W
+SERVE_OPENING = 7.5e-6  # pin opening per load step, in units of W
 # Served path: FP32 inner solves; the server runs the requests of at most
 # SERVE_FUSE_ROWS particles in fused Jacobi-CG launches and the larger ones with
 # the multigrid CG on their own streams (precond < 0: the server's choice).
@@ -109,12 +109,16 @@ SERVED_OPTS = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1)
 
 
 def ct_stream(n_req, seed=SERVE_SEED):
+    """The first n_req requests of the seeded stream (each request draws its
+    gap, size and a/W in turn, so a shorter replay is a prefix of a longer one)."""
     rng = np.random.default_rng(seed)
-    gaps = rng.exponential(1.0 / SERVE_RATE, n_req)
+    gaps, sizes, aw = [], [], []
+    for _ in range(n_req):
+        gaps.append(rng.exponential(1.0 / SERVE_RATE))
+        sizes.append(int(np.exp(rng.uniform(np.log(5e3), np.log(2.5e5)))))
+        aw.append(rng.uniform(0.2, 0.5))
     arrivals = np.cumsum(gaps) - gaps[0]
-    sizes = np.exp(rng.uniform(np.log(5e3), np.log(2.5e5), n_req)).astype(int)
-    aw = rng.uniform(0.2, 0.5, n_req)
-    return arrivals, sizes, aw
+    return arrivals, np.array(sizes), np.array(aw)
 
 
 def replay(probs, arrivals, opts, slots, fuse):
@@ -132,7 +136,14 @@ def replay(probs, arrivals, opts, slots, fuse):
             if dt > 0:
                 time.sleep(dt)
             tickets.append(srv.submit(r))
-        results = [srv.wait(k) for k in tickets]
+        results = [None] * len(reqs)
+        for k, tk in enumerate(tickets):
+            while results[k] is None:
+                results[k] = srv.wait(tk, timeout=20.0)
+                if results[k] is None:  # report the load steps the running requests have finished
+                    done = [int((q.step_stats[2::3] > 0).sum()) for q in reqs]
+                    print(f"#   {time.perf_counter() - t0:.0f} s: load steps done per request {done}",
+                          file=sys.stderr, flush=True)
         makespan = time.perf_counter() - t0
         info = srv.info()
     bad = [st.get("error") for rc, st in results if rc != 0]
@@ -188,10 +199,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--serve-requests", type=int, default=6,
-                    help="requests of the CT serving stream replayed (0 skips the serving section)")
+    ap.add_argument("--serve-requests", type=int, default=0,
+                    help="replay the first N requests of the CT serving stream through the plain FP64 "
+                         "path and the served mixed path (minutes per large request on the plain path)")
     ap.add_argument("--serve-only", action="store_true",
-                    help="run only the served replay of the CT stream (profiling)")
+                    help="run only the CT stream replay (--serve-requests N, at least 1)")
+    ap.add_argument("--no-serve-plain", action="store_true",
+                    help="with --serve-only: skip the plain FP64 replay (profiling the served path)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every solve of this workload runs on a single MI355X "
@@ -202,7 +216,7 @@ def main():
     from lpmg import problems as P
 
     if args.serve_only:
-        print(json.dumps(serving_bench(max(1, args.serve_requests), plain=False)))
+        print(json.dumps(serving_bench(max(1, args.serve_requests), plain=not args.no_serve_plain)))
         return
 
     prob = P.sc_cube(n=args.n, steps=20, total=0.05, E=70e3, nu=0.33)
