@@ -248,7 +248,7 @@ class Solver {
   std::string dumpFile;
   Assembly<nlayer>& ass;
   Stiffness<nlayer> stiffness;
-  // opts of lpmg_solve_linear: [precond, rtol, atol, maxit, precision, chunk]
+  // opts of lpmg_solve_linear: [precond, rtol, atol, maxit, precision, ...]
   std::array<double, 8> opts{1, 1e-12, 1e-12, 0, 0, 16, 0, 0};
   std::vector<double> last_stats = std::vector<double>(4, 0.0);
 
@@ -261,6 +261,8 @@ class Solver {
     stiffness.precision = on ? 1 : 0;
     if (on) opts[1] = inner_rtol_sq;
   }
+  // Multigrid-preconditioned CG instead of Jacobi (large 2D systems)
+  void setMultigrid(bool on) { opts[0] = on ? 2 : 1; }
   void updateDisplacementBC(LoadStep<nlayer>& ls) {
     for (auto& bc : ls.dispBCs)
       check(lpmg_disp_bc(ass.h, ass.group(bc.group), bc.load_mode == LoadMode::Relative ? LPMG_RELATIVE : LPMG_ABSOLUTE,
@@ -320,6 +322,36 @@ class SolverStatic : public Solver<nlayer> {
     }
     start_index += n_step;
   }
+};
+
+// Many solves at once on one GPU (lpmg_server_*; no reference counterpart).
+// Requests are lpmg_request structs; their output buffers must outlive wait().
+class Server {
+ public:
+  explicit Server(int max_active = 16, int64_t mem_budget = 0, int64_t fuse_max_rows = 50000) {
+    check(lpmg_server_create(max_active, mem_budget, fuse_max_rows, &s_), "lpmg_server_create");
+  }
+  ~Server() { lpmg_server_destroy(s_); }
+  Server(const Server&) = delete;
+  Server& operator=(const Server&) = delete;
+  int64_t submit(const lpmg_request& r) {
+    int64_t t = 0;
+    check(lpmg_server_submit(s_, &r, &t), "lpmg_server_submit");
+    return t;
+  }
+  // false on timeout; throws on a failed request; stats[LPMG_RS_N]
+  bool wait(int64_t ticket, std::array<double, LPMG_RS_N>* stats = nullptr, double timeout_s = -1.0) {
+    std::array<double, LPMG_RS_N> st{};
+    const int rc = lpmg_server_wait(s_, ticket, timeout_s, st.data());
+    if (rc == 1) return false;
+    if (stats) *stats = st;
+    check(rc, "request");
+    return true;
+  }
+  lpmg_server handle() const { return s_; }
+
+ private:
+  lpmg_server s_ = nullptr;
 };
 
 }  // namespace lpm_gpu

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 
 #include "lpm_gpu.hpp"
 
@@ -35,6 +36,37 @@ int main(int argc, char** argv) {
     step.dispBCs.push_back(DispBC<2>(bottom, LoadMode::Relative, 'z', 0.0));
     step.dispBCs.push_back(DispBC<2>(top, LoadMode::Relative, 'z', total / n_steps));
     load.push_back(step);
+  }
+  if (argc > 3 && std::string(argv[3]) == "serve") {
+    // the same solve as a request to the serving runtime, next to a second copy
+    std::vector<double> flat, params{0.0, 0, 70e3, 0.33, 1.0}, out0(3 * xyz.size()), out1(3 * xyz.size());
+    for (auto& q : xyz) flat.insert(flat.end(), q.begin(), q.end());
+    std::vector<int32_t> gsize{(int32_t)bottom.size(), (int32_t)top.size()}, gidx(bottom.begin(), bottom.end());
+    gidx.insert(gidx.end(), top.begin(), top.end());
+    lpmg_bc bcs[4] = {{0, 0, LPMG_RELATIVE, 'x', {0, 0, 0}}, {0, 0, LPMG_RELATIVE, 'y', {0, 0, 0}},
+                      {0, 0, LPMG_RELATIVE, 'z', {0, 0, 0}}, {0, 1, LPMG_RELATIVE, 'z', {total / n_steps, 0, 0}}};
+    lpmg_request r{};
+    r.lattice = LPMG_SC3D; r.radius = radius; r.ptype = LPMG_ELASTIC;
+    r.n = (int)xyz.size(); r.xyz = flat.data();
+    r.model = 0; r.n_params = 5; r.params = params.data();
+    r.n_groups = 2; r.group_size = gsize.data(); r.group_idx = gidx.data();
+    r.n_bc = 4; r.bc = bcs; r.n_steps = n_steps;
+    r.undamaged = -1; r.max_iter = 30; r.tol = 1e-11; r.max_halvings = 8;
+    const double o[8] = {1, 1e-24, 0.0, 0, 0, 16, 0, 0};
+    for (int k = 0; k < 8; ++k) r.opts[k] = o[k];
+    lpmg_out_field f0{LPMG_F_XYZ, (int64_t)out0.size(), out0.data()}, f1{LPMG_F_XYZ, (int64_t)out1.size(), out1.data()};
+    Server srv(2);
+    r.n_out = 1; r.out = &f0;
+    const int64_t t0 = srv.submit(r);
+    r.out = &f1;
+    const int64_t t1 = srv.submit(r);
+    srv.wait(t0);
+    srv.wait(t1);
+    for (size_t k = 0; k < out0.size(); ++k)
+      if (out0[k] != out1[k]) { std::fprintf(stderr, "served copies differ\n"); return 1; }
+    for (size_t p = 0; p < xyz.size(); ++p)
+      std::printf("%.17g %.17g %.17g\n", out0[3 * p] - xyz[p][0], out0[3 * p + 1] - xyz[p][1], out0[3 * p + 2] - xyz[p][2]);
+    return 0;
   }
   ass.updateGeometry();
   ass.updateForceState();
