@@ -32,7 +32,7 @@ typedef struct lpmg_context* lpmg_handle;
 /* LatticeType (include/lpm.h:99-108); the BCC slip variants share one cell. */
 enum { LPMG_SQUARE2D = 0, LPMG_HEXAGON2D = 1, LPMG_SC3D = 2, LPMG_FCC3D = 3, LPMG_BCC3D = 4 };
 /* ParticleType (include/lpm.h:91-97) */
-enum { LPMG_ELASTIC = 0, LPMG_ELASTIC_DAMAGE = 1, LPMG_J2 = 2 };
+enum { LPMG_ELASTIC = 0, LPMG_ELASTIC_DAMAGE = 1, LPMG_J2 = 2, LPMG_FATIGUE_HCF = 3 };
 /* LoadMode (include/lpm.h:67-71) */
 enum { LPMG_RELATIVE = 0, LPMG_ABSOLUTE = 1 };
 /* material models of lpmg_set_material */
@@ -41,7 +41,9 @@ enum {
   LPMG_MAT_ELASTIC_CUBIC = 1,  /* nonlocal_L, C11, C12, C44, critical_bstrain */
   LPMG_MAT_DAMAGE = 2,         /* nonlocal_L, plane_stress, E, mu, k0, k1, ct_ratio, d_thres */
   LPMG_MAT_DAMAGE_CUBIC = 3,   /* nonlocal_L, C11, C12, C44, k0, k1, ct_ratio, d_thres */
-  LPMG_MAT_J2 = 4              /* nonlocal_L, plane_stress, E, mu, sigmay, xi, H, A, crit */
+  LPMG_MAT_J2 = 4,             /* nonlocal_L, plane_stress, E, mu, sigmay, xi, H, A, crit */
+  LPMG_MAT_FATIGUE = 5         /* nonlocal_L, plane_stress, E, mu, A, B, d, d_thres, f_lmt_ratio
+                                  (fatigue particles only; particle_fatigue_hcf.h:163) */
 };
 /* fields of lpmg_get_field / lpmg_set_field (lengths in doubles) */
 enum {
@@ -53,7 +55,9 @@ enum {
   LPMG_F_STRESS = 5,       /* n*6   Particle::stress       */
   LPMG_F_DAMAGE = 6,       /* n     Particle::damage       (set: yes) */
   LPMG_F_DAMAGE_VISUAL = 7,
-  LPMG_F_STATE_VAR = 8,    /* n*7   Particle::state_var    (set: yes) */
+  LPMG_F_STATE_VAR = 8,    /* n*7   Particle::state_var    (set: yes; fatigue particles
+                                    use [0] history max, [1] cycle min, [2] cycle max
+                                    of the strain energy density) */
   LPMG_F_RESIDUAL = 9,     /* n*dim Stiffness::residual    */
   LPMG_F_BDAMAGE = 10,     /* nbond Bond::bdamage          (set: yes) */
   LPMG_F_DLP = 11,         /* nbond Bond::dLp              (set: yes) */
@@ -61,7 +65,7 @@ enum {
   LPMG_F_DIS = 13,
   LPMG_F_CS = 14,          /* nbond*3 Bond::csx/csy/csz    */
   LPMG_F_CONSTRAINT = 15,  /* n*3   Particle::disp_constraint (set: yes) */
-  LPMG_F_DDOT_LOCAL = 16,
+  LPMG_F_DDOT_LOCAL = 16,  /* n     Particle::Ddot_local   (set: yes) */
   LPMG_F_DDOT_NONLOCAL = 17,
   LPMG_F_DLE_TOTAL = 18,   /* n*2   Particle::dLe_total    */
   LPMG_F_CS_SUM = 19,      /* n*6   Particle::cs_sum{x,y,z}[layer] */
@@ -128,7 +132,8 @@ int lpmg_export_csr(lpmg_handle h, int64_t* K_pointer, int64_t* IK, int64_t* JK,
 int lpmg_set_types(lpmg_handle h, const int32_t* type);
 
 /* Particle*::setParticleProperty (particle_elastic.h:70-140,
- * particle_elastic_damage.h:175-250, particle_j2plasticity.h:154-198) for the
+ * particle_elastic_damage.h:175-250, particle_j2plasticity.h:154-198,
+ * particle_fatigue_hcf.h:162-206) for the
  * n_idx particles in idx, or for all particles when idx is NULL. */
 int lpmg_set_material(lpmg_handle h, int n_idx, const int32_t* idx, int model, const double* params);
 
@@ -161,6 +166,12 @@ int lpmg_newton(lpmg_handle h, int max_iter, double tol, const double* opts, dou
 int lpmg_update_state_var(lpmg_handle h);
 /* SolverStatic::updateStaticDamage (solver_static.h:31-62); returns 1 if any new damage */
 int lpmg_update_static_damage(lpmg_handle h, int undamaged_type);
+/* SolverFatigue::updateFatigueDamage (solver_fatigue.h:75-126) with
+ * ParticleFatigueHCF::updateParticleFatigueDamage (particle_fatigue_hcf.h:147-160):
+ * nonlocal averaging of the damage rate, damage += rate * dNdt (1 at the
+ * threshold) and the reset of the cycle's energy extremes.  Fatigue particles
+ * only; returns 1 if any damage changed by more than 1e-3, else 0. */
+int lpmg_update_fatigue_damage(lpmg_handle h, int undamaged_type, double dNdt);
 /* Assembly::storeStateVar / resetStateVar (assembly.h:140-159) */
 int lpmg_store_state(lpmg_handle h);
 int lpmg_reset_state(lpmg_handle h, int reset_xyz);

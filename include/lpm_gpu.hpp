@@ -1,6 +1,7 @@
 // lpm_gpu.hpp -- header-only C++17 adapter that keeps LPM-CPP's class and method
-// names (UnitCell, Assembly, Stiffness, Solver, SolverStatic, LoadStep, DispBC,
-// ForceBC) and forwards every call to the C ABI of liblpmg.so (lpm_gpu.h).
+// names (UnitCell, Assembly, Stiffness, Solver, SolverStatic, SolverFatigue,
+// LoadStep and the fatigue load steps, DispBC, ForceBC) and forwards every call
+// to the C ABI of liblpmg.so (lpm_gpu.h).
 //
 // A caller of the reference switches by including this header instead of
 // assembly.h / stiffness.h / solver_static.h and linking -llpmg.  Differences a
@@ -11,11 +12,15 @@
 //   * errors throw lpm_gpu::Error instead of calling exit();
 //   * SolverMode::CG and SolverMode::PARDISO both run the GPU (P)CG; the
 //     StiffnessMode is always the analytic tangent (the finite-difference mode
-//     of stiffness.h:144-195 is not on the GPU path).
+//     of stiffness.h:144-195 is not on the GPU path);
+//   * the solvers write no dump files (Assembly::writeDump); callers read the
+//     fields they need with Assembly::field().
 #pragma once
 
 #include <array>
+#include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,7 +43,10 @@ inline int check(int rc, const char* where) {
 enum class LoadMode : char { Relative, Absolute };
 enum class SolverMode : char { CG, PARDISO };
 enum class StiffnessMode : char { Analytical, FiniteDifference };
-enum class ParticleType : char { Elastic, ElasticDamage, J2Plasticity };
+enum class ParticleType : char { Elastic, ElasticDamage, J2Plasticity, FatigueHCF };
+enum class TimeMapMode : char { Linear, Exponential };
+// include/load_step_fatigue.h:13-19
+enum class FatigueLoadType : char { LoadUniaxialDisp, LoadUniaxialForce, LoadDogBoneDisp, LoadCTForce };
 enum class LatticeType : char { Square2D, Hexagon2D, SimpleCubic3D, FCC3D, BCC3D_1Slip, BCC3D_2Slip, BCC3D_3Slip };
 
 inline int lattice_id(LatticeType t) {
@@ -102,6 +110,52 @@ class LoadStep {
   void loadCutHalf() {
     for (auto& d : dispBCs) d.cutHalf();
     for (auto& f : forceBCs) f.cutHalf();
+  }
+};
+
+// The fatigue load steps (include/load_step_fatigue.h:21-85)
+template <int nlayer>
+class LoadUniaxialForce : public LoadStep<nlayer> {
+ public:
+  LoadUniaxialForce(double fy, const std::vector<int>& left, const std::vector<int>& top, const std::vector<int>& bottom) {
+    this->dispBCs.push_back(DispBC<nlayer>(left, LoadMode::Relative, 'x', 0.0));
+    this->forceBCs.push_back(ForceBC<nlayer>(top, LoadMode::Relative, 0.0, fy, 0.0));
+    this->dispBCs.push_back(DispBC<nlayer>(bottom, LoadMode::Relative, 'y', 0.0));
+  }
+};
+
+template <int nlayer>
+class LoadUniaxialDisp : public LoadStep<nlayer> {
+ public:
+  LoadUniaxialDisp(double dy, const std::vector<int>& left, const std::vector<int>& top, const std::vector<int>& bottom) {
+    this->dispBCs.push_back(DispBC<nlayer>(left, LoadMode::Relative, 'x', 0.0));
+    this->dispBCs.push_back(DispBC<nlayer>(top, LoadMode::Relative, 'y', dy));
+    this->dispBCs.push_back(DispBC<nlayer>(bottom, LoadMode::Relative, 'y', 0.0));
+  }
+};
+
+template <int nlayer>
+class LoadDogBoneDisp : public LoadStep<nlayer> {
+ public:
+  LoadDogBoneDisp(double x, double y, double z, const std::vector<int>& group1, const std::vector<int>& group2) {
+    this->dispBCs.push_back(DispBC<nlayer>(group1, LoadMode::Relative, 'x', x));
+    this->dispBCs.push_back(DispBC<nlayer>(group1, LoadMode::Relative, 'y', y));
+    this->dispBCs.push_back(DispBC<nlayer>(group1, LoadMode::Relative, 'z', z));
+    this->dispBCs.push_back(DispBC<nlayer>(group2, LoadMode::Relative, 'x', 0.0));
+    this->dispBCs.push_back(DispBC<nlayer>(group2, LoadMode::Relative, 'y', 0.0));
+    this->dispBCs.push_back(DispBC<nlayer>(group2, LoadMode::Relative, 'z', 0.0));
+  }
+};
+
+template <int nlayer>
+class LoadCTForce : public LoadStep<nlayer> {
+ public:
+  LoadCTForce(double fy, const std::vector<int>& mid, const std::vector<int>& top, const std::vector<int>& bottom) {
+    this->dispBCs.push_back(DispBC<nlayer>(mid, LoadMode::Relative, 'y', 0));
+    this->dispBCs.push_back(DispBC<nlayer>(top, LoadMode::Relative, 'x', 0.0));
+    this->dispBCs.push_back(DispBC<nlayer>(bottom, LoadMode::Relative, 'x', 0.0));
+    this->forceBCs.push_back(ForceBC<nlayer>(top, LoadMode::Relative, 0.0, fy, 0.0));
+    this->forceBCs.push_back(ForceBC<nlayer>(bottom, LoadMode::Relative, 0.0, -fy, 0.0));
   }
 };
 
@@ -174,6 +228,12 @@ class Assembly {
              double crit, const std::vector<int>* idx = nullptr) {
     const double p[9] = {nonlocal_L, plane_stress ? 1.0 : 0.0, E, mu, sigmay, xi, H, A, crit};
     material(LPMG_MAT_J2, p, idx);
+  }
+  // ParticleFatigueHCF::setParticleProperty (particle_fatigue_hcf.h:162-206)
+  void setFatigueHCF(double nonlocal_L, bool plane_stress, double E, double mu, double A, double B, double d,
+                     double d_thres, double f_lmt_ratio, const std::vector<int>* idx = nullptr) {
+    const double p[9] = {nonlocal_L, plane_stress ? 1.0 : 0.0, E, mu, A, B, d, d_thres, f_lmt_ratio};
+    material(LPMG_MAT_FATIGUE, p, idx);
   }
   void setTypes(const std::vector<int32_t>& type) { check(lpmg_set_types(h, type.data()), "setTypes"); }
 
@@ -319,6 +379,106 @@ class SolverStatic : public Solver<nlayer> {
         load.insert(load.begin() + i, load[i]);
         ++n_step;
       }
+    }
+    start_index += n_step;
+  }
+};
+
+// SolverFatigue (include/solver_fatigue.h:18-303): cyclic loading of FatigueHCF
+// particles; damage grows once per simulated cycle by the nonlocal damage rate
+// times dN/dt, the real cycles the simulated one stands for.
+template <int nlayer>
+class SolverFatigue : public Solver<nlayer> {
+ public:
+  int undamaged_pt_type{0};
+  TimeMapMode t_mode;
+  double tau{0};
+  std::vector<double> load_spectrum;
+
+  SolverFatigue(const int& p_undamaged, Assembly<nlayer>& p_ass, const StiffnessMode& sm, const SolverMode& so,
+                const TimeMapMode& p_t_mode, const double& p_tau, const std::string& dump, const int& niter,
+                const double& tol)
+      : Solver<nlayer>(p_ass, sm, so, dump, niter, tol), undamaged_pt_type(p_undamaged), t_mode(p_t_mode), tau(p_tau) {}
+
+  // solver_fatigue.h:41-63 (throws instead of exit(1) for a missing file)
+  void readLoad(const std::string& loadFile) {
+    FILE* f = std::fopen(loadFile.c_str(), "r");
+    if (!f) throw Error(-1, "'" + loadFile + "' does not exist");
+    double v;
+    while (std::fscanf(f, "%lf", &v) > 0) load_spectrum.push_back(v);
+    std::fclose(f);
+  }
+
+  // solver_fatigue.h:65-73
+  static int func_dNdt_linear(const double& p_tau) { return (int)(1 / p_tau); }
+  static int func_dNdt_exponental(const double& p_tau, const double& t) { return (int)(1 / p_tau * std::exp(t / p_tau)); }
+
+  bool updateFatigueDamage(double dNdt) {
+    return check(lpmg_update_fatigue_damage(this->ass.h, undamaged_pt_type, dNdt), "updateFatigueDamage") == 1;
+  }
+
+  // solver_fatigue.h:128-180; pt_groups are the load step's particle lists
+  std::vector<LoadStep<nlayer>> generateLoadCycle(FatigueLoadType ltype, int N, int n_interval,
+                                                  const std::vector<std::vector<int>>& g) {
+    const double f_0 = load_spectrum.at(2 * N), f_1 = load_spectrum.at(2 * N + 1), f_2 = load_spectrum.at(2 * N + 2);
+    std::vector<LoadStep<nlayer>> cyc;
+    for (const double inc : {(f_1 - f_0) / n_interval, (f_2 - f_1) / n_interval})
+      for (int i = 0; i < n_interval; ++i) {
+        if (ltype == FatigueLoadType::LoadUniaxialDisp) cyc.push_back(LoadUniaxialDisp<nlayer>(inc, g.at(0), g.at(1), g.at(2)));
+        if (ltype == FatigueLoadType::LoadUniaxialForce) cyc.push_back(LoadUniaxialForce<nlayer>(inc, g.at(0), g.at(1), g.at(2)));
+        if (ltype == FatigueLoadType::LoadDogBoneDisp) cyc.push_back(LoadDogBoneDisp<nlayer>(0, inc, 0, g.at(0), g.at(1)));
+        if (ltype == FatigueLoadType::LoadCTForce) cyc.push_back(LoadCTForce<nlayer>(inc, g.at(0), g.at(1), g.at(2)));
+      }
+    return cyc;
+  }
+
+  // solver_fatigue.h:277-301
+  bool solveProblemStep(LoadStep<nlayer>& load_step) {
+    this->updateForceBC(load_step);
+    this->updateDisplacementBC(load_step);
+    this->stiffness.reset();
+    this->stiffness.calcStiffness();
+    return this->NewtonIteration() < this->max_NR_iter;
+  }
+
+  // solver_fatigue.h:220-241
+  void solveProblemOneCycle(std::vector<LoadStep<nlayer>>& load, double dNdt) {
+    for (auto& step : load) {
+      solveProblemStep(step);
+      this->ass.updateStateVar();
+    }
+    updateFatigueDamage(dNdt);
+    this->ass.storeStateVar();
+    this->ass.updateGeometry();
+    this->ass.updateForceState();
+  }
+
+  // solver_fatigue.h:182-218; returns the real cycle count reached
+  int solveProblemCyclic(FatigueLoadType loadType, int n_interval, const std::vector<std::vector<int>>& pt_groups) {
+    const double n_cycles = 0.5 * ((double)load_spectrum.size() - 1);
+    double t = 0, dNdt = 1;
+    int N = 0;
+    do {
+      auto cyc = generateLoadCycle(loadType, N, n_interval, pt_groups);
+      dNdt = (t_mode == TimeMapMode::Linear) ? func_dNdt_linear(tau) : func_dNdt_exponental(tau, t++);
+      solveProblemOneCycle(cyc, dNdt);
+      N += (int)dNdt;
+    } while (N < n_cycles);
+    return N;
+  }
+
+  // solver_fatigue.h:243-275
+  void solveProblemStatic(std::vector<LoadStep<nlayer>>& load, int& start_index) {
+    int n_step = (int)load.size();
+    for (int i = 0; i < n_step; i++) {
+      while (!solveProblemStep(load[i])) {
+        this->ass.resetStateVar(true);
+        load[i].loadCutHalf();
+        load.insert(load.begin() + i, load[i]);
+        ++n_step;
+      }
+      this->ass.updateStateVar();
+      this->ass.storeStateVar();
     }
     start_index += n_step;
   }

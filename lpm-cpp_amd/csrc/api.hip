@@ -80,7 +80,7 @@ using namespace lpmg;
 struct lpmg_context {
   Context c;
   // host mirrors of the material arrays (small; uploaded after each change)
-  std::vector<double> Kn, Tv, crit, nlL, mE, mMu, mSy, mXi, mH, mA, k0, k1, dthr, ctr;
+  std::vector<double> Kn, Tv, crit, nlL, mE, mMu, mSy, mXi, mH, mA, k0, k1, dthr, ctr, fA, fB, fD;
 };
 
 namespace lpmg {
@@ -146,7 +146,8 @@ int upload_material(lpmg_context* h) {
     std::vector<double>* v;
   } m[] = {{c->Kn, &h->Kn}, {c->Tv, &h->Tv}, {c->crit, &h->crit}, {c->nlL, &h->nlL}, {c->mE, &h->mE},
            {c->mMu, &h->mMu}, {c->mSy, &h->mSy}, {c->mXi, &h->mXi}, {c->mH, &h->mH}, {c->mA, &h->mA},
-           {c->k0, &h->k0}, {c->k1, &h->k1}, {c->dthr, &h->dthr}, {c->ctr, &h->ctr}};
+           {c->k0, &h->k0}, {c->k1, &h->k1}, {c->dthr, &h->dthr}, {c->ctr, &h->ctr},
+           {c->fA, &h->fA}, {c->fB, &h->fB}, {c->fD, &h->fD}};
   for (auto& e : m)
     API_HIP(hipMemcpyAsync(e.d, e.v->data(), sizeof(double) * e.v->size(), hipMemcpyHostToDevice, c->stream));
   API_HIP(hipStreamSynchronize(c->stream));
@@ -168,7 +169,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   rc |= dalloc(c, &c->stress, 6 * N);
   for (double** p : {&c->damage, &c->damage_last, &c->damage_visual, &c->Ddot_local, &c->Ddot_nonlocal,
                      &c->Tv, &c->crit, &c->nlL, &c->mE, &c->mMu, &c->mSy, &c->mXi, &c->mH, &c->mA,
-                     &c->k0, &c->k1, &c->dthr, &c->ctr, &c->diag_sq})
+                     &c->k0, &c->k1, &c->dthr, &c->ctr, &c->fA, &c->fB, &c->fD, &c->diag_sq})
     rc |= dalloc(c, p, N);
   rc |= dalloc(c, &c->sv, 7 * N);
   rc |= dalloc(c, &c->sv_last, 7 * N);
@@ -199,8 +200,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   API_HIP(hipMemsetAsync(c->stress, 0, sizeof(double) * 6 * N, c->stream));
   for (double* p : {c->damage, c->damage_last, c->damage_visual, c->Ddot_local, c->Ddot_nonlocal})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * N, c->stream));
-  API_HIP(hipMemsetAsync(c->sv, 0, sizeof(double) * 7 * N, c->stream));
-  API_HIP(hipMemsetAsync(c->sv_last, 0, sizeof(double) * 7 * N, c->stream));
+  if (int rc = launch_init_state_var(c)) return rc;
   API_HIP(hipMemsetAsync(c->type, 0, sizeof(int) * N, c->stream));
   API_HIP(hipMemsetAsync(c->floating, 0, sizeof(int) * N, c->stream));
   API_HIP(hipMemsetAsync(c->cons, 0, sizeof(int) * 3 * N, c->stream));
@@ -209,7 +209,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   for (float* p : {c->f_r, c->f_p, c->f_q, c->f_z, c->f_w, c->f_x, c->f_dinv})
     API_HIP(hipMemsetAsync(p, 0, sizeof(float) * d * N, c->stream));
   for (auto* v : {&h->Tv, &h->crit, &h->nlL, &h->mE, &h->mMu, &h->mSy, &h->mXi, &h->mH, &h->mA, &h->k0,
-                  &h->k1, &h->dthr, &h->ctr})
+                  &h->k1, &h->dthr, &h->ctr, &h->fA, &h->fB, &h->fD})
     v->assign(N, 0.0);
   h->Kn.assign(2 * N, 0.0);
   // positions
@@ -257,7 +257,7 @@ int validate_create(int lattice, double radius, int ptype, int n, const double* 
   if (!out) { set_error("null output handle"); return -1; }
   *out = nullptr;
   if (!make_cell(lattice, radius, &cell)) { set_error("unknown lattice"); return -1; }
-  if (!(radius > 0) || n <= 0 || !xyz || ptype < 0 || ptype > 2) {
+  if (!(radius > 0) || n <= 0 || !xyz || ptype < 0 || ptype > 3) {
     set_error("bad radius, particle count, coordinates or particle type");
     return -1;
   }
@@ -515,9 +515,14 @@ int lpmg_set_types(lpmg_handle h, const int32_t* type) {
 int lpmg_set_material(lpmg_handle h, int n_idx, const int32_t* idx, int model, const double* p) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (!p || model < 0 || model > 4) { set_error("unknown material model"); return -1; }
+  if (!p || model < 0 || model > 5) { set_error("unknown material model"); return -1; }
+  if ((model == LPMG_MAT_FATIGUE) != (c->ptype == kFatigue)) {
+    set_error("the fatigue material needs fatigue particles and vice versa");
+    return -1;
+  }
   double Ce[3];
-  if (model == LPMG_MAT_ELASTIC || model == LPMG_MAT_DAMAGE || model == LPMG_MAT_J2) {
+  if (model == LPMG_MAT_ELASTIC || model == LPMG_MAT_DAMAGE || model == LPMG_MAT_J2 ||
+      model == LPMG_MAT_FATIGUE) {
     const double E = p[2], mu = p[3];
     if (!(int)p[1]) {
       Ce[0] = E * (1.0 - mu) / (1.0 + mu) / (1.0 - 2.0 * mu);
@@ -555,6 +560,10 @@ int lpmg_set_material(lpmg_handle h, int n_idx, const int32_t* idx, int model, c
     if (model == LPMG_MAT_J2) {
       h->mE[i] = p[2]; h->mMu[i] = p[3]; h->mSy[i] = p[4]; h->mXi[i] = p[5];
       h->mH[i] = p[6]; h->mA[i] = p[7]; h->crit[i] = p[8];
+    }
+    if (model == LPMG_MAT_FATIGUE) {  // particle_fatigue_hcf.h:163-172
+      h->mE[i] = p[2]; h->mMu[i] = p[3]; h->fA[i] = p[4]; h->fB[i] = p[5];
+      h->fD[i] = p[6]; h->dthr[i] = p[7];
     }
   }
   return upload_material(h);
@@ -633,6 +642,13 @@ int lpmg_update_static_damage(lpmg_handle h, int undamaged) {
   CHECK_HANDLE(h);
   int any = 0;
   if (int rc = launch_static_damage(&h->c, undamaged, &any)) return rc;
+  return any ? 1 : 0;
+}
+
+int lpmg_update_fatigue_damage(lpmg_handle h, int undamaged, double dNdt) {
+  CHECK_HANDLE(h);
+  int any = 0;
+  if (int rc = launch_fatigue_damage(&h->c, undamaged, dNdt, &any)) return rc;
   return any ? 1 : 0;
 }
 
@@ -750,7 +766,7 @@ int lpmg_set_field(lpmg_handle h, int f, const double* in) {
   const size_t n = field_len(c, f);
   const bool writable = f == LPMG_F_XYZ || f == LPMG_F_XYZ_LAST || f == LPMG_F_PEX || f == LPMG_F_DAMAGE ||
                         f == LPMG_F_STATE_VAR || f == LPMG_F_BDAMAGE || f == LPMG_F_DLP ||
-                        f == LPMG_F_CONSTRAINT;
+                        f == LPMG_F_CONSTRAINT || f == LPMG_F_DDOT_LOCAL;
   if (!n || !writable) { set_error("field is read-only or unknown"); return -1; }
   double* tmp = nullptr;
   API_HIP(hipMallocAsync((void**)&tmp, sizeof(double) * n, c->stream));
@@ -779,8 +795,7 @@ int lpmg_restart(lpmg_handle h) {
   API_HIP(hipMemsetAsync(c->cons, 0, sizeof(int) * 3 * N, c->stream));
   for (double* p : {c->damage, c->damage_last, c->Ddot_local, c->Ddot_nonlocal})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * N, c->stream));
-  API_HIP(hipMemsetAsync(c->sv, 0, sizeof(double) * 7 * N, c->stream));
-  API_HIP(hipMemsetAsync(c->sv_last, 0, sizeof(double) * 7 * N, c->stream));
+  if (int rc = launch_init_state_var(c)) return rc;
   for (double* p : {c->dLp, c->dLp_last, c->bdamage, c->bdamage_last, c->bforce, c->bforce_last})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * NB, c->stream));
   API_HIP(hipMemcpyAsync(c->dis, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));

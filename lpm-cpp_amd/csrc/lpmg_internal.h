@@ -20,7 +20,7 @@ namespace lpmg {
 constexpr double kEPS = 1e-6;  // include/lpm.h:41
 
 enum Lattice { kSquare2D = 0, kHexagon2D = 1, kSC3D = 2, kFCC3D = 3, kBCC3D = 4 };
-enum PType { kElastic = 0, kElasticDamage = 1, kJ2 = 2 };
+enum PType { kElastic = 0, kElasticDamage = 1, kJ2 = 2, kFatigue = 3 };  // lpm.h:91-97
 
 struct Cell {  // include/unit_cell.h:7-27
   int lattice = 0, dim = 3, nneighbors = 0;
@@ -117,6 +117,7 @@ struct Context {
   double *crit = nullptr, *nlL = nullptr, *mE = nullptr, *mMu = nullptr, *mSy = nullptr,
          *mXi = nullptr, *mH = nullptr, *mA = nullptr, *k0 = nullptr, *k1 = nullptr,
          *dthr = nullptr, *ctr = nullptr;
+  double *fA = nullptr, *fB = nullptr, *fD = nullptr;       // fatigue law A, B, d
   // bonds
   int *bptr = nullptr, *nb0 = nullptr, *p2 = nullptr, *rev = nullptr;
   double *dis0 = nullptr, *dis = nullptr, *dLe = nullptr, *bstrain = nullptr, *dLp = nullptr,
@@ -258,6 +259,8 @@ double spmv_bytes(const Context* c, int precision);
 // ---- state.hip
 int launch_update_state_var(Context* c);
 int launch_static_damage(Context* c, int undamaged, int* any);
+int launch_fatigue_damage(Context* c, int undamaged, double dNdt, int* any);
+int launch_init_state_var(Context* c);
 int update_floating(Context* c);
 int store_state(Context* c);
 int reset_state(Context* c, int reset_xyz);

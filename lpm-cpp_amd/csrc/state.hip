@@ -10,6 +10,9 @@
 // func_phi (lpm.h:49-53); Assembly::storeStateVar / resetStateVar
 // (assembly.h:140-159).  All arithmetic is FP64: these are the threshold tests
 // (yield, bond stretch, damage) whose flags must match the reference exactly.
+// The fatigue particle (particle_fatigue_hcf.h) adds its cycle energy extremes
+// and damage rate here, and SolverFatigue::updateFatigueDamage
+// (solver_fatigue.h:75-126) its once-per-cycle damage update.
 #include "lpmg_internal.h"
 
 namespace lpmg {
@@ -55,6 +58,25 @@ j2_update_kernel(const int* __restrict__ bptr, const double* __restrict__ cs,
   Ddot_local[i] = dlambda * (1.0 + mA[i] * triax);
 }
 
+// ParticleElasticDamage::calcStrainEnergyDensity (particle_elastic_damage.h:89-104);
+// ParticleFatigueHCF::calcEqEnergy returns the same sum in the same order
+// (energy_total, particle_fatigue_hcf.h:96-106, 123).
+__device__ __forceinline__ double particle_energy(const int* __restrict__ bptr, const int* __restrict__ nb0,
+                                                  const double* __restrict__ dLe, const double* __restrict__ Kn,
+                                                  const double* __restrict__ dLe_tot,
+                                                  const double* __restrict__ TdLe_tot, double volume,
+                                                  double nneighbors, int i) {
+  const int lo = bptr[i], mid = lo + nb0[i], hi = bptr[i + 1];
+  const double V_m = volume * (hi - lo) / nneighbors;
+  double e = 0;
+  for (int L = 0; L < 2; ++L) {
+    const double kn = Kn[2 * i + L];
+    for (int b = (L == 0 ? lo : mid); b < (L == 0 ? mid : hi); ++b) e += 0.5 * (kn * dLe[b] * dLe[b]) / V_m;
+    e += 0.5 * TdLe_tot[2 * i + L] * dLe_tot[2 * i + L] / V_m;
+  }
+  return e;
+}
+
 __global__ void __launch_bounds__(256)
 damage_energy_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
                      const double* __restrict__ dLe, const double* __restrict__ Kn,
@@ -65,18 +87,36 @@ damage_energy_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
                      double* __restrict__ Ddot_local) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
-  const int lo = bptr[i], mid = lo + nb0[i], hi = bptr[i + 1];
-  const double V_m = volume * (hi - lo) / nneighbors;
-  double e = 0;
-  for (int L = 0; L < 2; ++L) {
-    const double kn = Kn[2 * i + L];
-    for (int b = (L == 0 ? lo : mid); b < (L == 0 ? mid : hi); ++b) e += 0.5 * (kn * dLe[b] * dLe[b]) / V_m;
-    e += 0.5 * TdLe_tot[2 * i + L] * dLe_tot[2 * i + L] / V_m;
-  }
+  const double e = particle_energy(bptr, nb0, dLe, Kn, dLe_tot, TdLe_tot, volume, nneighbors, i);
   sv[7 * (size_t)i] = e;
   const double delta = e - sv_last[7 * (size_t)i];
   const double f = e - k1[i] * damage[i] - k0[i];
   Ddot_local[i] = (f > 0 && delta > 0) ? (1 / k1[i]) * delta : 0;
+}
+
+// ParticleFatigueHCF::updateParticleStateVariables (particle_fatigue_hcf.h:127-145):
+// sv[1] / sv[2] are the least / greatest energy of the current cycle, sv[0] the
+// greatest over the whole history, and the local damage rate is
+// A * sv[0]^B * (sv[2] - sv[1])^d while the cycle has a range.  The min/max
+// keep std::min/std::max's operand order, so a NaN energy propagates the same way.
+__global__ void __launch_bounds__(256)
+fatigue_state_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
+                     const double* __restrict__ dLe, const double* __restrict__ Kn,
+                     const double* __restrict__ dLe_tot, const double* __restrict__ TdLe_tot,
+                     const double* __restrict__ fA, const double* __restrict__ fB,
+                     const double* __restrict__ fD, double volume, double nneighbors, int N,
+                     double* __restrict__ sv, double* __restrict__ Ddot_local) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double e = particle_energy(bptr, nb0, dLe, Kn, dLe_tot, TdLe_tot, volume, nneighbors, i);
+  double* s = sv + 7 * (size_t)i;
+  const double lo = (s[1] < e) ? s[1] : e;
+  const double hi = (e < s[2]) ? s[2] : e;
+  const double his = (hi < s[0]) ? s[0] : hi;
+  s[0] = his;
+  s[1] = lo;
+  s[2] = hi;
+  Ddot_local[i] = (hi - lo > 0) ? fA[i] * pow(his, fB[i]) * pow(hi - lo, fD[i]) : 0;
 }
 
 __device__ __forceinline__ double func_phi(double x, double L) {
@@ -118,20 +158,54 @@ particle_damage_kernel(const int* __restrict__ type, const int* __restrict__ bpt
                        double* __restrict__ bdamage, double* __restrict__ damage, int* __restrict__ any) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N || type[i] == undamaged) return;
-  bool mine = false;
+  int flags = 0;  // 1: noticeable new damage (the return value), 2: a bond may have broken
   if (ptype == kElastic) {
     for (int b = bptr[i]; b < bptr[i + 1]; ++b)
       if (bstrain[b] >= crit[i] && fabs(bdamage[b] - 1.0) > kEPS) {
-        mine = true;
+        flags = 3;
         bdamage[b] = 1;
       }
-  } else {
-    double d = damage[i] + Ddot_nonlocal[i];
+  } else if (ptype != kFatigue) {  // the fatigue particle keeps Particle's no-op (particle.h:63)
+    const double d0 = damage[i];
+    double d = d0 + Ddot_nonlocal[i];
     if (d >= dthr[i]) d = 1;
     damage[i] = d;
-    mine = fabs(d - damage_last[i]) > 1e-3;
+    if (fabs(d - damage_last[i]) > 1e-3) flags |= 1;
+    if (fabs(d - 1.0) < kEPS && !(fabs(d0 - 1.0) < kEPS)) flags |= 2;
   }
-  if (mine) atomicOr(any, 1);
+  if (flags) atomicOr(any, flags);
+}
+
+// ParticleFatigueHCF::updateParticleFatigueDamage (particle_fatigue_hcf.h:147-160)
+// for every particle that is not of the undamaged type (solver_fatigue.h:118-123).
+__global__ void __launch_bounds__(256)
+fatigue_damage_kernel(const int* __restrict__ type, const double* __restrict__ Ddot_nonlocal,
+                      const double* __restrict__ dthr, const double* __restrict__ damage_last,
+                      double dNdt, int undamaged, int N, double* __restrict__ damage,
+                      double* __restrict__ sv, int* __restrict__ any) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N || type[i] == undamaged) return;
+  const double d0 = damage[i];
+  double d = d0 + Ddot_nonlocal[i] * dNdt;
+  if (d >= dthr[i]) d = 1;
+  damage[i] = d;
+  sv[7 * (size_t)i + 1] = 2147483647.0;  // INT_MAX
+  sv[7 * (size_t)i + 2] = 0;
+  int flags = 0;
+  if (fabs(d - damage_last[i]) > 1e-3) flags |= 1;
+  if (fabs(d - 1.0) < kEPS && !(fabs(d0 - 1.0) < kEPS)) flags |= 2;
+  if (flags) atomicOr(any, flags);
+}
+
+// Particle state variables at construction: zeros, except the fatigue
+// particle's cycle minimum, which starts at INT_MAX (particle_fatigue_hcf.h:25-35).
+__global__ void init_state_var_kernel(double* __restrict__ sv, double* __restrict__ sv_last, int fatigue,
+                                      int N) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 7 * N) return;
+  const double v = (fatigue && t % 7 == 1) ? 2147483647.0 : 0.0;
+  sv[t] = v;
+  sv_last[t] = v;
 }
 
 __global__ void copy_xyz_kernel(double4* __restrict__ dst, const double4* __restrict__ src, int N) {
@@ -151,7 +225,23 @@ int launch_update_state_var(Context* c) {
                                                    c->TdLe_tot, c->damage, c->k0, c->k1, c->cell.volume,
                                                    (double)c->cell.nneighbors, N, c->sv, c->sv_last,
                                                    c->Ddot_local);
+  else if (c->ptype == kFatigue)
+    fatigue_state_kernel<<<nb, tb, 0, c->stream>>>(c->bptr, c->nb0, c->dLe, c->Kn, c->dLe_tot, c->TdLe_tot,
+                                                   c->fA, c->fB, c->fD, c->cell.volume,
+                                                   (double)c->cell.nneighbors, N, c->sv, c->Ddot_local);
   LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+// The damage kernels OR bit 1 (noticeable new damage) and bit 2 (a particle
+// reached full damage or a bond broke, so the fragment labels are stale) into err_flag.
+static int read_damage_flags(Context* c, int* any) {
+  int h = 0;
+  LPMG_HIP(hipMemcpyAsync(&h, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  LPMG_HIP(hipStreamSynchronize(c->stream));
+  LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
+  *any = h & 1;
+  if (h & 2) c->floating_dirty = true;
   return 0;
 }
 
@@ -166,12 +256,28 @@ int launch_static_damage(Context* c, int undamaged, int* any) {
                                                    c->Ddot_nonlocal, c->dthr, c->damage_last, c->ptype,
                                                    undamaged, N, c->bdamage, c->damage, c->err_flag);
   LPMG_HIP(hipGetLastError());
-  int h = 0;
-  LPMG_HIP(hipMemcpyAsync(&h, c->err_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  LPMG_HIP(hipStreamSynchronize(c->stream));
+  return read_damage_flags(c, any);
+}
+
+int launch_fatigue_damage(Context* c, int undamaged, double dNdt, int* any) {
+  const int N = c->N, tb = 256, nb = (N + tb - 1) / tb;
   LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
-  *any = h;
-  if (h) c->floating_dirty = true;
+  nonlocal_average_kernel<<<nb, tb, 0, c->stream>>>(c->xyz, c->type, c->bptr, c->nlptr, c->nlidx,
+                                                    c->nlL, c->Ddot_local, c->cell.volume,
+                                                    (double)c->cell.nneighbors, undamaged, N,
+                                                    c->Ddot_nonlocal);
+  if (c->ptype == kFatigue)  // other particles keep Particle's no-op (particle.h:64)
+    fatigue_damage_kernel<<<nb, tb, 0, c->stream>>>(c->type, c->Ddot_nonlocal, c->dthr, c->damage_last,
+                                                    dNdt, undamaged, N, c->damage, c->sv, c->err_flag);
+  LPMG_HIP(hipGetLastError());
+  return read_damage_flags(c, any);
+}
+
+int launch_init_state_var(Context* c) {
+  const int n = 7 * c->N, tb = 256;
+  init_state_var_kernel<<<(n + tb - 1) / tb, tb, 0, c->stream>>>(c->sv, c->sv_last, c->ptype == kFatigue,
+                                                                  c->N);
+  LPMG_HIP(hipGetLastError());
   return 0;
 }
 

@@ -29,7 +29,7 @@ EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "
            "lpmg_destroy", "lpmg_sizes", "lpmg_export_bonds", "lpmg_export_conns", "lpmg_export_csr",
            "lpmg_set_types", "lpmg_set_material", "lpmg_forward", "lpmg_update_rr", "lpmg_assemble",
            "lpmg_spmv", "lpmg_solve_linear", "lpmg_newton", "lpmg_update_state_var",
-           "lpmg_update_static_damage", "lpmg_store_state", "lpmg_reset_state", "lpmg_search_nonlocal",
+           "lpmg_update_static_damage", "lpmg_update_fatigue_damage", "lpmg_store_state", "lpmg_reset_state", "lpmg_search_nonlocal",
            "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_restart", "lpmg_field_len",
            "lpmg_get_field", "lpmg_set_field", "lpmg_profile", "lpmg_profile_read", "lpmg_time_kernel",
            "lpmg_synchronize", "lpmg_request_bytes", "lpmg_server_create", "lpmg_server_submit",
@@ -82,6 +82,7 @@ def lib():
         L.lpmg_solve_linear.argtypes = [vp, dp, dp]
         L.lpmg_newton.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, dp]
         L.lpmg_update_static_damage.argtypes = [vp, ctypes.c_int]
+        L.lpmg_update_fatigue_damage.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.lpmg_reset_state.argtypes = [vp, ctypes.c_int]
         L.lpmg_search_nonlocal.argtypes = [vp, ctypes.c_double]
         L.lpmg_define_group.argtypes = [vp, ctypes.c_int, i32p]
@@ -248,6 +249,9 @@ class System:
 
     def update_static_damage(self, undamaged=-1):
         return bool(self._chk(lib().lpmg_update_static_damage(self.h, int(undamaged))))
+
+    def update_fatigue_damage(self, undamaged=-1, dNdt=1.0):
+        return bool(self._chk(lib().lpmg_update_fatigue_damage(self.h, int(undamaged), float(dNdt))))
 
     def store_state(self):
         self._chk(lib().lpmg_store_state(self.h))

@@ -14,8 +14,8 @@ import numpy as np
 
 # LatticeType (include/lpm.h:99-108); the three BCC slip variants share one unit cell
 SQUARE2D, HEXAGON2D, SC3D, FCC3D, BCC3D = 0, 1, 2, 3, 4
-# ParticleType (include/lpm.h:91-97); FatigueHCF is not on the quasi-static path
-ELASTIC, ELASTIC_DAMAGE, J2PLASTICITY = 0, 1, 2
+# ParticleType (include/lpm.h:91-97)
+ELASTIC, ELASTIC_DAMAGE, J2PLASTICITY, FATIGUE_HCF = 0, 1, 2, 3
 
 
 def create_plate_hex2d(box, radius):

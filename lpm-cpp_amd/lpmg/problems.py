@@ -23,7 +23,7 @@ import numpy as np
 
 from . import lattice as lat
 
-MAT_ELASTIC, MAT_ELASTIC_CUBIC, MAT_DAMAGE, MAT_DAMAGE_CUBIC, MAT_J2 = 0, 1, 2, 3, 4
+MAT_ELASTIC, MAT_ELASTIC_CUBIC, MAT_DAMAGE, MAT_DAMAGE_CUBIC, MAT_J2, MAT_FATIGUE = 0, 1, 2, 3, 4, 5
 RELATIVE, ABSOLUTE = 0, 1
 
 
@@ -213,4 +213,51 @@ def bcc_polycrystal(n_cells=80, n_grains=200, radius=0.1, steps=10, strain_step=
                 note=f"BCC polycrystal {n_cells}^3 cells, {n_grains} grains (seed {seed}), N={len(xyz)}, "
                      f"cubic C11/C12/C44 = {C11}/{C12}/{C44} MPa, tension {strain_step}/step x {steps}")
     p.extra["grain"] = pc["grain"]
+    return p
+
+
+def fatigue_gage(radius=0.12, d_max=0.043927, R=0.0, n_cycles=1_000_000, nonlocal_L=8.0, cutoff_ratio=1.5,
+                 tau=1e-4, hole=0.0, E=71.7e3, nu=0.306, A=2.003e-5, B=2.833, d=0.5, d_thres=1.0,
+                 f_lmt_ratio=1.108, max_iter=30, tol=1e-5):
+    """2D hexagonal gage section of FatigueHCF particles under cyclic uniaxial
+    displacement, as examples/ex5_isoelasticity_2d_fatigue_gage.cpp: the
+    5.08 x 14.7 mm plate (r = 0.12), Al 7075-T651 plane stress (E = 71.7 GPa,
+    nu = 0.306) with the fatigue law A = 2.003e-5, B = 2.833, d = 0.5; the
+    left edge (x < 0.05) is held in x, the grip particles (y > 13.7, y < 1) are
+    of the undamaged type 4, the bottom grip is held in y and the top grip
+    moves through the spectrum low, high, low, ... (R * d_max, d_max) one step
+    per half cycle.  `hole` > 0 cuts a central circular hole of that radius,
+    which concentrates the strain energy so that damage localises.
+
+    Problem.steps holds the static step to the spectrum's first value
+    (solveProblemStatic); extra["fatigue"] holds what solve_cyclic needs."""
+    from . import fatigue as F
+    box = [0.0, 5.08, 0.0, 12.7 + 2, 0.0, 4.8]
+    xyz = lat.create_plate_hex2d(box, radius)
+    if hole > 0:
+        keep = np.hypot(xyz[:, 0] - 2.54, xyz[:, 1] - 7.35) >= hole
+        xyz = xyz[keep]
+    x, y = xyz[:, 0], xyz[:, 1]
+    types = np.zeros(len(xyz), dtype=np.int32)
+    left = np.nonzero(x < 0.05)[0]
+    types[left] = 1
+    top = np.nonzero(y > 14.7 / 2 + 12.7 / 2)[0]
+    types[top] = 2
+    bottom = np.nonzero(y < 14.7 / 2 - 12.7 / 2)[0]
+    types[bottom] = 3
+    undamaged = 4
+    types[(types == 2) | (types == 3)] = undamaged
+    d_min = R * d_max
+    static = [[("disp", "left", RELATIVE, "x", 0.0), ("disp", "top", RELATIVE, "y", d_min),
+               ("disp", "bottom", RELATIVE, "y", 0.0)]]
+    p = Problem("fatigue_gage", lat.HEXAGON2D, radius, lat.FATIGUE_HCF, xyz,
+                (MAT_FATIGUE, [nonlocal_L, 1, E, nu, A, B, d, d_thres, f_lmt_ratio]),
+                dict(left=left, top=top, bottom=bottom), static, types=types, undamaged=undamaged,
+                max_iter=max_iter, tol=tol,
+                note=f"hex gage 5.08x14.7 r={radius} N={len(xyz)}" + (f" hole r={hole}" if hole else "") +
+                     f", fatigue A={A} B={B} d={d}, d_max={d_max} R={R}, L={nonlocal_L}, tau={tau}")
+    p.extra["nonlocal_ratio"] = cutoff_ratio
+    p.extra["fatigue"] = dict(ltype=F.UNIAXIAL_DISP, groups=("left", "top", "bottom"),
+                              spectrum=F.constant_amplitude(d_min, d_max, n_cycles), n_interval=1,
+                              tau=tau, t_mode=F.LINEAR)
     return p

@@ -32,6 +32,7 @@
 // CG iteration counts are parity unpinned (MKL's internals are closed).
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -45,7 +46,7 @@ namespace {
 constexpr double kEPS = 1e-6;  // include/lpm.h:41
 
 enum Lattice { kSquare2D = 0, kHexagon2D = 1, kSC3D = 2, kFCC3D = 3, kBCC3D = 4 };
-enum PType { kElastic = 0, kElasticDamage = 1, kJ2 = 2 };
+enum PType { kElastic = 0, kElasticDamage = 1, kJ2 = 2, kFatigue = 3 };  // lpm.h:91-97
 
 thread_local std::string g_err;
 
@@ -126,6 +127,7 @@ struct Ctx {
   // material (Particle*::setParticleProperty)
   std::vector<double> Kn, Tv;                // Kn[i*2+L], Tv[i]
   std::vector<double> crit, nlL, mE, mMu, mSy, mXi, mH, mA, k0, k1, dthr, ctr;
+  std::vector<double> fA, fB, fD;            // fatigue law A, B, d
   // bonds (per particle: layer-0 bonds then layer-1 bonds)
   std::vector<int> bptr, nb0, p2, rev;
   std::vector<double> dis0, dis, dL, dLe, bstrain, dLp, dLp_last, cs, bforce,
@@ -163,10 +165,12 @@ void alloc_particles(Ctx* c) {
   c->damage.assign(N, 0); c->damage_last.assign(N, 0); c->damage_visual.assign(N, 0);
   c->Ddot_local.assign(N, 0); c->Ddot_nonlocal.assign(N, 0);
   c->sv.assign(7 * N, 0); c->sv_last.assign(7 * N, 0);
+  if (c->ptype == kFatigue)  // state_var = {0, INT_MAX, 0} (particle_fatigue_hcf.h:25-35)
+    for (int i = 0; i < N; ++i) c->sv[7 * i + 1] = c->sv_last[7 * i + 1] = INT_MAX;
   c->dLe_tot.assign(2 * N, 0); c->TdLe_tot.assign(2 * N, 0); c->cs_sum.assign(6 * N, 0);
   c->Kn.assign(2 * N, 0); c->Tv.assign(N, 0);
   for (auto* v : {&c->crit, &c->nlL, &c->mE, &c->mMu, &c->mSy, &c->mXi, &c->mH, &c->mA,
-                  &c->k0, &c->k1, &c->dthr, &c->ctr})
+                  &c->k0, &c->k1, &c->dthr, &c->ctr, &c->fA, &c->fB, &c->fD})
     v->assign(N, 0);
   c->nlptr.assign(N + 1, 0);
 }
@@ -771,8 +775,8 @@ double strain_energy(const Ctx* c, int i) {
 
 // Assembly::updateStateVar (assembly.h:161-166) -> updateParticleStateVariables
 // of ParticleElasticDamage (particle_elastic_damage.h:150-161) and
-// ParticleJ2Plasticity (particle_j2plasticity.h:78-140); the elastic particle
-// has none (particle.h:62).
+// ParticleJ2Plasticity (particle_j2plasticity.h:78-140) and ParticleFatigueHCF
+// (below); the elastic particle has none (particle.h:62).
 void update_state_var(Ctx* c) {
   const int N = c->N;
   if (c->ptype == kElasticDamage) {
@@ -781,6 +785,21 @@ void update_state_var(Ctx* c) {
       const double delta = c->sv[7 * i] - c->sv_last[7 * i];
       const double f = c->sv[7 * i] - c->k1[i] * c->damage[i] - c->k0[i];
       c->Ddot_local[i] = (f > 0 && delta > 0) ? (1 / c->k1[i]) * delta : 0;
+    }
+  } else if (c->ptype == kFatigue) {
+    // ParticleFatigueHCF::updateParticleStateVariables (particle_fatigue_hcf.h:127-145);
+    // calcEqEnergy returns energy_total, the sum strain_energy computes
+    // (particle_fatigue_hcf.h:96-106, 123).
+    for (int i = 0; i < N; ++i) {
+      double* sv = &c->sv[7 * i];
+      const double e = strain_energy(c, i);
+      sv[1] = std::min(e, sv[1]);
+      sv[2] = std::max(e, sv[2]);
+      sv[0] = std::max(sv[2], sv[0]);
+      if (sv[2] - sv[1] > 0)
+        c->Ddot_local[i] = c->fA[i] * std::pow(sv[0], c->fB[i]) * std::pow(sv[2] - sv[1], c->fD[i]);
+      else
+        c->Ddot_local[i] = 0;
     }
   } else if (c->ptype == kJ2) {
     for (int i = 0; i < N; ++i) {
@@ -822,11 +841,11 @@ inline double func_phi(double x, double L) {
   return t * t;
 }
 
-// SolverStatic::updateStaticDamage (solver_static.h:31-62) with
-// updateParticleStaticDamage of ParticleElastic (particle_elastic.h:52-68),
-// ParticleElasticDamage (particle_elastic_damage.h:163-173) and
-// ParticleJ2Plasticity (particle_j2plasticity.h:142-152).
-bool update_static_damage(Ctx* c, int undamaged) {
+// The nonlocal damage-rate average shared by SolverStatic::updateStaticDamage
+// (solver_static.h:33-53) and SolverFatigue::updateFatigueDamage
+// (solver_fatigue.h:78-116): phi-weighted over the nonlocal neighbours, both
+// ends skipped when of the undamaged type.
+void nonlocal_average(Ctx* c, int undamaged) {
   const int N = c->N;
   for (int i = 0; i < N; ++i) {
     if (c->type[i] == undamaged) continue;
@@ -841,6 +860,14 @@ bool update_static_damage(Ctx* c, int undamaged) {
     }
     c->Ddot_nonlocal[i] = Dn / A;
   }
+}
+// SolverStatic::updateStaticDamage (solver_static.h:31-62) with
+// updateParticleStaticDamage of ParticleElastic (particle_elastic.h:52-68),
+// ParticleElasticDamage (particle_elastic_damage.h:163-173) and
+// ParticleJ2Plasticity (particle_j2plasticity.h:142-152).
+bool update_static_damage(Ctx* c, int undamaged) {
+  const int N = c->N;
+  nonlocal_average(c, undamaged);
   bool any = false;
   for (int i = 0; i < N; ++i) {
     if (c->type[i] == undamaged) continue;
@@ -851,12 +878,31 @@ bool update_static_damage(Ctx* c, int undamaged) {
           mine = true;
           c->bdamage[b] = 1;
         }
-    } else {
+    } else if (c->ptype != kFatigue) {  // the fatigue particle keeps the no-op (particle.h:63)
       c->damage[i] += c->Ddot_nonlocal[i];
       if (c->damage[i] >= c->dthr[i]) c->damage[i] = 1;
       mine = std::fabs(c->damage[i] - c->damage_last[i]) > 1e-3;
     }
     any = mine || any;
+  }
+  return any;
+}
+
+// SolverFatigue::updateFatigueDamage (solver_fatigue.h:75-126): the nonlocal
+// averaging above, then ParticleFatigueHCF::updateParticleFatigueDamage
+// (particle_fatigue_hcf.h:147-160) on every particle not of the undamaged type;
+// other particle types keep Particle's no-op (particle.h:64).
+bool update_fatigue_damage(Ctx* c, int undamaged, double dt) {
+  nonlocal_average(c, undamaged);
+  if (c->ptype != kFatigue) return false;
+  bool any = false;
+  for (int i = 0; i < c->N; ++i) {
+    if (c->type[i] == undamaged) continue;
+    c->damage[i] += c->Ddot_nonlocal[i] * dt;
+    if (c->damage[i] >= c->dthr[i]) c->damage[i] = 1;
+    c->sv[7 * i + 1] = INT_MAX;
+    c->sv[7 * i + 2] = 0;
+    any = std::fabs(c->damage[i] - c->damage_last[i]) > 1e-3 || any;
   }
   return any;
 }
@@ -961,7 +1007,7 @@ int lpmo_unit_cell(int lattice, double radius, double* out /*16*/) {
 // Assembly(p_xyz, box, cell, ptype) (assembly.h:97-106) + Stiffness::initialize
 void* lpmo_create(int lattice, double radius, int ptype, int n, const double* xyz, int brute) {
   Ctx* c = new Ctx;
-  if (!make_cell(lattice, radius, &c->cell) || n <= 0 || ptype < 0 || ptype > 2) {
+  if (!make_cell(lattice, radius, &c->cell) || n <= 0 || ptype < 0 || ptype > 3) {
     g_err = "bad lattice, particle count or particle type";
     delete c;
     return nullptr;
@@ -979,7 +1025,7 @@ void* lpmo_create_with_bonds(int lattice, double radius, int ptype, int n, const
                              int64_t nbond, const int32_t* p1, const int32_t* p2,
                              const int32_t* layer) {
   Ctx* c = new Ctx;
-  if (!make_cell(lattice, radius, &c->cell) || n <= 0 || ptype < 0 || ptype > 2) {
+  if (!make_cell(lattice, radius, &c->cell) || n <= 0 || ptype < 0 || ptype > 3) {
     g_err = "bad lattice, particle count or particle type";
     delete c;
     return nullptr;
@@ -1044,13 +1090,18 @@ int lpmo_set_types(void* h, const int32_t* type) {
 //  2: nonlocal_L, plane_stress, E, mu, k0, k1, ct_ratio, d_thres
 //  3: nonlocal_L, C11, C12, C44, k0, k1, ct_ratio, d_thres
 //  4: nonlocal_L, plane_stress, E, mu, sigmay, xi, H, A, crit
+//  5: nonlocal_L, plane_stress, E, mu, A, B, d, d_thres, f_lmt_ratio (fatigue particles)
 int lpmo_set_material(void* h, int n_idx, const int32_t* idx, int model, const double* p) {
   Ctx* c = static_cast<Ctx*>(h);
   double Ce[3], KnTv[3];
   switch (model) {
-    case 0: case 2: case 4: ce_isotropic((int)p[1], p[2], p[3], Ce); break;
+    case 0: case 2: case 4: case 5: ce_isotropic((int)p[1], p[2], p[3], Ce); break;
     case 1: case 3: Ce[0] = p[1]; Ce[1] = p[2]; Ce[2] = p[3]; break;
     default: g_err = "unknown material model"; return -1;
+  }
+  if ((model == 5) != (c->ptype == kFatigue)) {
+    g_err = "the fatigue material needs fatigue particles and vice versa";
+    return -1;
   }
   kntv_from_ce(c->cell, Ce, KnTv);
   const int n = idx ? n_idx : c->N;
@@ -1067,6 +1118,10 @@ int lpmo_set_material(void* h, int n_idx, const int32_t* idx, int model, const d
     if (model == 4) {
       c->mE[i] = p[2]; c->mMu[i] = p[3]; c->mSy[i] = p[4]; c->mXi[i] = p[5];
       c->mH[i] = p[6]; c->mA[i] = p[7]; c->crit[i] = p[8];
+    }
+    if (model == 5) {  // ParticleFatigueHCF::setParticleProperty (particle_fatigue_hcf.h:162-172)
+      c->mE[i] = p[2]; c->mMu[i] = p[3]; c->fA[i] = p[4]; c->fB[i] = p[5]; c->fD[i] = p[6];
+      c->dthr[i] = p[7];
     }
   }
   return 0;
@@ -1139,6 +1194,9 @@ int lpmo_newton(void* h, int max_iter, double tol, const double* opts, double* s
 int lpmo_update_state_var(void* h) { update_state_var(static_cast<Ctx*>(h)); return 0; }
 int lpmo_update_static_damage(void* h, int undamaged) {
   return update_static_damage(static_cast<Ctx*>(h), undamaged) ? 1 : 0;
+}
+int lpmo_update_fatigue_damage(void* h, int undamaged, double dNdt) {
+  return update_fatigue_damage(static_cast<Ctx*>(h), undamaged, dNdt) ? 1 : 0;
 }
 int lpmo_store_state(void* h) { store_state(static_cast<Ctx*>(h)); return 0; }
 int lpmo_reset_state(void* h, int reset_xyz) { reset_state(static_cast<Ctx*>(h), reset_xyz); return 0; }
@@ -1284,6 +1342,7 @@ int lpmo_set_field(void* h, int f, const double* in) {
     case F_STATE_VAR: v = &c->sv; break;
     case F_BDAMAGE: v = &c->bdamage; break;
     case F_DLP: v = &c->dLp; break;
+    case F_DDOT_LOCAL: v = &c->Ddot_local; break;
     case F_CONSTRAINT:
       for (size_t t = 0; t < c->cons.size(); ++t) c->cons[t] = (int)in[t];
       return 0;

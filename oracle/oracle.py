@@ -46,7 +46,8 @@ def lib():
                      "lpmo_set_types", "lpmo_set_material", "lpmo_update_geometry",
                      "lpmo_update_force_state", "lpmo_forward", "lpmo_update_rr", "lpmo_assemble",
                      "lpmo_spmv", "lpmo_solve_linear", "lpmo_newton", "lpmo_update_state_var",
-                     "lpmo_update_static_damage", "lpmo_store_state", "lpmo_reset_state",
+                     "lpmo_update_static_damage", "lpmo_update_fatigue_damage", "lpmo_store_state",
+                     "lpmo_reset_state",
                      "lpmo_search_nonlocal", "lpmo_define_group", "lpmo_disp_bc", "lpmo_force_bc",
                      "lpmo_solve_step", "lpmo_get_field", "lpmo_set_field", "lpmo_unit_cell"):
             getattr(L, name).restype = ctypes.c_int
@@ -57,6 +58,7 @@ def lib():
         L.lpmo_force_bc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double]
         L.lpmo_search_nonlocal.argtypes = [vp, ctypes.c_double]
+        L.lpmo_update_fatigue_damage.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.lpmo_unit_cell.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.lpmo_set_threads.argtypes = [ctypes.c_int]
         L.lpmo_ct_search_neighbor.restype = ctypes.c_int
@@ -261,6 +263,9 @@ class Oracle:
 
     def update_static_damage(self, undamaged=-1):
         return bool(lib().lpmo_update_static_damage(self.h, int(undamaged)))
+
+    def update_fatigue_damage(self, undamaged=-1, dNdt=1.0):
+        return bool(lib().lpmo_update_fatigue_damage(self.h, int(undamaged), float(dNdt)))
 
     def store_state(self):
         lib().lpmo_store_state(self.h)
