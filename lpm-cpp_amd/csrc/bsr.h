@@ -8,19 +8,60 @@
 namespace lpmg {
 
 constexpr int kBS = 256;         // vector kernels
-constexpr int kSpBS = 1024;      // SpMV kernels: 16 wavefronts per workgroup
+// SpMV kernels: 8 wavefronts per workgroup.  The CG SpMV ends in a workgroup
+// reduction of its dot product, whose barrier holds every wavefront until the
+// slowest row is done; 8 wavefronts wait less than 16 (SC 40^3: 51.9 vs
+// 58.2 us per CG SpMV).
+constexpr int kSpBS = 512;
 constexpr int kMaxGrid = 1024;   // workgroups of the grid-strided vector kernels
-constexpr int kMaxParts = 4096;  // SpMV partials summed directly by the consumer
+constexpr int kMaxParts = 8192;  // SpMV partials summed directly by the consumer
+
+// One chunk of LPR consecutive blocks of a row: this lane's block e0 + sl
+// (clamped to the row's last block when past the end; `ok` says whether it counts).
+template <int D, typename T, int LPR>
+struct BsrChunk {
+  int col;
+  bool ok;
+  T v[D * D];
+  __device__ __forceinline__ void load(int e0, int sl, int hi, const int* __restrict__ conns,
+                                       const T* __restrict__ val) {
+    const int e = e0 + sl;
+    ok = e < hi;
+    const int ec = ok ? e : hi - 1;  // every row holds its self block, so hi - 1 >= lo
+    col = conns[ec];
+    const T* vb = val + (size_t)ec * D * D;
+#pragma unroll
+    for (int k = 0; k < D * D; ++k) v[k] = vb[k];
+  }
+  __device__ __forceinline__ void apply(const T* __restrict__ x, T* acc) const {
+    T xv[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xv[k] = x[(size_t)col * D + k];
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      T t = v[q * D] * xv[0];
+#pragma unroll
+      for (int k = 1; k < D; ++k) t += v[q * D + k] * xv[k];
+      acc[q] += ok ? t : T(0);
+    }
+  }
+};
 
 // y = A x for the rows of wavefronts of workgroup `blk` (64 / LPR rows per
 // wavefront, LPR lanes each).  With DOT, returns this lane's share of x.y
 // (nonzero only in the row's first b lanes).
+//
+// A row's pass is latency-bound: row start -> column index -> x gather are
+// three dependent loads.  With 32 lanes per row a pass covers two chunks (64
+// blocks) and issues the second chunk's loads before the first chunk's x
+// gather waits, so two rows per wavefront keep twice the bytes in flight; the
+// second chunk is skipped when no row of the wavefront reaches it.
 template <int D, typename T, int LPR, bool DOT>
 __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            T* __restrict__ y, int N) {
-  constexpr int DD = D * D;
   constexpr int RPW = 64 / LPR;
+  constexpr int U = LPR == 32 ? 2 : 1;
   const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
   const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
   const int row = wave * RPW + sub;
@@ -30,23 +71,22 @@ __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr
   T acc[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) acc[q] = T(0);
-  for (int e0 = lo; e0 < hi; e0 += LPR) {
-    const int e = e0 + sl;
-    const bool ok = e < hi;
-    const int ec = ok ? e : hi - 1;  // every row holds its self block, so hi - 1 >= lo
-    const int col = conns[ec];
-    const T* vb = val + (size_t)ec * DD;
-    T v[DD], xv[D];
-#pragma unroll
-    for (int k = 0; k < DD; ++k) v[k] = vb[k];
-#pragma unroll
-    for (int k = 0; k < D; ++k) xv[k] = x[(size_t)col * D + k];
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      T t = v[q * D] * xv[0];
-#pragma unroll
-      for (int k = 1; k < D; ++k) t += v[q * D + k] * xv[k];
-      acc[q] += ok ? t : T(0);
+  for (int e0 = lo; e0 < hi; e0 += U * LPR) {
+    BsrChunk<D, T, LPR> c0;
+    c0.load(e0, sl, hi, conns, val);
+    if (U == 1) {
+      c0.apply(x, acc);
+    } else {
+      // wavefront-uniform: does any row of this wavefront have blocks past e0 + LPR?
+      const bool more = __any(e0 + LPR < hi);
+      if (more) {
+        BsrChunk<D, T, LPR> c1;
+        c1.load(e0 + LPR, sl, hi, conns, val);
+        c0.apply(x, acc);
+        c1.apply(x, acc);
+      } else {
+        c0.apply(x, acc);
+      }
     }
   }
 #pragma unroll

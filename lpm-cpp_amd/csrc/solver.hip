@@ -306,18 +306,25 @@ int grid_for(int64_t work, int per) {
   return (int)g;
 }
 
-// lanes per row: 32 when every row fits in half a wavefront, else 64
-int lanes_per_row(const Context* c) { return c->max_nconn <= 32 ? 32 : 64; }
+// Lanes per row.  32 lanes (two rows per wavefront) when every row fits in
+// half a wavefront.  FP32 rows of up to 64 blocks also take 32 lanes, with two
+// chunks per pass: its bytes per row are half FP64's, so the row's dependent
+// loads bound it, and two rows in flight per wavefront pay (SC 40^3: 22.9 vs
+// 30.0 us per SpMV); FP64 rows beyond 32 blocks keep one row of 64 lanes per
+// wavefront (48.2 vs 57.0 us).
+int lanes_per_row(const Context* c, bool fp32) {
+  return c->max_nconn <= (fp32 ? 64 : 32) ? 32 : 64;
+}
 
-int spmv_grid(const Context* c) {
-  const int rows_per_block = (kSpBS / 64) * (64 / lanes_per_row(c));
+int spmv_grid(const Context* c, bool fp32) {
+  const int rows_per_block = (kSpBS / 64) * (64 / lanes_per_row(c, fp32));
   return (c->N + rows_per_block - 1) / rows_per_block;
 }
 
 template <int D, typename T>
 void launch_bsr(Context* c, const T* val, const T* x, T* y) {
-  const int g = spmv_grid(c);
-  if (lanes_per_row(c) == 32)
+  const int g = spmv_grid(c, sizeof(T) == 4);
+  if (lanes_per_row(c, sizeof(T) == 4) == 32)
     spmv_kernel<D, T, 32><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
   else
     spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
@@ -369,7 +376,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
   q->N = c->N;
   q->n = (int64_t)D * c->N;
   q->gv = grid_for(q->n, kVBS);
-  q->gs = spmv_grid(c);
+  q->gs = spmv_grid(c, sizeof(T) == 4);
   q->two_level = q->gs > kMaxParts;
   q->nS = q->two_level ? kBS : q->gs;
   q->partV0 = c->partials;
@@ -377,7 +384,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
   q->partS_raw = c->partials + 4 * kMaxGrid;
   q->partS = q->two_level ? q->partS_raw + q->gs : q->partS_raw;
   q->sc = c->sc;
-  q->cls = cg_class(D, sizeof(T) == 4, lanes_per_row(c));
+  q->cls = cg_class(D, sizeof(T) == 4, lanes_per_row(c, sizeof(T) == 4));
   dinv_kernel<D, T><<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->dpos, (const T*)q->val, c->N, o.precond,
                                                               (T*)q->dinv);
   const int64_t maxit = o.maxit > 0 ? o.maxit : q->n;
@@ -393,7 +400,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
 // parity of the vector-step partial buffers is fixed per launch slot.
 template <int D, typename T>
 int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
-  const int lpr = lanes_per_row(c);
+  const int lpr = lanes_per_row(c, sizeof(T) == 4);
   c->sc_host_iter_launched = 0;
   auto enqueue_chunk = [&](std::vector<hipEvent_t>* evs) -> int {
     for (int k = 0; k < kCgChunk; ++k) {
