@@ -6,9 +6,12 @@
 // Stiffness::updateStiffnessDispBC (stiffness.h:363-438).
 //
 // Kernel 1 (tangent): one wavefront per particle row i, one lane per connection
-// j.  Every lane walks the same bond paths i->k->m of the row (wave-uniform
-// loads) and keeps the terms whose end is its own j, in the reference's loop
-// order, so each lane's K_local(i,j) is the reference's value bit for bit.
+// j.  Every lane walks the same bond paths i->k->m of the row and keeps the
+// terms whose end is its own j, in the reference's loop order, so each lane's
+// K_local(i,j) is the reference's value bit for bit.  Rows of up to 64 bonds
+// (every lattice here) stage the paths in lanes and broadcast them
+// (tangent_lanes_kernel); longer rows walk them with wave-uniform loads
+// (tangent_kernel).
 // Kernel 2 (symmetrise + Dirichlet): owner-computes symmetrisation with no
 // atomics: the lane of block (i,j), i<j, reads K_local(i,j) and K_local(j,i)
 // through the precomputed transpose position, writes
@@ -147,6 +150,172 @@ tangent_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
   }
 }
 
+// The same tangent with the row's bond paths staged in lanes instead of walked
+// through wave-uniform loads (rows of at most 64 bonds).  Lane t holds bond
+// b_lo + t of particle i (its end k, direction, 1 - bdamage, k's layer ranges
+// and Tv[k]); for each bond b2 of a layer the lanes load k's bonds of that
+// layer in one parallel round, and every lane scans them through cross-lane
+// broadcasts for its own j.  The walk -- b2 ascending, the direct or
+// common-neighbour term, then k's bonds ascending -- and every expression are
+// those of tangent_kernel, so each K_local(i,j) has the same bits; what changes
+// is that one path step costs one parallel load round instead of a chain of
+// dependent scalar loads.
+template <int D>
+__global__ void __launch_bounds__(64 * kWaves)
+tangent_lanes_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
+                     const int* __restrict__ p2, const double* __restrict__ cs,
+                     const double* __restrict__ bdamage, const double* __restrict__ cs_sum,
+                     const double* __restrict__ Kn, const double* __restrict__ Tv,
+                     const int* __restrict__ cptr, const int* __restrict__ conns, int N,
+                     double* __restrict__ H, double* __restrict__ diag_sq) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kWaves + w;
+  if (i >= N) return;  // wave-uniform
+  const int c_lo = cptr[i], c_hi = cptr[i + 1];
+  const int b_lo = bptr[i], n0 = nb0[i], nbi = bptr[i + 1] - b_lo;
+  const double tv_i = Tv[i];
+  // lane t: bond b_lo + t of i
+  int bk = 0, kr[3] = {0, 0, 0};
+  double bc[3] = {0, 0, 0}, bom = 0, btv = 0;
+  if (lane < nbi) {
+    const int b = b_lo + lane;
+    bk = p2[b];
+    bc[0] = cs[3 * b]; bc[1] = cs[3 * b + 1]; bc[2] = cs[3 * b + 2];
+    bom = 1 - bdamage[b];
+    kr[0] = bptr[bk]; kr[1] = kr[0] + nb0[bk]; kr[2] = bptr[bk + 1];
+    btv = Tv[bk];
+  }
+  for (int e0 = c_lo; e0 < c_hi; e0 += 64) {
+    const int e = e0 + lane;
+    const bool active = e < c_hi;
+    const int j = active ? conns[e] : -1;
+    double K[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+    for (int L = 0; L < 2; ++L) {
+      const int tlo = L == 0 ? 0 : n0, thi = L == 0 ? n0 : nbi;
+      int t1 = -1;
+      for (int t = tlo; t < thi; ++t)
+        if (__shfl(bk, t, 64) == j && t1 < 0) t1 = t;
+      bool found = t1 >= 0;
+      // Every lane takes part in the broadcasts (a lane reads another lane's
+      // registers only while both run the same instruction); lanes without a
+      // direct bond discard what they read.
+      const int src1 = t1 >= 0 ? t1 : 0;
+      double c1[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double v = __shfl(bc[q], src1, 64);
+        c1[q] = t1 >= 0 ? v : 0.0;
+      }
+      const double om1v = __shfl(bom, src1, 64);
+      const double om1 = t1 >= 0 ? om1v : 0.0;
+      double T[D][D];
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) T[r][s] = 0.0;
+      const double kn = Kn[2 * i + L];
+      for (int t = tlo; t < thi; ++t) {
+        const int k = __shfl(bk, t, 64);
+        double c2[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) c2[q] = __shfl(bc[q], t, 64);
+        const double om2 = __shfl(bom, t, 64);
+        const int klo = __shfl(kr[L], t, 64), khi = __shfl(kr[L + 1], t, 64);
+        const double tv_k = __shfl(btv, t, 64);
+        const bool direct = t1 >= 0 && k == j;
+        if (direct) {
+          // direct bond term (derivative.h:59-88 / 156-185)
+          const double* csum = cs_sum + (2 * k + L) * 3;
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) {
+              const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
+              const double geo = (r == s) ? (kn + tv_i) * (c2[r] * c2[r])
+                                          : (kn + tv_i) * c2[hi_rs] * c2[lo_rs];
+              T[r][s] += om2 * ((-kn * c2[s] + tv_i * csum[s]) * c2[r] - geo);
+            }
+        } else if (t1 >= 0) {
+          // common-neighbour term through i's dilatation (derivative.h:91-99 / 188-196)
+          const double coef = 0.5 * (om1 * tv_i + om2 * tv_i);
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) T[r][s] -= coef * c2[r] * c1[s];
+        }
+        // two-hop paths i->k->j through k's dilatation (derivative.h:101-115 /
+        // 122-139 and 198-212 / 219-236): k's bonds of layer L, one per lane
+        const int nk = khi - klo;
+        int q3 = -1;
+        double c3l[3] = {0, 0, 0}, om3l = 0;
+        if (lane < nk) {
+          const int b3 = klo + lane;
+          q3 = p2[b3];
+          c3l[0] = cs[3 * b3]; c3l[1] = cs[3 * b3 + 1]; c3l[2] = cs[3 * b3 + 2];
+          om3l = 1 - bdamage[b3];
+        }
+        for (int u = 0; u < nk; ++u) {
+          const int m = __shfl(q3, u, 64);
+          double c3[3];
+#pragma unroll
+          for (int q = 0; q < 3; ++q) c3[q] = __shfl(c3l[q], u, 64);
+          const double om3 = __shfl(om3l, u, 64);
+          if (direct || m != j) continue;
+          found = true;
+          const double coef3 = 0.5 * (om3 * tv_k + om2 * tv_i);
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) T[r][s] -= coef3 * c2[r] * c3[s];
+        }
+      }
+      if (found)
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int s = 0; s < D; ++s) K[r][s] = K[r][s] + T[r][s];
+    }
+    if (j == i) {
+      // fdu2dxyz (derivative.h:7-45): upper triangle of the diagonal block
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+      for (int L = 0; L < 2; ++L) {
+        const int lo = L == 0 ? b_lo : b_lo + n0, hi = L == 0 ? b_lo + n0 : b_lo + nbi;
+        const double kn = Kn[2 * i + L];
+        const double* csum = cs_sum + (2 * i + L) * 3;
+        for (int b = lo; b < hi; ++b) {
+          const double om = 1 - bdamage[b];
+          const double c[3] = {cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = r; s < D; ++s) {
+              const double geo = (r == s) ? (kn + tv_i) * (c[r] * c[r]) : (kn + tv_i) * c[s] * c[r];
+              K[r][s] += om * ((kn * c[s] + tv_i * csum[s]) * c[r] + geo);
+            }
+        }
+      }
+      double dsq = 0;
+#pragma unroll
+      for (int r = 0; r < D; ++r) dsq += K[r][r] * K[r][r];
+      diag_sq[i] = dsq;
+    }
+    if (active) {
+      double* out = H + (size_t)e * D * D;
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) out[r * D + s] = K[r][s];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
 norm_diag_kernel(const double* __restrict__ diag_sq, int N, double* __restrict__ partials,
                  Scalars* sc) {
@@ -271,17 +440,27 @@ int launch_assemble(Context* c, int want32) {
     if (int rc = update_floating(c)) return rc;
   float* k32 = want32 ? c->Kval32 : nullptr;
   if (c->dim == 3) {
-    tangent_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
-                                                         c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                         N, c->Kval, c->diag_sq);
+    if (c->max_nb <= 64)
+      tangent_lanes_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                                 c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                                 N, c->Kval, c->diag_sq);
+    else
+      tangent_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                           c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                           N, c->Kval, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     symmetrize_bc_kernel<3><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
                                                       c->damage_visual, c->floating, c->sc, c->Kval, k32);
   } else {
-    tangent_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
-                                                         c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                         N, c->Kval, c->diag_sq);
+    if (c->max_nb <= 64)
+      tangent_lanes_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                                 c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                                 N, c->Kval, c->diag_sq);
+    else
+      tangent_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                           c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                           N, c->Kval, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     symmetrize_bc_kernel<2><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,

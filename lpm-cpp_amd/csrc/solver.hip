@@ -93,14 +93,22 @@ struct CGStep {
   bool stop;
 };
 
+// Both partial sets go through one workgroup reduction; each sum keeps its own
+// per-thread order, so the bits are those of two separate reductions.
 __device__ __forceinline__ CGStep cg_scalars(const double* __restrict__ partV, int nV,
                                              const double* __restrict__ partS, int nS, const Scalars* sc, int i) {
-  Vec<2> gv = sum_partials<kVBS, 2>(partV, nV);
-  Vec<1> dv = sum_partials<kVBS, 1>(partS, nS);
+  Vec<3> a;
+  a.v[0] = a.v[1] = a.v[2] = 0;
+  for (int b = threadIdx.x; b < nV; b += kVBS) {
+    a.v[0] += partV[2 * (size_t)b];
+    a.v[1] += partV[2 * (size_t)b + 1];
+  }
+  for (int b = threadIdx.x; b < nS; b += kVBS) a.v[2] += partS[b];
+  a = block_sum<kVBS, 3>(a);
   CGStep st;
-  st.gamma = gv.v[0];
-  st.rr = gv.v[1];
-  st.delta = dv.v[0];
+  st.gamma = a.v[0];
+  st.rr = a.v[1];
+  st.delta = a.v[2];
   if (i == 0) {
     st.beta = 0;
     st.alpha = st.gamma / st.delta;
