@@ -252,8 +252,11 @@ __global__ void bond_init_kernel(const double4* __restrict__ xyz, const int* __r
   }
 }
 
-constexpr int kConnWaves = 4;     // particles (wavefronts) per workgroup
-constexpr int kConnCap = 1024;    // candidate capacity per particle
+// Candidates of a particle are its bonds' ends and their same-layer bonds' ends
+// (a regular lattice: about 200-350); 4096 per particle leave room for bond
+// lists with hub particles.  LDS: kConnWaves * kConnCap * 5 bytes = 40 KB.
+constexpr int kConnWaves = 2;     // particles (wavefronts) per workgroup
+constexpr int kConnCap = 4096;    // candidate capacity per particle
 
 // Assembly::updateConnections: one wavefront per particle gathers the 1- and
 // 2-hop same-layer candidates into LDS, then each lane ranks its candidates

@@ -94,6 +94,20 @@ def bcc_block(nx, ny, nz, radius):
     return out
 
 
+def fcc_block(nx, ny, nz, radius):
+    """FCC block of nx*ny*nz cubic cells (4 particles per cell), cell edge
+    2*sqrt(2)*r, which makes the first shell 2r and the second shell 2*sqrt(2)*r
+    as the FCC unit cell's cutoffs (include/unit_cell.h:92-106); per cell the
+    corner particle, then the xy-, xz- and yz-face centres, cells x-fastest.
+    The reference reads FCC systems from dump files and has no generator."""
+    a = 2.0 * math.sqrt(2.0) * radius
+    Z, Y, X = np.meshgrid(np.arange(nz, dtype=np.float64), np.arange(ny, dtype=np.float64),
+                          np.arange(nx, dtype=np.float64), indexing="ij")
+    corner = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    basis = np.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    return ((corner[:, None, :] + basis[None, :, :]) * a).reshape(-1, 3)
+
+
 def unit_cell_info(lattice_type, radius):
     """Neighbour counts and shell cutoffs of UnitCell (include/unit_cell.h:29-122)."""
     r = float(radius)

@@ -36,6 +36,7 @@ LATTICES = [("hex", lat.HEXAGON2D, lambda: lat.hex_plate(11, 9, R)),
             ("square", lat.SQUARE2D, lambda: lat.sc_block(8, 7, 1, R)),
             ("sc", lat.SC3D, lambda: lat.sc_block(7, 6, 5, R)),
             ("bcc", lat.BCC3D, lambda: lat.bcc_block(4, 4, 5, R)),
+            ("fcc", lat.FCC3D, lambda: lat.fcc_block(3, 3, 4, R)),
             ("bcc_poly", lat.BCC3D, lambda: geometry.bcc_polycrystal(6, 5, radius=R, seed=11)["xyz"])]
 
 

@@ -35,7 +35,8 @@ def _jitter(xyz, amp, seed=0):
 CASES = [("hex", lat.HEXAGON2D, lambda: lat.hex_plate(10, 10, R)),
          ("square", lat.SQUARE2D, lambda: lat.sc_block(9, 9, 1, R)),
          ("sc", lat.SC3D, lambda: _sc(6)),
-         ("bcc", lat.BCC3D, lambda: lat.bcc_block(4, 4, 4, R))]
+         ("bcc", lat.BCC3D, lambda: lat.bcc_block(4, 4, 4, R)),
+         ("fcc", lat.FCC3D, lambda: lat.fcc_block(3, 3, 3, R))]
 
 
 def test_unit_cell_closed_forms():
@@ -79,7 +80,8 @@ def test_bonds_match_brute_force(name, L, gen):
 @pytest.mark.parametrize("name,L,gen,nb_full,nb0_full,nconn_full", [
     ("hex", lat.HEXAGON2D, lambda: lat.hex_plate(12, 12, R), 12, 6, 31),
     ("sc", lat.SC3D, lambda: _sc(8), 18, 6, 61),
-    ("bcc", lat.BCC3D, lambda: lat.bcc_block(6, 6, 6, R), 14, 8, 41)])
+    ("bcc", lat.BCC3D, lambda: lat.bcc_block(6, 6, 6, R), 14, 8, 41),
+    ("fcc", lat.FCC3D, lambda: lat.fcc_block(5, 5, 5, R), 18, 12, 61)])
 def test_shell_and_conn_counts(name, L, gen, nb_full, nb0_full, nconn_full):
     # unit_cell.h: nneighbors, nneighbors1 and nneighbors_AFEM + 1 (self)
     o = Oracle(L, R, lat.ELASTIC, gen())
