@@ -34,11 +34,16 @@ NEWTON_TOL = 1e-10
 # FP64 drop-in: Jacobi-PCG stopped where the reference stops MKL's CG (||r||^2 <=
 # 1e-12 ||r0||^2, solver.h:312), Newton tightened to 1e-10 so results meet 1e-8.
 FP64_OPTS = dict(precond=1, rtol=1e-12, atol=0.0, precision=0)
-# Mixed: FP32 stiffness and CG as an inexact inner solve (||r||^2 <= 1e-4 ||r0||^2)
-# of the FP64 modified Newton, whose FP64 residual refines to the same 1e-10.
-MIXED_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=1)
-# The FP64 path with the mixed path's inexact inner tolerance (isolates the
-# gain that comes from FP32 alone).
+# Mixed (the served configuration): FP32 stiffness and CG as an inexact inner
+# solve (||r||^2 <= 1e-4 ||r0||^2) of the FP64 modified Newton, whose FP64
+# residual refines to the same 1e-10, each load step starting from the last
+# step's increment (solver option 6).
+MIXED_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
+# Its parts, each measured alone: the mixed path without the predictor, the
+# FP64 path with the predictor, and the FP64 path at the mixed path's inexact
+# inner tolerance (the gain of FP32 alone is mixed_no_predictor over it).
+MIXED_NOPRED_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=1)
+FP64_PRED_OPTS = dict(precond=1, rtol=1e-12, atol=0.0, precision=0, predict=1.0)
 FP64_INEXACT_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=0)
 # the reference's settings: MKL CG unpreconditioned, dpar[0] = dpar[1] = 1e-12
 # (solver.h:311-313), Newton tol_iter = 1e-5 and 30 iterations (examples)
@@ -102,10 +107,11 @@ SERVE_TOL = 1e-9
 SERVE_FUSE_ROWS = 50000
 SERVE_SLOTS = 16
 SERVE_OPENING = 7.5e-6  # pin opening per load step, in units of W
-# Served path: FP32 inner solves; the server runs the requests of at most
-# SERVE_FUSE_ROWS particles in fused Jacobi-CG launches and the larger ones with
-# the multigrid CG on their own streams (precond < 0: the server's choice).
-SERVED_OPTS = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1)
+# Served path: FP32 inner solves and the load-step predictor; the server runs
+# the requests of at most SERVE_FUSE_ROWS particles in fused Jacobi-CG launches
+# and the larger ones with the multigrid CG on their own streams (precond < 0:
+# the server's choice).
+SERVED_OPTS = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
 
 
 def ct_stream(n_req, seed=SERVE_SEED):
@@ -269,8 +275,20 @@ def main():
         mprof = s.profile_read()
         s.profile(False)
         m_spmv_ms = mprof["spmv_ms"] / max(mprof["spmv_launches"], 1)
-        fi = [solve_once(s, gids, prob, FP64_INEXACT_OPTS, P) for _ in range(max(1, args.steps))]
-        ufi = s.get("XYZ") - s.get("XYZ_INITIAL")
+
+        def part(opts):
+            runs = [solve_once(s, gids, prob, opts, P) for _ in range(max(1, args.steps))]
+            u = s.get("XYZ") - s.get("XYZ_INITIAL")
+            return dict(value=len(runs) / sum(t for t, _, _ in runs), unit="solves/s",
+                        cg_iterations=float(np.mean([c for _, c, _ in runs])),
+                        newton_iterations=float(np.mean([n for _, _, n in runs])),
+                        disp_rel_err_vs_fp64=float(np.linalg.norm(u - u64) / np.linalg.norm(u64)))
+
+        parts = dict(mixed_no_predictor=part(MIXED_NOPRED_OPTS), fp64_predictor=part(FP64_PRED_OPTS),
+                     fp64_same_inner_tol=part(FP64_INEXACT_OPTS))
+        for k, v in parts.items():
+            print(f"# {k}: {1e3 / v['value']:.1f} ms, {v['cg_iterations']:.0f} CG iterations, "
+                  f"{v['newton_iterations']:.0f} Newton", file=sys.stderr, flush=True)
         mixed = dict(value=len(mt) / sum(mt), unit="solves/s", p50_ms=percentile(mt, 50) * 1e3,
                      p99_ms=percentile(mt, 99) * 1e3, cg_iterations=float(np.mean(mcg)),
                      speedup_vs_fp64=(len(mt) / sum(mt)) / (len(times) / sum(times)),
@@ -278,11 +296,8 @@ def main():
                      force_rel_err_vs_fp64=float(np.linalg.norm(fm - f64) / np.linalg.norm(f64)),
                      spmv_fp32_ms=m_spmv_ms,
                      spmv_fp32_GBps=mprof["spmv_bytes_fp32"] / (m_spmv_ms * 1e-3) / 1e9,
-                     inner_rtol_sq=MIXED_OPTS["rtol"],
-                     fp64_same_inner_tol=dict(value=len(fi) / sum(t for t, _, _ in fi), unit="solves/s",
-                                              cg_iterations=float(np.mean([c for _, c, _ in fi])),
-                                              disp_rel_err_vs_fp64=float(np.linalg.norm(ufi - u64) /
-                                                                         np.linalg.norm(u64))))
+                     inner_rtol_sq=MIXED_OPTS["rtol"], predictor=MIXED_OPTS["predict"],
+                     parts=parts)
 
     asm_ms, asm_bytes = s.time_kernel(2, reps=10)
     fwd_ms, fwd_bytes = s.time_kernel(3, reps=20)

@@ -88,6 +88,13 @@ enum {
  *   [4] precision: 0 FP64 stiffness and CG; 1 FP32 stiffness and CG inside the
  *       FP64 Newton (residual, refinement update and state stay FP64)
  *   [5] unused (16 CG iterations run between convergence checks)
+ *   [6] load-step predictor (lpmg_solve_step only; no reference counterpart):
+ *       s > 0 records where each converged load step ends, and a step after
+ *       one so recorded starts with every DOF that the stiffness does not
+ *       eliminate moved by s times its displacement over that step (pass
+ *       s = this step's load increment / the last converged step's, 1 for
+ *       uniform steps).  The equilibrium reached is the same; Newton starts
+ *       nearer to it.  0: the reference's start (only the BCs move particles).
  * Solver statistics, stats[4]: [0] CG iterations, [1] final ||r||^2 (CG) or
  * ||r|| (Newton), [2] Newton tolerance multiplier, [3] Newton iterations. */
 

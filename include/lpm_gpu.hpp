@@ -369,17 +369,28 @@ class SolverStatic : public Solver<nlayer> {
                                  this->opts.data(), this->last_stats.data()), "solveProblemStep") == 1;
   }
 
+  // With opts[6] > 0 (the load-step predictor) each step's predictor scale
+  // follows its size relative to the last converged step's across halvings.
   void solveProblem(std::vector<LoadStep<nlayer>>& load, int& start_index) {
     int n_step = (int)load.size();
     double dt = 1;
+    const double predict = this->opts[6];
+    std::vector<double> size(load.size(), 1.0);
+    double last = 1.0;
     for (int i = 0; i < n_step; i++) {
+      if (predict > 0) this->opts[6] = predict * size[i] / last;
       while (!solveProblemStep(load[i], dt)) {  // solver_static.h:78-87
         this->ass.resetStateVar(true);
         load[i].loadCutHalf();
         load.insert(load.begin() + i, load[i]);
+        size[i] /= 2;
+        size.insert(size.begin() + i, size[i]);
         ++n_step;
+        if (predict > 0) this->opts[6] = predict * size[i] / last;
       }
+      last = size[i];
     }
+    this->opts[6] = predict;
     start_index += n_step;
   }
 };

@@ -281,6 +281,7 @@ void parse_opts(const double* o, SolveOpts* so) {
   so->maxit = (int64_t)o[3];
   so->precision = (int)o[4];
   so->chunk = o[5] > 0 ? (int)o[5] : 16;
+  so->predict = o[6] > 0 ? o[6] : 0;
 }
 
 int do_update_rr(lpmg_context* h, double* nr, double* nreac) {
@@ -708,6 +709,8 @@ int lpmg_solve_step(lpmg_handle h, int undamaged, int max_iter, double tol, cons
   int n_newton = 0, passes = 0;
   double cgit = 0;
   bool new_damaged = false;
+  if (so.predict > 0 && c->inc_valid)
+    if (int rc = launch_predict(c, so.predict)) return rc;
   do {
     if (int rc = launch_assemble(c, so.precision)) return rc;
     double st[4] = {0, 0, 0, 0};
@@ -725,6 +728,8 @@ int lpmg_solve_step(lpmg_handle h, int undamaged, int max_iter, double tol, cons
       if (int rc = launch_forward(c)) return rc;
   } while (new_damaged);
   if (stats) { stats[0] = cgit; stats[1] = passes; stats[2] = n_newton; }
+  if (so.predict > 0 && n_newton < max_iter)
+    if (int rc = record_step_end(c)) return rc;
   API_HIP(hipStreamSynchronize(c->stream));
   return n_newton < max_iter ? 1 : 0;
 }
@@ -772,6 +777,7 @@ int lpmg_set_field(lpmg_handle h, int f, const double* in) {
   API_HIP(hipMallocAsync((void**)&tmp, sizeof(double) * n, c->stream));
   API_HIP(hipMemcpyAsync(tmp, in, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   if (double4* p = xyz_ptr(c, f)) {
+    c->step_valid = c->inc_valid = false;  // positions no longer follow the load steps
     pack_xyz_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(tmp, p, c->N);
   } else if (f == LPMG_F_CONSTRAINT) {
     d2i_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(tmp, c->cons, n);
@@ -802,6 +808,7 @@ int lpmg_restart(lpmg_handle h) {
   API_HIP(hipMemcpyAsync(c->dis_last, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));
   API_HIP(hipMemsetAsync(c->floating, 0, sizeof(int) * N, c->stream));
   c->floating_dirty = true;
+  c->step_valid = c->inc_valid = false;
   c->reac_sumsq = 0;
   c->assembled64 = c->assembled32 = false;
   API_HIP(hipStreamSynchronize(c->stream));

@@ -105,6 +105,10 @@ struct Context {
 
   // particles
   double4 *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr;
+  // load-step predictor: positions at the end of the last converged load step
+  // and that step's displacement increment (allocated on first use)
+  double4 *xyz_step = nullptr, *xyz_inc = nullptr;
+  bool step_valid = false, inc_valid = false;
   double *Pex = nullptr, *Pin = nullptr, *stress = nullptr;
   double *damage = nullptr, *damage_last = nullptr, *damage_visual = nullptr;
   double *Ddot_local = nullptr, *Ddot_nonlocal = nullptr;
@@ -242,6 +246,7 @@ struct SolveOpts {
   double rtol = 1e-12, atol = 1e-12;  // ||r||^2 <= rtol*||r0||^2 + atol
   int64_t maxit = 0;    // 0 -> problem size
   int chunk = 16;       // unused: the chunk length is fixed (graph capture)
+  double predict = 0;   // > 0: lpmg_solve_step starts from this multiple of the last step's increment
 };
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 // ---- amg.hip
@@ -261,6 +266,8 @@ int launch_update_state_var(Context* c);
 int launch_static_damage(Context* c, int undamaged, int* any);
 int launch_fatigue_damage(Context* c, int undamaged, double dNdt, int* any);
 int launch_init_state_var(Context* c);
+int launch_predict(Context* c, double scale);
+int record_step_end(Context* c);
 int update_floating(Context* c);
 int store_state(Context* c);
 int reset_state(Context* c, int reset_xyz);

@@ -441,7 +441,7 @@ int64_t estimate_bytes(const lpmg_request* r) {
   const double nnzb = N + 4.0 * NB;
   const bool mixed = r->opts[4] != 0;
   const double n_partials = std::max(std::ceil(N / 256), 4096.0) + std::ceil(N / 8) + 4096 + 64;
-  double b = N * (584.0 + 92.0 * dim) + 16.0 * n_partials;     // particle and solver arrays, partials
+  double b = N * (672.0 + 92.0 * dim) + 16.0 * n_partials;     // particle and solver arrays, partials
   b += NB * 120.0;                                              // bond arrays
   b += nnzb * (12.0 + DD * 8.0 + (mixed ? DD * 4.0 : 0.0));    // conns, BSR values
   b += N * 48.0;                                                // setup scratch
@@ -521,6 +521,7 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
   if ((rc = lpmg_forward(h))) return rc;
   if (r.prepare_state && (rc = lpmg_update_state_var(h))) return rc;
   std::vector<double> scales(r.n_steps, 1.0);
+  double last_scale = 1.0;  // size of the last converged step, for the predictor
   int halvings = 0, done_steps = 0;
   double newton = 0, cg = 0, passes = 0;
   for (size_t i = 0; i < scales.size();) {
@@ -536,6 +537,7 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
       if (x.kind == 0 && (rc = lpmg_disp_bc(h, gid[x.group], x.mode, x.flag, x.value[0] * sc))) return rc;
     }
     double st[3] = {0, 0, 0};
+    if (r.opts[6] > 0) opts[6] = r.opts[6] * sc / last_scale;
     rc = lpmg_solve_step(h, r.undamaged, r.max_iter, r.tol, opts, st);
     if (rc < 0) return rc;
     if (rc == 0) {  // not converged: restore the last converged state, halve the step
@@ -558,6 +560,7 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
     newton += st[2];
     cg += st[0];
     passes += st[1];
+    last_scale = sc;
     ++done_steps;
     ++i;
   }

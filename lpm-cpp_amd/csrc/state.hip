@@ -213,6 +213,35 @@ __global__ void copy_xyz_kernel(double4* __restrict__ dst, const double4* __rest
   if (i < N) dst[i] = src[i];
 }
 
+// Load-step predictor: every DOF the stiffness does not eliminate (no
+// displacement BC, not fully damaged, not in a loose fragment) moves by
+// `scale` times its displacement over the last converged load step.
+__global__ void predict_kernel(double4* __restrict__ xyz, const double4* __restrict__ inc,
+                               const int* __restrict__ cons, const double* __restrict__ dvis,
+                               const int* __restrict__ floating, double scale, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double4 p = xyz[i];
+  const double4 d = inc[i];
+  if (!dof_fixed(cons, dvis, floating, i, 0)) p.x += scale * d.x;
+  if (!dof_fixed(cons, dvis, floating, i, 1)) p.y += scale * d.y;
+  if (!dof_fixed(cons, dvis, floating, i, 2)) p.z += scale * d.z;
+  xyz[i] = p;
+}
+
+// End of a converged load step: increment since the last one, and the new end
+__global__ void step_end_kernel(const double4* __restrict__ xyz, double4* __restrict__ step,
+                                double4* __restrict__ inc, int valid, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double4 p = xyz[i];
+  if (valid) {
+    const double4 q = step[i];
+    inc[i] = make_double4(p.x - q.x, p.y - q.y, p.z - q.z, 0.0);
+  }
+  step[i] = p;
+}
+
 }  // namespace
 
 int launch_update_state_var(Context* c) {
@@ -394,6 +423,25 @@ int store_state(Context* c) {
   LPMG_HIP(cp(c->bforce_last, c->bforce, NB * sizeof(double)));
   LPMG_HIP(cp(c->dLp_last, c->dLp, NB * sizeof(double)));
   LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_predict(Context* c, double scale) {
+  if (c->floating_dirty)
+    if (int rc = update_floating(c)) return rc;
+  predict_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->xyz, c->xyz_inc, c->cons, c->damage_visual,
+                                                             c->floating, scale, c->N);
+  LPMG_HIP(hipGetLastError());
+  return 0;
+}
+
+int record_step_end(Context* c) {
+  if (!c->xyz_step && (dalloc(c, &c->xyz_step, c->N) || dalloc(c, &c->xyz_inc, c->N))) return -3;
+  step_end_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->xyz, c->xyz_step, c->xyz_inc, c->step_valid ? 1 : 0,
+                                                              c->N);
+  LPMG_HIP(hipGetLastError());
+  c->inc_valid = c->step_valid;
+  c->step_valid = true;
   return 0;
 }
 

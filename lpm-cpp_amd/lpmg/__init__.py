@@ -108,9 +108,9 @@ def _err():
     return lib().lpmg_last_error().decode()
 
 
-def solver_opts(precond=1, rtol=1e-12, atol=1e-12, maxit=0, precision=0, chunk=16):
+def solver_opts(precond=1, rtol=1e-12, atol=1e-12, maxit=0, precision=0, chunk=16, predict=0.0):
     """opts[8] of lpmg_solve_linear / lpmg_newton / lpmg_solve_step."""
-    return np.array([precond, rtol, atol, maxit, precision, chunk, 0, 0], dtype=np.float64)
+    return np.array([precond, rtol, atol, maxit, precision, chunk, predict, 0], dtype=np.float64)
 
 
 class System:

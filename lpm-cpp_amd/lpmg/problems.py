@@ -120,15 +120,20 @@ def apply_step(s, gids, step, scale=1.0):
 
 
 def solve_problem(s, gids, prob, opts, on_step=None, max_halvings=8):
-    """SolverStatic::solveProblem (solver_static.h:64-95) with step halving."""
+    """SolverStatic::solveProblem (solver_static.h:64-95) with step halving.
+    With opts["predict"] > 0 (GPU path only) each step's predictor scale is
+    that value times the step's size over the last converged step's size."""
     stats = []
     queue = [(st, 1.0) for st in prob.steps]
     i = 0
     halvings = 0
+    predict = float(opts.get("predict", 0.0))
+    last_scale = 1.0
     while i < len(queue):
         step, scale = queue[i]
         apply_step(s, gids, step, scale)
-        ok, st = s.solve_step(prob.undamaged, prob.max_iter, prob.tol, **opts)
+        kw = dict(opts, predict=predict * scale / last_scale) if predict > 0 else opts
+        ok, st = s.solve_step(prob.undamaged, prob.max_iter, prob.tol, **kw)
         if not ok:
             if halvings >= max_halvings:
                 raise RuntimeError(f"step {i} did not converge after {halvings} halvings")
@@ -138,6 +143,7 @@ def solve_problem(s, gids, prob, opts, on_step=None, max_halvings=8):
             queue.insert(i, (step, scale / 2))
             continue
         stats.append(st)
+        last_scale = scale
         if on_step:
             on_step(i, st)
         i += 1
