@@ -253,24 +253,45 @@ __global__ void bond_init_kernel(const double4* __restrict__ xyz, const int* __r
 }
 
 // Candidates of a particle are its bonds' ends and their same-layer bonds' ends
-// (a regular lattice: about 200-350); 4096 per particle leave room for bond
-// lists with hub particles.  LDS: kConnWaves * kConnCap * 5 bytes = 40 KB.
-constexpr int kConnWaves = 2;     // particles (wavefronts) per workgroup
-constexpr int kConnCap = 4096;    // candidate capacity per particle
+// (a regular lattice: about 200-350).  The LDS per wavefront is sized to the
+// largest candidate count of the system (cand_count_kernel), so regular
+// lattices keep many wavefronts per CU and bond lists with hub particles still
+// fit: 4 wavefronts per workgroup up to kConnCap4 candidates, 1 up to
+// kConnCap1 (5 bytes per candidate, at most 64 KB per workgroup).
+constexpr int kConnCap4 = 3072;
+constexpr int kConnCap1 = 12288;
+
+__global__ void cand_count_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
+                                  const int* __restrict__ p2, int N, int* __restrict__ cmax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int C = 0;
+  for (int L = 0; L < 2; ++L) {
+    const int lo = L == 0 ? bptr[i] : bptr[i] + nb0[i];
+    const int hi = L == 0 ? bptr[i] + nb0[i] : bptr[i + 1];
+    for (int b = lo; b < hi; ++b) {
+      const int k = p2[b];
+      C += 1 + (L == 0 ? nb0[k] : bptr[k + 1] - bptr[k] - nb0[k]);
+    }
+  }
+  atomicMax(cmax, C);
+}
 
 // Assembly::updateConnections: one wavefront per particle gathers the 1- and
 // 2-hop same-layer candidates into LDS, then each lane ranks its candidates
 // (first occurrence, number of distinct smaller ids) -- a sort+unique without
 // a sort.  mode 0 counts (nconn, nconn_largeq), mode 1 writes conns and dpos.
-__global__ void __launch_bounds__(64 * kConnWaves)
+__global__ void __launch_bounds__(256)
 conns_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, const int* __restrict__ p2,
-             int N, int mode, int* __restrict__ nconn, int* __restrict__ ncl,
+             int N, int mode, int cap, int* __restrict__ nconn, int* __restrict__ ncl,
              const int* __restrict__ cptr, int* __restrict__ conns, int* __restrict__ dpos,
              int* __restrict__ err) {
-  __shared__ int cand[kConnWaves][kConnCap];
-  __shared__ unsigned char first[kConnWaves][kConnCap];
+  extern __shared__ int smem[];  // cand[waves][cap] then first[waves][cap]
+  const int waves = blockDim.x >> 6;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = blockIdx.x * kConnWaves + w;
+  int* const candw = smem + (size_t)w * cap;
+  unsigned char* const firstw = (unsigned char*)(smem + (size_t)waves * cap) + (size_t)w * cap;
+  const int i = blockIdx.x * waves + w;
   const bool active = i < N;
   int C = 0;
   if (active) {
@@ -289,37 +310,37 @@ conns_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, const in
         const int incl = wave_incl_scan(cnt);
         const int off = C + incl - cnt;
         if (b < hi) {
-          if (off + cnt <= kConnCap) {
-            cand[w][off] = k;
-            for (int q = klo; q < khi; ++q) cand[w][off + 1 + (q - klo)] = p2[q];
+          if (off + cnt <= cap) {
+            candw[off] = k;
+            for (int q = klo; q < khi; ++q) candw[off + 1 + (q - klo)] = p2[q];
           }
         }
         C += __shfl(incl, 63, 64);
       }
     }
-    if (C > kConnCap) {
+    if (C > cap) {
       if (lane == 0) atomicOr(err, 2);
       C = 0;
     }
   }
   __syncthreads();
   for (int t = lane; t < C; t += 64) {
-    const int v = cand[w][t];
+    const int v = candw[t];
     unsigned char f = 1;
     for (int u = 0; u < t; ++u)
-      if (cand[w][u] == v) { f = 0; break; }
-    first[w][t] = f;
+      if (candw[u] == v) { f = 0; break; }
+    firstw[t] = f;
   }
   __syncthreads();
   int my_distinct = 0, my_ge = 0;
   for (int t = lane; t < C; t += 64) {
-    if (!first[w][t]) continue;
-    const int v = cand[w][t];
+    if (!firstw[t]) continue;
+    const int v = candw[t];
     ++my_distinct;
     if (v >= i) ++my_ge;
     if (mode == 1) {
       int rank = 0;
-      for (int u = 0; u < C; ++u) rank += (first[w][u] && cand[w][u] < v) ? 1 : 0;
+      for (int u = 0; u < C; ++u) rank += (firstw[u] && candw[u] < v) ? 1 : 0;
       conns[cptr[i] + rank] = v;
       if (v == i) dpos[i] = cptr[i] + rank;
     }
@@ -533,9 +554,25 @@ int build_topology(Context* c) {
   int* nconn = nullptr;
   LPMG_HIP(hipMallocAsync((void**)&nconn, sizeof(int) * N, c->stream));
   if (dalloc(c, &c->ncl, N) || dalloc(c, &c->cptr, N + 1) || dalloc(c, &c->dpos, N)) return -3;
-  const int cb = (N + kConnWaves - 1) / kConnWaves;
-  conns_kernel<<<cb, 64 * kConnWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, N, 0, nconn, c->ncl,
-                                                      nullptr, nullptr, nullptr, c->err_flag);
+  int* cmax = nullptr;
+  int hmax = 0;
+  LPMG_HIP(hipMallocAsync((void**)&cmax, sizeof(int), c->stream));
+  LPMG_HIP(hipMemsetAsync(cmax, 0, sizeof(int), c->stream));
+  cand_count_kernel<<<nb, tb, 0, c->stream>>>(c->bptr, c->nb0, c->p2, N, cmax);
+  LPMG_HIP(hipGetLastError());
+  LPMG_HIP(copy_sync(c, &hmax, cmax, sizeof(int), hipMemcpyDeviceToHost));
+  LPMG_HIP(hipFreeAsync(cmax, c->stream));
+  if (hmax > kConnCap1) {
+    set_error("a particle has " + std::to_string(hmax) + " connection candidates, more than " +
+              std::to_string(kConnCap1));
+    return -1;
+  }
+  const int waves = hmax <= kConnCap4 ? 4 : 1;
+  const int cap = std::max(64, (hmax + 63) / 64 * 64);
+  const size_t shm = (size_t)waves * cap * (sizeof(int) + 1);
+  const int cb = (N + waves - 1) / waves;
+  conns_kernel<<<cb, 64 * waves, shm, c->stream>>>(c->bptr, c->nb0, c->p2, N, 0, cap, nconn, c->ncl, nullptr,
+                                                   nullptr, nullptr, c->err_flag);
   LPMG_HIP(hipGetLastError());
   if (check_err(c, "connection search (capacity, or a particle without bonds)")) return -1;
   int total = 0;
@@ -543,8 +580,8 @@ int build_topology(Context* c) {
   c->nnzb = total;
   if (dalloc(c, &c->conns, c->nnzb) || dalloc(c, &c->tpos, c->nnzb) || dalloc(c, &c->brow, c->nnzb))
     return -3;
-  conns_kernel<<<cb, 64 * kConnWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, N, 1, nconn, c->ncl,
-                                                      c->cptr, c->conns, c->dpos, c->err_flag);
+  conns_kernel<<<cb, 64 * waves, shm, c->stream>>>(c->bptr, c->nb0, c->p2, N, 1, cap, nconn, c->ncl, c->cptr,
+                                                   c->conns, c->dpos, c->err_flag);
   bsr_aux_kernel<<<nb, tb, 0, c->stream>>>(c->cptr, c->conns, N, c->brow, c->tpos, c->err_flag);
   LPMG_HIP(hipGetLastError());
   if (check_err(c, "BSR transpose lookup")) return -1;
