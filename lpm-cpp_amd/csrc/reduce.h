@@ -61,6 +61,31 @@ __device__ inline void write_partial(double* __restrict__ part, Vec<NV> a) {
   write_partial_at<BS, NV>(part, blockIdx.x, a);
 }
 
+// This thread's share of n partials (entries threadIdx.x, +BS, +2BS, ...,
+// NV values each) added into a in that order.  Eight entries are loaded before
+// any is added, so their loads are in flight together (a plain loop waits for
+// each load before issuing the next); the loads are unconditional -- entries
+// past the end read the last one and are not added -- since a load inside a
+// branch is waited for at the branch's end.
+template <int BS, int NV>
+__device__ inline void add_partials(const double* __restrict__ part, int n, Vec<NV>& a) {
+  constexpr int U = 8;
+  for (int b0 = threadIdx.x; b0 < n; b0 += U * BS) {
+    double v[U][NV];
+#pragma unroll
+    for (int j = 0; j < U; ++j) {
+      const int bj = min(b0 + j * BS, n - 1);
+#pragma unroll
+      for (int k = 0; k < NV; ++k) v[j][k] = part[(size_t)NV * bj + k];
+    }
+#pragma unroll
+    for (int j = 0; j < U; ++j)
+      if (b0 + j * BS < n)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) a.v[k] += v[j][k];
+  }
+}
+
 // Sum of n partials written by an earlier kernel, in a fixed order; the same
 // bits in every thread of every workgroup.
 template <int BS, int NV>
@@ -68,9 +93,7 @@ __device__ inline Vec<NV> sum_partials(const double* __restrict__ part, int n) {
   Vec<NV> a;
 #pragma unroll
   for (int k = 0; k < NV; ++k) a.v[k] = 0;
-  for (int b = threadIdx.x; b < n; b += BS)
-#pragma unroll
-    for (int k = 0; k < NV; ++k) a.v[k] += part[(size_t)NV * b + k];
+  add_partials<BS, NV>(part, n, a);
   return block_sum<BS, NV>(a);
 }
 

@@ -93,18 +93,9 @@ struct CGStep {
   bool stop;
 };
 
-// Both partial sets go through one workgroup reduction; each sum keeps its own
-// per-thread order, so the bits are those of two separate reductions.
-__device__ __forceinline__ CGStep cg_scalars(const double* __restrict__ partV, int nV,
-                                             const double* __restrict__ partS, int nS, const Scalars* sc, int i) {
-  Vec<3> a;
-  a.v[0] = a.v[1] = a.v[2] = 0;
-  for (int b = threadIdx.x; b < nV; b += kVBS) {
-    a.v[0] += partV[2 * (size_t)b];
-    a.v[1] += partV[2 * (size_t)b + 1];
-  }
-  for (int b = threadIdx.x; b < nS; b += kVBS) a.v[2] += partS[b];
-  a = block_sum<kVBS, 3>(a);
+// gamma = (r,u), rr = (r,r), delta = (w,u) summed over the workgroup give the
+// step's coefficients and stop decision.
+__device__ __forceinline__ CGStep cg_step(const Vec<3>& a, const Scalars* sc, int i) {
   CGStep st;
   st.gamma = a.v[0];
   st.rr = a.v[1];
@@ -128,6 +119,12 @@ __device__ __forceinline__ CGStep cg_scalars(const double* __restrict__ partV, i
 // wavefronts of a workgroup take the same branch into the barriers below.  A
 // workgroup that still sees 0 reaches the same stop decision from the same
 // partials and returns without writing.
+//
+// Every load the step needs is issued before its first barrier (whose fence
+// waits for them): this thread's first element, its share of the partials and,
+// in thread 0, the flag -- one memory latency instead of three in a row.  The
+// partial sums keep the per-thread order and the wavefront-then-LDS tree of
+// block_sum, so their bits are unchanged.
 template <typename T>
 __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x, T* __restrict__ r,
                                             T* __restrict__ u, T* __restrict__ p, T* __restrict__ s,
@@ -136,11 +133,59 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
                                             const double* __restrict__ partS, int nS,
                                             double* __restrict__ partV_out, Scalars* sc) {
   __shared__ int done_s;
-  if (threadIdx.x == 0) done_s = sc->done;
+  __shared__ double red[kVBS / 64][3];
+  const int64_t k0 = (int64_t)blk * kVBS + threadIdx.x;
+  const bool has0 = k0 < n;
+  T u0 = 0, p0 = 0, w0 = 0, s0 = 0, x0 = 0, r0 = 0, d0 = 0;
+  if (has0) {
+    u0 = u[k0]; p0 = p[k0]; w0 = w[k0]; s0 = s[k0]; x0 = x[k0]; r0 = r[k0]; d0 = dinv[k0];
+  }
+  Vec<3> a;
+  a.v[0] = a.v[1] = a.v[2] = 0;
+  if (nV <= kVBS && nS <= 8 * kVBS) {  // one share of each set: all loads in flight at once
+    // unconditional loads (indices past the end read the last entry and are
+    // not added): a load inside a branch is waited for at the branch's end
+    const int b = threadIdx.x;
+    const bool hv = b < nV;
+    const int bv = hv ? b : max(nV - 1, 0);
+    const double v0 = partV_in[2 * (size_t)bv], v1 = partV_in[2 * (size_t)bv + 1];
+    double sv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sv[j] = partS[min(b + j * kVBS, max(nS - 1, 0))];
+    if (hv) {
+      a.v[0] += v0;
+      a.v[1] += v1;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (b + j * kVBS < nS) a.v[2] += sv[j];
+  } else {
+    Vec<2> av;
+    av.v[0] = av.v[1] = 0;
+    add_partials<kVBS, 2>(partV_in, nV, av);
+    Vec<1> as;
+    as.v[0] = 0;
+    add_partials<kVBS, 1>(partS, nS, as);
+    a.v[0] = av.v[0];
+    a.v[1] = av.v[1];
+    a.v[2] = as.v[0];
+  }
+  const int done0 = threadIdx.x == 0 ? sc->done : 0;
+  a = wave_sum(a);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) red[threadIdx.x >> 6][k] = a.v[k];
+  if (threadIdx.x == 0) done_s = done0;
   __syncthreads();
   if (done_s) return;
+  Vec<3> tot;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    tot.v[k] = 0;
+    for (int q = 0; q < kVBS / 64; ++q) tot.v[k] += red[q][k];
+  }
   const int i = sc->icur;
-  const CGStep st = cg_scalars(partV_in, nV, partS, nS, sc, i);
+  const CGStep st = cg_step(tot, sc, i);
   if (blk == 0 && threadIdx.x == 0) {
     if (i == 0) {
       sc->rr0 = st.rr;
@@ -160,7 +205,20 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
   if (st.stop) return;  // every workgroup reaches the same decision
   const T alpha = (T)st.alpha, beta = (T)st.beta;
   double ru = 0, rr = 0;
-  for (int64_t k = (int64_t)blk * kVBS + threadIdx.x; k < n; k += (int64_t)nblk * kVBS) {
+  if (has0) {
+    const T pk = u0 + beta * p0;
+    const T sk = w0 + beta * s0;
+    p[k0] = pk;
+    s[k0] = sk;
+    x[k0] = x0 + alpha * pk;
+    const T rk = r0 - alpha * sk;
+    r[k0] = rk;
+    const T uk = d0 * rk;
+    u[k0] = uk;
+    ru += (double)rk * (double)uk;
+    rr += (double)rk * (double)rk;
+  }
+  for (int64_t k = k0 + (int64_t)nblk * kVBS; k < n; k += (int64_t)nblk * kVBS) {
     const T pk = u[k] + beta * p[k];
     const T sk = w[k] + beta * s[k];
     p[k] = pk;
@@ -173,10 +231,10 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
     ru += (double)rk * (double)uk;
     rr += (double)rk * (double)rk;
   }
-  Vec<2> a;
-  a.v[0] = ru;
-  a.v[1] = rr;
-  write_partial_at<kVBS, 2>(partV_out, blk, a);
+  Vec<2> part;
+  part.v[0] = ru;
+  part.v[1] = rr;
+  write_partial_at<kVBS, 2>(partV_out, blk, part);
 }
 
 // ---- per-system launches
