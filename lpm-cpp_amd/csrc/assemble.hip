@@ -9,9 +9,10 @@
 // j.  Every lane walks the same bond paths i->k->m of the row and keeps the
 // terms whose end is its own j, in the reference's loop order, so each lane's
 // K_local(i,j) is the reference's value bit for bit.  Rows of up to 64 bonds
-// (every lattice here) stage the paths in lanes and broadcast them
-// (tangent_lanes_kernel); longer rows walk them with wave-uniform loads
-// (tangent_kernel).
+// whose two-hop path data fits in LDS (every lattice here) gather it in two
+// batched rounds (tangent_lds_kernel); otherwise rows of up to 64 bonds keep
+// them in lanes (tangent_lanes_kernel) and longer rows walk them with
+// wave-uniform loads (tangent_kernel).
 // Kernel 2 (symmetrise + Dirichlet): owner-computes symmetrisation with no
 // atomics: the lane of block (i,j), i<j, reads K_local(i,j) and K_local(j,i)
 // through the precomputed transpose position, writes
@@ -27,6 +28,9 @@ namespace lpmg {
 namespace {
 
 constexpr int kWaves = 4;
+// Path entries per wavefront in tangent_lds_kernel's LDS slice (36 bytes each
+// plus 512 bytes of offsets): kWaves * (36 * 416 + 512) B = 61.9 KB
+constexpr int kTangentLdsMax = 416;
 
 template <int D>
 __global__ void __launch_bounds__(64 * kWaves)
@@ -316,6 +320,218 @@ tangent_lanes_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
   }
 }
 
+// The tangent with every load of a row issued in two batched rounds: the
+// row's bonds one per lane (as in tangent_lanes_kernel), then all two-hop path
+// data -- for each bond i->k, k's bonds of the same layer (end, direction,
+// 1 - bdamage) -- gathered into this wavefront's slice of LDS, E entries at
+// most.  The path walk then reads LDS (all lanes the same address: a
+// broadcast) instead of waiting on global memory at every step.  Walk order and
+// expressions are those of tangent_kernel, so the bits are the same.
+template <int D>
+__global__ void __launch_bounds__(64 * kWaves)
+tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
+                   const int* __restrict__ p2, const double* __restrict__ cs,
+                   const double* __restrict__ bdamage, const double* __restrict__ cs_sum,
+                   const double* __restrict__ Kn, const double* __restrict__ Tv,
+                   const int* __restrict__ cptr, const int* __restrict__ conns, int N, int E,
+                   double* __restrict__ H, double* __restrict__ diag_sq) {
+  extern __shared__ double lds_tangent[];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kWaves + w;
+  const bool row = i < N;  // wave-uniform; every wavefront still reaches both barriers
+  double* const Lcs = lds_tangent + (size_t)w * 4 * E;  // [E][3]
+  double* const Lom = Lcs + 3 * E;                       // [E]
+  int* const Lm = (int*)(lds_tangent + (size_t)kWaves * 4 * E) + (size_t)w * (E + 128);
+  int* const Loff = Lm + E;                              // [64] first entry of bond t
+  int* const Lklo = Loff + 64;                           // [64] k's first bond of the layer
+  int c_lo = 0, c_hi = 0, b_lo = 0, n0 = 0, nbi = 0;
+  double tv_i = 0;
+  if (row) {
+    c_lo = cptr[i]; c_hi = cptr[i + 1];
+    b_lo = bptr[i]; n0 = nb0[i]; nbi = bptr[i + 1] - b_lo;
+    tv_i = Tv[i];
+  }
+  // lane t: bond b_lo + t of i and the layer range of its end's bonds
+  int bk = 0, klo = 0, nk = 0;
+  double bc[3] = {0, 0, 0}, bom = 0, btv = 0;
+  if (lane < nbi) {
+    const int b = b_lo + lane;
+    bk = p2[b];
+    bc[0] = cs[3 * b]; bc[1] = cs[3 * b + 1]; bc[2] = cs[3 * b + 2];
+    bom = 1 - bdamage[b];
+    const int k0 = bptr[bk], k1 = k0 + nb0[bk], k2 = bptr[bk + 1];
+    klo = lane < n0 ? k0 : k1;
+    nk = lane < n0 ? k1 - k0 : k2 - k1;
+    btv = Tv[bk];
+  }
+  const int incl = wave_incl_scan(nk);
+  const int off = incl - nk;
+  const int T = __shfl(incl, 63, 64);
+  if (lane < nbi) {
+    Loff[lane] = off;
+    Lklo[lane] = klo;
+  }
+  __syncthreads();
+  // gather: entry e = off_t + u is bond klo_t + u; four entries per lane in flight
+  for (int e0 = lane; e0 < T; e0 += 4 * 64) {
+    int b3[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = min(e0 + 64 * q, T - 1);
+      int lo = 0, hi = nbi - 1;  // last t with Loff[t] <= e
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (Loff[mid] <= e) lo = mid;
+        else hi = mid - 1;
+      }
+      b3[q] = Lklo[lo] + (e - Loff[lo]);
+    }
+    int m[4];
+    double c3[4][3], om3[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      m[q] = p2[b3[q]];
+      c3[q][0] = cs[3 * b3[q]]; c3[q][1] = cs[3 * b3[q] + 1]; c3[q][2] = cs[3 * b3[q] + 2];
+      om3[q] = 1 - bdamage[b3[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = e0 + 64 * q;
+      if (e < T) {
+        Lm[e] = m[q];
+        Lcs[3 * e] = c3[q][0]; Lcs[3 * e + 1] = c3[q][1]; Lcs[3 * e + 2] = c3[q][2];
+        Lom[e] = om3[q];
+      }
+    }
+  }
+  __syncthreads();
+  if (!row) return;
+  for (int e0 = c_lo; e0 < c_hi; e0 += 64) {
+    const int e = e0 + lane;
+    const bool active = e < c_hi;
+    const int j = active ? conns[e] : -1;
+    double K[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+    for (int L = 0; L < 2; ++L) {
+      const int tlo = L == 0 ? 0 : n0, thi = L == 0 ? n0 : nbi;
+      int t1 = -1;
+      for (int t = tlo; t < thi; ++t)
+        if (__shfl(bk, t, 64) == j && t1 < 0) t1 = t;
+      bool found = t1 >= 0;
+      const int src1 = t1 >= 0 ? t1 : 0;
+      double c1[3];
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const double v = __shfl(bc[q], src1, 64);
+        c1[q] = t1 >= 0 ? v : 0.0;
+      }
+      const double om1v = __shfl(bom, src1, 64);
+      const double om1 = t1 >= 0 ? om1v : 0.0;
+      double Tm[D][D];
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) Tm[r][s] = 0.0;
+      const double kn = Kn[2 * i + L];
+      for (int t = tlo; t < thi; ++t) {
+        const int k = __shfl(bk, t, 64);
+        double c2[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) c2[q] = __shfl(bc[q], t, 64);
+        const double om2 = __shfl(bom, t, 64);
+        const double tv_k = __shfl(btv, t, 64);
+        const int toff = __shfl(off, t, 64), tnk = __shfl(nk, t, 64);
+        const bool direct = t1 >= 0 && k == j;
+        if (direct) {
+          // direct bond term (derivative.h:59-88 / 156-185)
+          const double* csum = cs_sum + (2 * k + L) * 3;
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) {
+              const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
+              const double geo = (r == s) ? (kn + tv_i) * (c2[r] * c2[r])
+                                          : (kn + tv_i) * c2[hi_rs] * c2[lo_rs];
+              Tm[r][s] += om2 * ((-kn * c2[s] + tv_i * csum[s]) * c2[r] - geo);
+            }
+        } else {
+          if (t1 >= 0) {
+            // common-neighbour term through i's dilatation (derivative.h:91-99 / 188-196)
+            const double coef = 0.5 * (om1 * tv_i + om2 * tv_i);
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+              for (int s = 0; s < D; ++s) Tm[r][s] -= coef * c2[r] * c1[s];
+          }
+          // two-hop paths i->k->j through k's dilatation (derivative.h:101-115 /
+          // 122-139 and 198-212 / 219-236)
+          // four path ends read per round (reads past the list repeat its last
+          // entry and are skipped), matches added in ascending order
+          for (int u0 = 0; u0 < tnk; u0 += 4) {
+            int mm[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) mm[q] = Lm[toff + min(u0 + q, tnk - 1)];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              if (u0 + q >= tnk || mm[q] != j) continue;
+              const int qq = toff + u0 + q;
+              found = true;
+              const double coef3 = 0.5 * (Lom[qq] * tv_k + om2 * tv_i);
+              const double c3[3] = {Lcs[3 * qq], Lcs[3 * qq + 1], Lcs[3 * qq + 2]};
+#pragma unroll
+              for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int s = 0; s < D; ++s) Tm[r][s] -= coef3 * c2[r] * c3[s];
+            }
+          }
+        }
+      }
+      if (found)
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int s = 0; s < D; ++s) K[r][s] = K[r][s] + Tm[r][s];
+    }
+    if (j == i) {
+      // fdu2dxyz (derivative.h:7-45): upper triangle of the diagonal block
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+      for (int L = 0; L < 2; ++L) {
+        const int lo = L == 0 ? b_lo : b_lo + n0, hi = L == 0 ? b_lo + n0 : b_lo + nbi;
+        const double kn = Kn[2 * i + L];
+        const double* csum = cs_sum + (2 * i + L) * 3;
+        for (int b = lo; b < hi; ++b) {
+          const double om = 1 - bdamage[b];
+          const double c[3] = {cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = r; s < D; ++s) {
+              const double geo = (r == s) ? (kn + tv_i) * (c[r] * c[r]) : (kn + tv_i) * c[s] * c[r];
+              K[r][s] += om * ((kn * c[s] + tv_i * csum[s]) * c[r] + geo);
+            }
+        }
+      }
+      double dsq = 0;
+#pragma unroll
+      for (int r = 0; r < D; ++r) dsq += K[r][r] * K[r][r];
+      diag_sq[i] = dsq;
+    }
+    if (active) {
+      double* out = H + (size_t)e * D * D;
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) out[r * D + s] = K[r][s];
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
 norm_diag_kernel(const double* __restrict__ diag_sq, int N, double* __restrict__ partials,
                  Scalars* sc) {
@@ -439,8 +655,15 @@ int launch_assemble(Context* c, int want32) {
   if (c->floating_dirty)
     if (int rc = update_floating(c)) return rc;
   float* k32 = want32 ? c->Kval32 : nullptr;
+  // LDS-staged tangent when a row's two-hop path data fits 64 KB per workgroup
+  const int lds_E = (c->max_nb <= 64 && c->max_cand <= kTangentLdsMax) ? std::max(32, (c->max_cand + 31) / 32 * 32) : 0;
+  const size_t lds_bytes = (size_t)kWaves * ((size_t)4 * lds_E * sizeof(double) + (size_t)(lds_E + 128) * sizeof(int));
   if (c->dim == 3) {
-    if (c->max_nb <= 64)
+    if (lds_E > 0)
+      tangent_lds_kernel<3><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                                       c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                                       N, lds_E, c->Kval, c->diag_sq);
+    else if (c->max_nb <= 64)
       tangent_lanes_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                  c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
                                                                  N, c->Kval, c->diag_sq);
@@ -453,7 +676,11 @@ int launch_assemble(Context* c, int want32) {
     symmetrize_bc_kernel<3><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
                                                       c->damage_visual, c->floating, c->sc, c->Kval, k32);
   } else {
-    if (c->max_nb <= 64)
+    if (lds_E > 0)
+      tangent_lds_kernel<2><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
+                                                                       c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
+                                                                       N, lds_E, c->Kval, c->diag_sq);
+    else if (c->max_nb <= 64)
       tangent_lanes_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                  c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
                                                                  N, c->Kval, c->diag_sq);

@@ -102,6 +102,7 @@ struct Context {
   int ptype = 0, N = 0, dim = 3;
   int64_t NB = 0, nnzb = 0, nnz_upper = 0;
   int max_nb = 0, max_nconn = 0;
+  int max_cand = 0;  // largest sum over a particle's bonds of (1 + the bond end's same-layer bonds)
 
   // particles
   double4 *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr;

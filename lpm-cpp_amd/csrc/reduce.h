@@ -26,6 +26,18 @@ __device__ inline Vec<NV> wave_sum(Vec<NV> a) {
   return a;
 }
 
+// Inclusive prefix sum across one 64-wide wavefront.
+template <typename T>
+__device__ inline T wave_incl_scan(T v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
 // Sum over the workgroup; the result is valid in every thread.
 template <int BS, int NV>
 __device__ inline Vec<NV> block_sum(Vec<NV> a) {

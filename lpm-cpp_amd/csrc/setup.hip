@@ -13,6 +13,7 @@
 #include <cmath>
 
 #include "lpmg_internal.h"
+#include "reduce.h"
 
 namespace lpmg {
 
@@ -21,18 +22,6 @@ namespace {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 4;  // elements per thread
 constexpr int kScanTile = kScanBlock * kScanItems;
-
-// Inclusive prefix sum across one 64-wide wavefront.
-template <typename T>
-__device__ inline T wave_incl_scan(T v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    T t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
 
 // Exclusive scan of one tile of kScanTile elements; the block total goes to sums[blockIdx.x].
 template <typename T>
@@ -567,6 +556,7 @@ int build_topology(Context* c) {
               std::to_string(kConnCap1));
     return -1;
   }
+  c->max_cand = hmax;
   const int waves = hmax <= kConnCap4 ? 4 : 1;
   const int cap = std::max(64, (hmax + 63) / 64 * 64);
   const size_t shm = (size_t)waves * cap * (sizeof(int) + 1);
