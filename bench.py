@@ -194,6 +194,37 @@ def serving_bench(n_req, plain=True):
                 max_disp_rel_err_vs_fp64=max(errs), requests_with_identical_damage_flags=flags_equal)
 
 
+def spmv_configs():
+    """Back-to-back SpMV (50 launches each, FP64 then FP32) on the largest
+    system of each configuration, assembled at rest: the hex plate 100x100, SC
+    40^3, the BCC J2 bar 30x30x60 cells, a 250k-particle CT plate and the BCC
+    polycrystal 80^3 cells.  GB/s count the algorithmic bytes (matrix values
+    and block columns once, row pointers, x and y once); the FP32 matrices up to
+    the CT plate fit the 256 MB Infinity Cache, the polycrystal's (1.6 GB)
+    does not."""
+    import lpmg
+    from lpmg import problems as P
+    out = []
+    for name, mk in [("hex_plate_100x100", lambda: P.hex_plate()), ("sc_cube_40", lambda: P.sc_cube(n=40)),
+                     ("bcc_j2_bar_30x30x60", lambda: P.bcc_j2_bar()),
+                     ("ct_plate_250k", lambda: P.ct_plate(250000, 0.35)),
+                     ("bcc_polycrystal_80", lambda: P.bcc_polycrystal(80, 200))]:
+        prob = mk()
+        s, _ = P.build(lpmg.System, prob)
+        s.forward()
+        s.assemble(precision=1)
+        rec = dict(workload=name, particles=s.N, bsr_blocks=s.nnzb)
+        for prec, which in (("fp64", 0), ("fp32", 1)):
+            s.time_kernel(which, reps=5)
+            ms, nbytes = s.time_kernel(which, reps=50)
+            rec[prec] = dict(us=ms * 1e3, algorithmic_bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9,
+                             frac_of_hbm_peak=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        print(f"# {name}: {rec}", file=sys.stderr, flush=True)
+        out.append(rec)
+        s.close()
+    return dict(metric="back-to-back SpMV rate per configuration", results=out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +243,9 @@ def main():
                     help="run only the CT stream replay (--serve-requests N, at least 1)")
     ap.add_argument("--no-serve-plain", action="store_true",
                     help="with --serve-only: skip the plain FP64 replay (profiling the served path)")
+    ap.add_argument("--spmv-configs", action="store_true",
+                    help="only the back-to-back SpMV rates (FP64 and FP32) on every configuration's "
+                         "largest system, one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every solve of this workload runs on a single MI355X "
@@ -223,6 +257,9 @@ def main():
 
     if args.serve_only:
         print(json.dumps(serving_bench(max(1, args.serve_requests), plain=not args.no_serve_plain)))
+        return
+    if args.spmv_configs:
+        print(json.dumps(spmv_configs()))
         return
 
     prob = P.sc_cube(n=args.n, steps=20, total=0.05, E=70e3, nu=0.33)

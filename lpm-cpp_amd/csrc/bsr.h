@@ -52,16 +52,16 @@ struct BsrChunk {
 // (nonzero only in the row's first b lanes).
 //
 // A row's pass is latency-bound: row start -> column index -> x gather are
-// three dependent loads.  With 32 lanes per row a pass covers two chunks (64
-// blocks) and issues the second chunk's loads before the first chunk's x
-// gather waits, so two rows per wavefront keep twice the bytes in flight; the
-// second chunk is skipped when no row of the wavefront reaches it.
+// three dependent loads.  With LPR < 64 lanes per row a pass covers 64 / LPR
+// chunks (64 blocks) and issues every chunk's loads before the first chunk's x
+// gather waits, so 64 / LPR rows per wavefront keep that many times the bytes
+// in flight; a chunk that no row of the wavefront reaches is skipped.
 template <int D, typename T, int LPR, bool DOT>
 __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            T* __restrict__ y, int N) {
   constexpr int RPW = 64 / LPR;
-  constexpr int U = LPR == 32 ? 2 : 1;
+  constexpr int U = 64 / LPR;
   const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
   const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
   const int row = wave * RPW + sub;
@@ -72,22 +72,18 @@ __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr
 #pragma unroll
   for (int q = 0; q < D; ++q) acc[q] = T(0);
   for (int e0 = lo; e0 < hi; e0 += U * LPR) {
-    BsrChunk<D, T, LPR> c0;
-    c0.load(e0, sl, hi, conns, val);
-    if (U == 1) {
-      c0.apply(x, acc);
-    } else {
-      // wavefront-uniform: does any row of this wavefront have blocks past e0 + LPR?
-      const bool more = __any(e0 + LPR < hi);
-      if (more) {
-        BsrChunk<D, T, LPR> c1;
-        c1.load(e0 + LPR, sl, hi, conns, val);
-        c0.apply(x, acc);
-        c1.apply(x, acc);
-      } else {
-        c0.apply(x, acc);
+    BsrChunk<D, T, LPR> ch[U];
+    ch[0].load(e0, sl, hi, conns, val);
+    int nc = 1;
+#pragma unroll
+    for (int c = 1; c < U; ++c)
+      if (__any(e0 + c * LPR < hi)) {  // wavefront-uniform
+        ch[c].load(e0 + c * LPR, sl, hi, conns, val);
+        nc = c + 1;
       }
-    }
+#pragma unroll
+    for (int c = 0; c < U; ++c)
+      if (c < nc) ch[c].apply(x, acc);
   }
 #pragma unroll
   for (int o = LPR / 2; o > 0; o >>= 1)

@@ -87,7 +87,7 @@ struct CgBatchClass {
 };
 
 constexpr int kCgChunk = 16;     // CG iterations per launch batch (even)
-constexpr int kCgClasses = 8;
+constexpr int kCgClasses = 12;
 
 struct Context;
 struct Amg;  // multigrid hierarchy (amg.hip)

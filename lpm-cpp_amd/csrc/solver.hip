@@ -378,7 +378,11 @@ int grid_for(int64_t work, int per) {
 // loads bound it, and two rows in flight per wavefront pay (SC 40^3: 22.9 vs
 // 30.0 us per SpMV); FP64 rows beyond 32 blocks keep one row of 64 lanes per
 // wavefront (48.2 vs 57.0 us).
+// Rows of 33-48 blocks (BCC: 41) take 16 lanes, four rows per wavefront, in
+// both precisions: 64 lanes would leave a third idle (BCC J2 bar FP64: 58.4 vs
+// 68.7 us; the 1.0M polycrystal: 518 vs 641 us FP64, 274 vs 294 us FP32).
 int lanes_per_row(const Context* c, bool fp32) {
+  if (c->max_nconn > 32 && c->max_nconn <= 48) return 16;
   return c->max_nconn <= (fp32 ? 64 : 32) ? 32 : 64;
 }
 
@@ -390,7 +394,10 @@ int spmv_grid(const Context* c, bool fp32) {
 template <int D, typename T>
 void launch_bsr(Context* c, const T* val, const T* x, T* y) {
   const int g = spmv_grid(c, sizeof(T) == 4);
-  if (lanes_per_row(c, sizeof(T) == 4) == 32)
+  const int lpr = lanes_per_row(c, sizeof(T) == 4);
+  if (lpr == 16)
+    spmv_kernel<D, T, 16><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
+  else if (lpr == 32)
     spmv_kernel<D, T, 32><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
   else
     spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
@@ -402,12 +409,18 @@ void launch_bsr(Context* c, const T* val, const T* x, T* y) {
 template <int D, typename T>
 void launch_cg_spmv(hipStream_t st, const CgSys& q, int lpr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
   if (e0) {  // instrumented (eager) launches only; captured chunks use plain launches
-    if (lpr == 32)
+    if (lpr == 16)
+      hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 16>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
+                            (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
+    else if (lpr == 32)
       hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 32>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
                             (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
     else
       hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 64>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
                             (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
+  } else if (lpr == 16) {
+    cg_spmv_kernel<D, T, 16><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
+                                                     q.partS_raw, q.sc);
   } else if (lpr == 32) {
     cg_spmv_kernel<D, T, 32><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
                                                      q.partS_raw, q.sc);
@@ -539,7 +552,9 @@ void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
 
 }  // namespace
 
-int cg_class(int dim, int fp32, int lpr) { return (dim == 3 ? 4 : 0) + (fp32 ? 2 : 0) + (lpr == 64 ? 1 : 0); }
+int cg_class(int dim, int fp32, int lpr) {
+  return (dim == 3 ? 6 : 0) + (fp32 ? 3 : 0) + (lpr == 64 ? 2 : lpr == 32 ? 1 : 0);
+}
 
 // Workgroup offsets of each system in the fused grids; the live totals.
 void cg_batch_layout(CgSys* tab, int nsys, CgBatchHdr* h) {
@@ -564,14 +579,18 @@ int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int ncla
       const CgBatchClass& b = classes[c];
       if (b.sp_grid == 0) continue;
       switch (b.cls) {
-        case 0: launch_fused_class<2, double, 32>(st, b, k); break;
-        case 1: launch_fused_class<2, double, 64>(st, b, k); break;
-        case 2: launch_fused_class<2, float, 32>(st, b, k); break;
-        case 3: launch_fused_class<2, float, 64>(st, b, k); break;
-        case 4: launch_fused_class<3, double, 32>(st, b, k); break;
-        case 5: launch_fused_class<3, double, 64>(st, b, k); break;
-        case 6: launch_fused_class<3, float, 32>(st, b, k); break;
-        case 7: launch_fused_class<3, float, 64>(st, b, k); break;
+        case 0: launch_fused_class<2, double, 16>(st, b, k); break;
+        case 1: launch_fused_class<2, double, 32>(st, b, k); break;
+        case 2: launch_fused_class<2, double, 64>(st, b, k); break;
+        case 3: launch_fused_class<2, float, 16>(st, b, k); break;
+        case 4: launch_fused_class<2, float, 32>(st, b, k); break;
+        case 5: launch_fused_class<2, float, 64>(st, b, k); break;
+        case 6: launch_fused_class<3, double, 16>(st, b, k); break;
+        case 7: launch_fused_class<3, double, 32>(st, b, k); break;
+        case 8: launch_fused_class<3, double, 64>(st, b, k); break;
+        case 9: launch_fused_class<3, float, 16>(st, b, k); break;
+        case 10: launch_fused_class<3, float, 32>(st, b, k); break;
+        case 11: launch_fused_class<3, float, 64>(st, b, k); break;
         default: set_error("bad CG batch class"); return -1;
       }
     }
