@@ -2,17 +2,16 @@
 // MKL's RCI CG (Solver::LPM_CG, solver.h:276-354) and PARDISO
 // (Solver::LPM_PARDISO, solver.h:190-274) as the linear solve of the Newton step.
 //
-// SpMV: one lane per b x b block.  A row (<= 64 blocks for the SC/BCC lattices,
-// <= 32 for the 2D ones) is owned by 64 or 32 lanes, so one wavefront covers one
-// 3D row or two 2D rows in a single pass: each lane loads its block's b*b values
-// (a contiguous 72/32-byte run, the wave's loads together one contiguous
-// segment), its block column and the b entries of x, multiplies, and the row's b
-// sums are reduced across the row's lanes.  All loads are unconditional
-// straight-line code (indices past the row's end are clamped and their products
-// dropped), so they are in flight together; with one row per wavefront the
-// hardware scheduler hides the latency with other wavefronts.  Measured on the
-// SC 40^3 pattern this streams at 6.4 TB/s (FP64), 80% of the 8 TB/s HBM peak.
-// The operator is sparse, so MFMA is not used.
+// SpMV: one lane per b x b block.  A row of up to 64 blocks (every lattice here)
+// is owned by 16 lanes, four rows per wavefront, and a pass issues up to four
+// 16-block chunks of the row: each lane loads its block's b*b values (a
+// contiguous 72/32-byte run, the row's loads together one contiguous segment),
+// its block column and the b entries of x, multiplies, and the row's b sums are
+// reduced across the row's lanes.  All loads are unconditional straight-line
+// code (indices past the row's end are clamped and their products dropped), so
+// a row's loads are in flight together.  Measured on the SC 40^3 pattern this
+// streams at 6.3 TB/s back to back (FP64), 79% of the 8 TB/s HBM peak.  The
+// operator is sparse, so MFMA is not used.
 //
 // CG is the Chronopoulos-Gear single-reduction form of preconditioned CG: per
 // iteration one SpMV kernel (w = A u, with the partials of (w,u)) and one fused
@@ -372,18 +371,17 @@ int grid_for(int64_t work, int per) {
   return (int)g;
 }
 
-// Lanes per row.  32 lanes (two rows per wavefront) when every row fits in
-// half a wavefront.  FP32 rows of up to 64 blocks also take 32 lanes, with two
-// chunks per pass: its bytes per row are half FP64's, so the row's dependent
-// loads bound it, and two rows in flight per wavefront pay (SC 40^3: 22.9 vs
-// 30.0 us per SpMV); FP64 rows beyond 32 blocks keep one row of 64 lanes per
-// wavefront (48.2 vs 57.0 us).
-// Rows of 33-48 blocks (BCC: 41) take 16 lanes, four rows per wavefront, in
-// both precisions: 64 lanes would leave a third idle (BCC J2 bar FP64: 58.4 vs
-// 68.7 us; the 1.0M polycrystal: 518 vs 641 us FP64, 274 vs 294 us FP32).
+// Lanes per row.  Rows of up to 64 blocks (every lattice here: hexagonal 2D
+// 31, BCC 41, SC and FCC 61) take 16 lanes, four rows per wavefront with up to
+// four 16-block chunks each in flight, in both precisions: a row's dependent
+// loads (row start, block column, x) bound one row per wavefront, and 16 lanes
+// keep four rows' bytes in flight (back to back on SC 40^3: FP64 42.2 us with
+// 16 lanes, 47.2 us with 64, 57.0 us with 32; the BCC J2 bar: 58.4 vs 68.7 us
+// with 64; the 250k CT plate: 43.9 vs 50.4 us with 32).  Longer rows take 64
+// lanes and as many passes as they need.
 int lanes_per_row(const Context* c, bool fp32) {
-  if (c->max_nconn > 32 && c->max_nconn <= 48) return 16;
-  return c->max_nconn <= (fp32 ? 64 : 32) ? 32 : 64;
+  (void)fp32;
+  return c->max_nconn <= 64 ? 16 : 64;
 }
 
 int spmv_grid(const Context* c, bool fp32) {
