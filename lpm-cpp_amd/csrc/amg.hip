@@ -309,7 +309,7 @@ __global__ void __launch_bounds__(kSpBS) pcg_spmv_kernel(const int* __restrict__
   if (sc->done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // read by the next kernels
   Vec<1> a;
-  a.v[0] = bsr_rows<D, T, LPR, true>(blockIdx.x, cptr, conns, val, p, w, N);
+  a.v[0] = bsr_rows<D, T, LPR, 64 / LPR, true>(blockIdx.x, cptr, conns, val, p, w, N);
   write_partial<kSpBS, 1>(part, a);
 }
 

@@ -16,74 +16,77 @@ constexpr int kSpBS = 512;
 constexpr int kMaxGrid = 1024;   // workgroups of the grid-strided vector kernels
 constexpr int kMaxParts = 8192;  // SpMV partials summed directly by the consumer
 
-// One chunk of LPR consecutive blocks of a row: this lane's block e0 + sl
-// (clamped to the row's last block when past the end; `ok` says whether it counts).
-template <int D, typename T, int LPR>
-struct BsrChunk {
-  int col;
-  bool ok;
-  T v[D * D];
-  __device__ __forceinline__ void load(int e0, int sl, int hi, const int* __restrict__ conns,
-                                       const T* __restrict__ val) {
-    const int e = e0 + sl;
-    ok = e < hi;
-    const int ec = ok ? e : hi - 1;  // every row holds its self block, so hi - 1 >= lo
-    col = conns[ec];
-    const T* vb = val + (size_t)ec * D * D;
-#pragma unroll
-    for (int k = 0; k < D * D; ++k) v[k] = vb[k];
-  }
-  __device__ __forceinline__ void apply(const T* __restrict__ x, T* acc) const {
-    T xv[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) xv[k] = x[(size_t)col * D + k];
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      T t = v[q * D] * xv[0];
-#pragma unroll
-      for (int k = 1; k < D; ++k) t += v[q * D + k] * xv[k];
-      acc[q] += ok ? t : T(0);
-    }
-  }
-};
+// Lanes per block row of the SpMV and CG kernels: four rows per wavefront.
+constexpr int kLPR = 16;
 
 // y = A x for the rows of wavefronts of workgroup `blk` (64 / LPR rows per
-// wavefront, LPR lanes each).  With DOT, returns this lane's share of x.y
-// (nonzero only in the row's first b lanes).
+// wavefront, LPR lanes each, U chunks of LPR blocks per pass over a row).
+// With DOT, returns this lane's share of x.y (nonzero only in the row's first
+// b lanes).
 //
-// A row's pass is latency-bound: row start -> column index -> x gather are
-// three dependent loads.  With LPR < 64 lanes per row a pass covers 64 / LPR
-// chunks (64 blocks) and issues every chunk's loads before the first chunk's x
-// gather waits, so 64 / LPR rows per wavefront keep that many times the bytes
-// in flight; a chunk that no row of the wavefront reaches is skipped.
-template <int D, typename T, int LPR, bool DOT>
+// A row's pass is latency-bound: row start -> block columns -> x gathers are
+// three dependent loads.  A pass issues the block columns of all U chunks,
+// then all their x gathers and the values of every chunk that some row of the
+// wavefront reaches, and waits once before the products: one latency for the
+// gathers of the whole pass.  With LPR < 64, 64 / LPR rows per wavefront keep
+// that many times the bytes in flight.
+template <int D, typename T, int LPR, int U, bool DOT>
 __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            T* __restrict__ y, int N) {
   constexpr int RPW = 64 / LPR;
-  constexpr int U = 64 / LPR;
   const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
   const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
   const int row = wave * RPW + sub;
   const bool live = row < N;
   const int rw = live ? row : N - 1;  // dead rows shadow the last row: all lanes stay in the shuffles
   const int lo = cptr[rw], hi = cptr[rw + 1];
+  // With DOT: this lane's own entry of x for the dot product, loaded with the
+  // row's extent.
+  const T xo = DOT ? x[(size_t)rw * D + min(sl, D - 1)] : T(0);
   T acc[D];
 #pragma unroll
   for (int q = 0; q < D; ++q) acc[q] = T(0);
   for (int e0 = lo; e0 < hi; e0 += U * LPR) {
-    BsrChunk<D, T, LPR> ch[U];
-    ch[0].load(e0, sl, hi, conns, val);
-    int nc = 1;
+    // Block columns of every chunk (unconditional; past the row's end they
+    // repeat its last block and `ok` drops the product), the x gathers they
+    // address, then the values of the chunks some row of the wavefront reaches.
+    int col[U];
+    bool ok[U];
+    int ec[U];
 #pragma unroll
-    for (int c = 1; c < U; ++c)
-      if (__any(e0 + c * LPR < hi)) {  // wavefront-uniform
-        ch[c].load(e0 + c * LPR, sl, hi, conns, val);
-        nc = c + 1;
-      }
+    for (int c = 0; c < U; ++c) {
+      const int e = e0 + c * LPR + sl;
+      ok[c] = e < hi;
+      ec[c] = ok[c] ? e : hi - 1;
+      col[c] = conns[ec[c]];
+    }
+    T xv[U][D];
 #pragma unroll
     for (int c = 0; c < U; ++c)
-      if (c < nc) ch[c].apply(x, acc);
+#pragma unroll
+      for (int k = 0; k < D; ++k) xv[c][k] = x[(size_t)col[c] * D + k];
+    T v[U][D * D];
+#pragma unroll
+    for (int c = 0; c < U; ++c) {
+      if (c == 0 || __any(e0 + c * LPR < hi)) {  // wavefront-uniform
+        const T* vb = val + (size_t)ec[c] * D * D;
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) v[c][k] = vb[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < D * D; ++k) v[c][k] = T(0);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < U; ++c)
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        T t = v[c][q * D] * xv[c][0];
+#pragma unroll
+        for (int k = 1; k < D; ++k) t += v[c][q * D + k] * xv[c][k];
+        acc[q] += ok[c] ? t : T(0);
+      }
   }
 #pragma unroll
   for (int o = LPR / 2; o > 0; o >>= 1)
@@ -96,7 +99,7 @@ __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr
     for (int q = 1; q < D; ++q)
       if (sl == q) yv = acc[q];
     y[(size_t)row * D + sl] = yv;
-    if (DOT) dot = (double)x[(size_t)row * D + sl] * (double)yv;
+    if (DOT) dot = (double)xo * (double)yv;
   }
   return dot;
 }

@@ -65,7 +65,7 @@ struct CgSys {
   int64_t n;                      // dim * N unknowns
   int N, gs, gv, nS, two_level;   // block rows, SpMV / vector workgroups, SpMV partials
   int sp_begin, red_begin, vec_begin;  // first workgroup in each fused grid
-  int cls;                        // batch class: cg_class(dim, fp32, lanes per row)
+  int cls;                        // batch class: cg_class(dim, fp32, chunks per pass)
 };
 
 // Live size of one batch class, read by the fused kernels from device memory
@@ -255,7 +255,7 @@ template <int D, typename T>
 int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 void amg_free(Context* c);
 int amg_levels(Context* c, int* out, int cap);
-int cg_class(int dim, int fp32, int lpr);
+int cg_class(int dim, int fp32, int chunks);
 void cg_batch_layout(CgSys* tab, int nsys, CgBatchHdr* h);
 int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int nclass);
 int cg_batch_flags(hipStream_t st, const CgSys* tab, int nsys, int* out);

@@ -49,11 +49,11 @@ namespace {
 // workgroups cost less than many small ones)
 constexpr int kVBS = 1024;
 
-template <int D, typename T, int LPR>
+template <int D, typename T, int U>
 __global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
                                                      const T* __restrict__ val, const T* __restrict__ x,
                                                      T* __restrict__ y, int N) {
-  bsr_rows<D, T, LPR, false>(blockIdx.x, cptr, conns, val, x, y, N);
+  bsr_rows<D, T, kLPR, U, false>(blockIdx.x, cptr, conns, val, x, y, N);
 }
 
 // ---- one CG iteration, written for workgroup `blk` of `nblk` of one system.
@@ -63,14 +63,14 @@ __global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cpt
 
 // SpMV: w = A u and the per-workgroup partials of (w, u).  `done` is written
 // only by the vector step, never during this launch.
-template <int D, typename T, int LPR>
+template <int D, typename T, int U>
 __device__ __forceinline__ void cg_spmv_body(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                              const T* __restrict__ val, const T* __restrict__ u,
                                              T* __restrict__ w, int N, double* __restrict__ partS, Scalars* sc) {
   if (sc->done) return;
   if (blk == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // iteration index of the next vector step
   Vec<1> a;
-  a.v[0] = bsr_rows<D, T, LPR, true>(blk, cptr, conns, val, u, w, N);
+  a.v[0] = bsr_rows<D, T, kLPR, U, true>(blk, cptr, conns, val, u, w, N);
   write_partial_at<kSpBS, 1>(partS, blk, a);
 }
 
@@ -237,12 +237,12 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
 }
 
 // ---- per-system launches
-template <int D, typename T, int LPR>
+template <int D, typename T, int U>
 __global__ void __launch_bounds__(kSpBS) cg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
                                                         const T* __restrict__ val, const T* __restrict__ u,
                                                         T* __restrict__ w, int N, double* __restrict__ partS,
                                                         Scalars* sc) {
-  cg_spmv_body<D, T, LPR>(blockIdx.x, cptr, conns, val, u, w, N, partS, sc);
+  cg_spmv_body<D, T, U>(blockIdx.x, cptr, conns, val, u, w, N, partS, sc);
 }
 
 __global__ void __launch_bounds__(kBS) reduce_partials_kernel(const double* __restrict__ in, int n,
@@ -277,12 +277,12 @@ __device__ __forceinline__ int cg_sys_of(const CgSys* __restrict__ tab, int nsys
 
 // The launches are sized for a bucket >= the live workgroup counts in `hdr`;
 // workgroups past them return before any barrier.
-template <int D, typename T, int LPR>
+template <int D, typename T, int U>
 __global__ void __launch_bounds__(kSpBS) fused_cg_spmv_kernel(const CgBatchHdr* __restrict__ hdr,
                                                               const CgSys* __restrict__ tab) {
   if ((int)blockIdx.x >= hdr->sp_blocks) return;
   const CgSys& q = tab[cg_sys_of<0>(tab, hdr->nsys, blockIdx.x)];
-  cg_spmv_body<D, T, LPR>(blockIdx.x - q.sp_begin, q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
+  cg_spmv_body<D, T, U>(blockIdx.x - q.sp_begin, q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
                           q.partS_raw, q.sc);
 }
 
@@ -371,60 +371,53 @@ int grid_for(int64_t work, int per) {
   return (int)g;
 }
 
-// Lanes per row.  Rows of up to 64 blocks (every lattice here: hexagonal 2D
-// 31, BCC 41, SC and FCC 61) take 16 lanes, four rows per wavefront with up to
-// four 16-block chunks each in flight, in both precisions: a row's dependent
-// loads (row start, block column, x) bound one row per wavefront, and 16 lanes
-// keep four rows' bytes in flight (back to back on SC 40^3: FP64 42.2 us with
-// 16 lanes, 47.2 us with 64, 57.0 us with 32; the BCC J2 bar: 58.4 vs 68.7 us
-// with 64; the 250k CT plate: 43.9 vs 50.4 us with 32).  Longer rows take 64
-// lanes and as many passes as they need.
-int lanes_per_row(const Context* c, bool fp32) {
-  (void)fp32;
-  return c->max_nconn <= 64 ? 16 : 64;
+// Chunks of 16 blocks per pass over a row: enough for the longest row (2 for
+// the hexagonal 2D lattice's 31 blocks, 3 for BCC's 41, 4 for SC's and FCC's
+// 61); longer rows take several passes.  A row takes 16 lanes, four rows per
+// wavefront, in both precisions: a row's dependent loads (row start, block
+// columns, x) bound one row per wavefront, and 16 lanes keep four rows' bytes
+// in flight (back to back on SC 40^3: FP64 42.2 us with 16 lanes, 47.2 us with
+// 64, 57.0 us with 32; the BCC J2 bar: 58.4 vs 68.7 us with 64; the 250k CT
+// plate: 43.9 vs 50.4 us with 32).  Chunks past every row of a wavefront cost
+// issue slots, so the pass is sized to the system's longest row.
+int chunks_per_pass(const Context* c) {
+  return std::min(4, std::max(2, (c->max_nconn + kLPR - 1) / kLPR));
 }
 
-int spmv_grid(const Context* c, bool fp32) {
-  const int rows_per_block = (kSpBS / 64) * (64 / lanes_per_row(c, fp32));
+int spmv_grid(const Context* c) {
+  const int rows_per_block = (kSpBS / 64) * (64 / kLPR);
   return (c->N + rows_per_block - 1) / rows_per_block;
 }
 
 template <int D, typename T>
 void launch_bsr(Context* c, const T* val, const T* x, T* y) {
-  const int g = spmv_grid(c, sizeof(T) == 4);
-  const int lpr = lanes_per_row(c, sizeof(T) == 4);
-  if (lpr == 16)
-    spmv_kernel<D, T, 16><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
-  else if (lpr == 32)
-    spmv_kernel<D, T, 32><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
-  else
-    spmv_kernel<D, T, 64><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N);
+  const int g = spmv_grid(c);
+  switch (chunks_per_pass(c)) {
+    case 2: spmv_kernel<D, T, 2><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N); break;
+    case 3: spmv_kernel<D, T, 3><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N); break;
+    default: spmv_kernel<D, T, 4><<<g, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val, x, y, c->N); break;
+  }
 }
 
 // With events, the kernel's own start and end are recorded with its dispatch
 // (hipExtLaunchKernelGGL): the interval is the kernel, without the gaps that
 // separately enqueued event records add.
+template <int D, typename T, int U>
+void launch_cg_spmv_u(hipStream_t st, const CgSys& q, hipEvent_t e0, hipEvent_t e1) {
+  if (e0)  // instrumented (eager) launches only; captured chunks use plain launches
+    hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, U>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
+                          (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
+  else
+    cg_spmv_kernel<D, T, U><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
+                                                    q.partS_raw, q.sc);
+}
+
 template <int D, typename T>
-void launch_cg_spmv(hipStream_t st, const CgSys& q, int lpr, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-  if (e0) {  // instrumented (eager) launches only; captured chunks use plain launches
-    if (lpr == 16)
-      hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 16>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
-                            (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
-    else if (lpr == 32)
-      hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 32>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
-                            (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
-    else
-      hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, 64>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
-                            (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
-  } else if (lpr == 16) {
-    cg_spmv_kernel<D, T, 16><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
-                                                     q.partS_raw, q.sc);
-  } else if (lpr == 32) {
-    cg_spmv_kernel<D, T, 32><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
-                                                     q.partS_raw, q.sc);
-  } else {
-    cg_spmv_kernel<D, T, 64><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
-                                                     q.partS_raw, q.sc);
+void launch_cg_spmv(hipStream_t st, const CgSys& q, int chunks, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+  switch (chunks) {
+    case 2: launch_cg_spmv_u<D, T, 2>(st, q, e0, e1); break;
+    case 3: launch_cg_spmv_u<D, T, 3>(st, q, e0, e1); break;
+    default: launch_cg_spmv_u<D, T, 4>(st, q, e0, e1); break;
   }
 }
 
@@ -453,7 +446,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
   q->N = c->N;
   q->n = (int64_t)D * c->N;
   q->gv = grid_for(q->n, kVBS);
-  q->gs = spmv_grid(c, sizeof(T) == 4);
+  q->gs = spmv_grid(c);
   q->two_level = q->gs > kMaxParts;
   q->nS = q->two_level ? kBS : q->gs;
   q->partV0 = c->partials;
@@ -461,7 +454,7 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
   q->partS_raw = c->partials + 4 * kMaxGrid;
   q->partS = q->two_level ? q->partS_raw + q->gs : q->partS_raw;
   q->sc = c->sc;
-  q->cls = cg_class(D, sizeof(T) == 4, lanes_per_row(c, sizeof(T) == 4));
+  q->cls = cg_class(D, sizeof(T) == 4, chunks_per_pass(c));
   dinv_kernel<D, T><<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->dpos, (const T*)q->val, c->N, o.precond,
                                                               (T*)q->dinv);
   const int64_t maxit = o.maxit > 0 ? o.maxit : q->n;
@@ -477,14 +470,14 @@ int cg_begin(Context* c, const SolveOpts& o, CgSys* q) {
 // parity of the vector-step partial buffers is fixed per launch slot.
 template <int D, typename T>
 int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
-  const int lpr = lanes_per_row(c, sizeof(T) == 4);
+  const int chunks = chunks_per_pass(c);
   c->sc_host_iter_launched = 0;
   auto enqueue_chunk = [&](std::vector<hipEvent_t>* evs) -> int {
     for (int k = 0; k < kCgChunk; ++k) {
       if (evs)
-        launch_cg_spmv<D, T>(c->stream, q, lpr, (*evs)[2 * k], (*evs)[2 * k + 1]);
+        launch_cg_spmv<D, T>(c->stream, q, chunks, (*evs)[2 * k], (*evs)[2 * k + 1]);
       else
-        launch_cg_spmv<D, T>(c->stream, q, lpr);
+        launch_cg_spmv<D, T>(c->stream, q, chunks);
       if (q.two_level) reduce_partials_kernel<<<kBS, kBS, 0, c->stream>>>(q.partS_raw, q.gs, q.partS, q.sc);
       cg_vec_kernel<T><<<q.gv, kVBS, 0, c->stream>>>((T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
                                                     (const T*)q.dinv, q.n, (k & 1) ? q.partV1 : q.partV0, q.gv,
@@ -540,18 +533,18 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
   return 0;
 }
 
-template <int D, typename T, int LPR>
+template <int D, typename T, int U>
 void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
   const CgSys* tab = reinterpret_cast<const CgSys*>(reinterpret_cast<const char*>(b.hdr) + kCgHdrBytes);
-  fused_cg_spmv_kernel<D, T, LPR><<<b.sp_grid, kSpBS, 0, st>>>(b.hdr, tab);
+  fused_cg_spmv_kernel<D, T, U><<<b.sp_grid, kSpBS, 0, st>>>(b.hdr, tab);
   if (b.red_grid) fused_reduce_partials_kernel<<<b.red_grid, kBS, 0, st>>>(b.hdr, tab);
   fused_cg_vec_kernel<T><<<b.vec_grid, kVBS, 0, st>>>(b.hdr, tab, k & 1);
 }
 
 }  // namespace
 
-int cg_class(int dim, int fp32, int lpr) {
-  return (dim == 3 ? 6 : 0) + (fp32 ? 3 : 0) + (lpr == 64 ? 2 : lpr == 32 ? 1 : 0);
+int cg_class(int dim, int fp32, int chunks) {
+  return (dim == 3 ? 6 : 0) + (fp32 ? 3 : 0) + (chunks - 2);
 }
 
 // Workgroup offsets of each system in the fused grids; the live totals.
@@ -577,18 +570,18 @@ int cg_batch_enqueue_chunk(hipStream_t st, const CgBatchClass* classes, int ncla
       const CgBatchClass& b = classes[c];
       if (b.sp_grid == 0) continue;
       switch (b.cls) {
-        case 0: launch_fused_class<2, double, 16>(st, b, k); break;
-        case 1: launch_fused_class<2, double, 32>(st, b, k); break;
-        case 2: launch_fused_class<2, double, 64>(st, b, k); break;
-        case 3: launch_fused_class<2, float, 16>(st, b, k); break;
-        case 4: launch_fused_class<2, float, 32>(st, b, k); break;
-        case 5: launch_fused_class<2, float, 64>(st, b, k); break;
-        case 6: launch_fused_class<3, double, 16>(st, b, k); break;
-        case 7: launch_fused_class<3, double, 32>(st, b, k); break;
-        case 8: launch_fused_class<3, double, 64>(st, b, k); break;
-        case 9: launch_fused_class<3, float, 16>(st, b, k); break;
-        case 10: launch_fused_class<3, float, 32>(st, b, k); break;
-        case 11: launch_fused_class<3, float, 64>(st, b, k); break;
+        case 0: launch_fused_class<2, double, 2>(st, b, k); break;
+        case 1: launch_fused_class<2, double, 3>(st, b, k); break;
+        case 2: launch_fused_class<2, double, 4>(st, b, k); break;
+        case 3: launch_fused_class<2, float, 2>(st, b, k); break;
+        case 4: launch_fused_class<2, float, 3>(st, b, k); break;
+        case 5: launch_fused_class<2, float, 4>(st, b, k); break;
+        case 6: launch_fused_class<3, double, 2>(st, b, k); break;
+        case 7: launch_fused_class<3, double, 3>(st, b, k); break;
+        case 8: launch_fused_class<3, double, 4>(st, b, k); break;
+        case 9: launch_fused_class<3, float, 2>(st, b, k); break;
+        case 10: launch_fused_class<3, float, 3>(st, b, k); break;
+        case 11: launch_fused_class<3, float, 4>(st, b, k); break;
         default: set_error("bad CG batch class"); return -1;
       }
     }
