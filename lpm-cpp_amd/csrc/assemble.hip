@@ -9,8 +9,9 @@
 // j.  Every lane walks the same bond paths i->k->m of the row and keeps the
 // terms whose end is its own j, in the reference's loop order, so each lane's
 // K_local(i,j) is the reference's value bit for bit.  Rows of up to 64 bonds
-// whose two-hop path data fits in LDS (every lattice here) gather it in two
-// batched rounds (tangent_lds_kernel); otherwise rows of up to 64 bonds keep
+// whose two-hop path data fits in LDS (every lattice here) gather it in
+// batched rounds and rank each path under the lane it ends at
+// (tangent_lds_kernel); otherwise rows of up to 64 bonds keep
 // them in lanes (tangent_lanes_kernel) and longer rows walk them with
 // wave-uniform loads (tangent_kernel).
 // Kernel 2 (symmetrise + Dirichlet): owner-computes symmetrisation with no
@@ -29,8 +30,9 @@ namespace {
 
 constexpr int kWaves = 4;
 // Path entries per wavefront in tangent_lds_kernel's LDS slice (36 bytes each
-// plus 512 bytes of offsets): kWaves * (36 * 416 + 512) B = 61.9 KB
-constexpr int kTangentLdsMax = 416;
+// plus 1.5 KB of per-bond and per-connection tables):
+// kWaves * (36 * 384 + 1536) B = 60 KB per workgroup
+constexpr int kTangentLdsMax = 384;
 
 template <int D>
 __global__ void __launch_bounds__(64 * kWaves)
@@ -320,13 +322,21 @@ tangent_lanes_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
   }
 }
 
-// The tangent with every load of a row issued in two batched rounds: the
-// row's bonds one per lane (as in tangent_lanes_kernel), then all two-hop path
-// data -- for each bond i->k, k's bonds of the same layer (end, direction,
-// 1 - bdamage) -- gathered into this wavefront's slice of LDS, E entries at
-// most.  The path walk then reads LDS (all lanes the same address: a
-// broadcast) instead of waiting on global memory at every step.  Walk order and
-// expressions are those of tangent_kernel, so the bits are the same.
+// The tangent with every load of a row issued in batched rounds and no search
+// in the path walk.  The row's bonds sit one per lane (as in
+// tangent_lanes_kernel) and its connections one per lane (j = conns[lane]).
+// All two-hop path data -- for each bond i->k, k's bonds of the same layer
+// (end m, direction, 1 - bdamage) -- is gathered into this wavefront's slice of
+// LDS, and each path entry finds its owner, the lane whose j is m, by a binary
+// search over the row's sorted connections.  Owners are recorded as one bit per
+// bond t in the owner's mask, and the entries are ranked into per-owner lists in
+// bond order (the rank of entry (t, owner) is the owner's list start plus the
+// bits of its mask below t).  k's bonds end at distinct particles, so a lane
+// has at most one path per bond; the walk then takes the lane's next list entry
+// at each bond whose bit is set, all lanes together, instead of scanning every
+// path end for its own j.  A bond list in which two of k's bonds share an end
+// takes the scan (detected while ranking).  Walk order and expressions are those
+// of tangent_kernel, so the bits are the same.  Rows of up to 64 connections.
 template <int D>
 __global__ void __launch_bounds__(64 * kWaves)
 tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
@@ -338,19 +348,26 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
   extern __shared__ double lds_tangent[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = blockIdx.x * kWaves + w;
-  const bool row = i < N;  // wave-uniform; every wavefront still reaches both barriers
-  double* const Lcs = lds_tangent + (size_t)w * 4 * E;  // [E][3]
-  double* const Lom = Lcs + 3 * E;                       // [E]
-  int* const Lm = (int*)(lds_tangent + (size_t)kWaves * 4 * E) + (size_t)w * (E + 128);
-  int* const Loff = Lm + E;                              // [64] first entry of bond t
-  int* const Lklo = Loff + 64;                           // [64] k's first bond of the layer
-  int c_lo = 0, c_hi = 0, b_lo = 0, n0 = 0, nbi = 0;
+  const bool row = i < N;  // wave-uniform; every wavefront still reaches the barriers
+  double* const Lcs = lds_tangent + (size_t)w * (4 * E + 64);  // [E][3]
+  double* const Lom = Lcs + 3 * E;                              // [E]
+  unsigned long long* const Lmask = reinterpret_cast<unsigned long long*>(Lom + E);  // [64] per owner
+  int* const Loff = (int*)(lds_tangent + (size_t)kWaves * (4 * E + 64)) + (size_t)w * 256;  // [64] first entry of bond t
+  int* const Lklo = Loff + 64;    // [64] k's first bond of the layer
+  int* const Lconn = Lklo + 64;   // [64] the row's connections, padded with INT_MAX
+  int* const Lbase = Lconn + 64;  // [64] first list entry of each owner
+  short* const Lot = (short*)(lds_tangent + (size_t)kWaves * (4 * E + 64) + (size_t)kWaves * 128) +
+                     (size_t)w * 2 * E;  // [E] owner << 8 | bond, -1 without owner
+  short* const Llist = Lot + E;          // [E] entries by owner, in bond order
+  int c_lo = 0, nc = 0, b_lo = 0, n0 = 0, nbi = 0;
   double tv_i = 0;
   if (row) {
-    c_lo = cptr[i]; c_hi = cptr[i + 1];
+    c_lo = cptr[i]; nc = cptr[i + 1] - c_lo;
     b_lo = bptr[i]; n0 = nb0[i]; nbi = bptr[i + 1] - b_lo;
     tv_i = Tv[i];
   }
+  const bool active = lane < nc;
+  const int j = active ? conns[c_lo + lane] : -1;
   // lane t: bond b_lo + t of i and the layer range of its end's bonds
   int bk = 0, klo = 0, nk = 0;
   double bc[3] = {0, 0, 0}, bom = 0, btv = 0;
@@ -371,10 +388,14 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
     Loff[lane] = off;
     Lklo[lane] = klo;
   }
+  Lconn[lane] = active ? j : INT_MAX;
+  Lmask[lane] = 0ull;
   __syncthreads();
-  // gather: entry e = off_t + u is bond klo_t + u; four entries per lane in flight
+  // gather: entry e = off_t + u is bond klo_t + u; four entries per lane in
+  // flight.  Each entry's owner sets bit t of its mask.
+  bool dup = false;
   for (int e0 = lane; e0 < T; e0 += 4 * 64) {
-    int b3[4];
+    int b3[4], t3[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = min(e0 + 64 * q, T - 1);
@@ -384,6 +405,7 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
         if (Loff[mid] <= e) lo = mid;
         else hi = mid - 1;
       }
+      t3[q] = lo;
       b3[q] = Lklo[lo] + (e - Loff[lo]);
     }
     int m[4];
@@ -398,137 +420,184 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
     for (int q = 0; q < 4; ++q) {
       const int e = e0 + 64 * q;
       if (e < T) {
-        Lm[e] = m[q];
         Lcs[3 * e] = c3[q][0]; Lcs[3 * e + 1] = c3[q][1]; Lcs[3 * e + 2] = c3[q][2];
         Lom[e] = om3[q];
+        int pos = 0;  // last connection <= m
+#pragma unroll
+        for (int st = 32; st > 0; st >>= 1)
+          if (Lconn[pos + st] <= m[q]) pos += st;
+        const bool own = Lconn[pos] == m[q];
+        Lot[e] = (short)(own ? (pos << 8 | t3[q]) : -1);
+        if (own) {
+          const unsigned long long bit = 1ull << t3[q];
+          dup = dup || (atomicOr(&Lmask[pos], bit) & bit) != 0;
+        }
       }
     }
   }
   __syncthreads();
+  const bool scan = __any(dup);  // wave-uniform
+  const unsigned long long mine = active ? Lmask[lane] : 0ull;
+  const int cnt = __popcll(mine);
+  const int base = wave_incl_scan(cnt) - cnt;
+  Lbase[lane] = base;
+  __syncthreads();
+  if (!scan)
+    for (int e = lane; e < T; e += 64) {
+      const int v = Lot[e];
+      if (v >= 0) {
+        const int o = v >> 8, t = v & 255;
+        Llist[Lbase[o] + __popcll(Lmask[o] & ((1ull << t) - 1ull))] = (short)e;
+      }
+    }
+  __syncthreads();
   if (!row) return;
-  for (int e0 = c_lo; e0 < c_hi; e0 += 64) {
-    const int e = e0 + lane;
-    const bool active = e < c_hi;
-    const int j = active ? conns[e] : -1;
-    double K[D][D];
+  double K[D][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+  int cursor = base;
+  for (int L = 0; L < 2; ++L) {
+    const int tlo = L == 0 ? 0 : n0, thi = L == 0 ? n0 : nbi;
+    int t1 = -1;
+    for (int t = tlo; t < thi; ++t)
+      if (__shfl(bk, t, 64) == j && t1 < 0) t1 = t;
+    const unsigned long long below_n0 = n0 >= 64 ? ~0ull : (1ull << n0) - 1ull;
+    const unsigned long long layer = L == 0 ? below_n0 : ~below_n0;
+    bool found = t1 >= 0 || (!scan && (mine & layer) != 0ull);
+    const int src1 = t1 >= 0 ? t1 : 0;
+    double c1[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double v = __shfl(bc[q], src1, 64);
+      c1[q] = t1 >= 0 ? v : 0.0;
+    }
+    const double om1v = __shfl(bom, src1, 64);
+    const double om1 = t1 >= 0 ? om1v : 0.0;
+    double Tm[D][D];
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
-      for (int s = 0; s < D; ++s) K[r][s] = 0.0;
-    for (int L = 0; L < 2; ++L) {
-      const int tlo = L == 0 ? 0 : n0, thi = L == 0 ? n0 : nbi;
-      int t1 = -1;
-      for (int t = tlo; t < thi; ++t)
-        if (__shfl(bk, t, 64) == j && t1 < 0) t1 = t;
-      bool found = t1 >= 0;
-      const int src1 = t1 >= 0 ? t1 : 0;
-      double c1[3];
+      for (int s = 0; s < D; ++s) Tm[r][s] = 0.0;
+    const double kn = Kn[2 * i + L];
+    for (int t = tlo; t < thi; ++t) {
+      const int k = __shfl(bk, t, 64);
+      double c2[3];
 #pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const double v = __shfl(bc[q], src1, 64);
-        c1[q] = t1 >= 0 ? v : 0.0;
+      for (int q = 0; q < 3; ++q) c2[q] = __shfl(bc[q], t, 64);
+      const double om2 = __shfl(bom, t, 64);
+      const double tv_k = __shfl(btv, t, 64);
+      int toff = 0, tnk = 0;
+      if (scan) {  // wave-uniform: every lane runs the broadcasts
+        toff = __shfl(off, t, 64);
+        tnk = __shfl(nk, t, 64);
       }
-      const double om1v = __shfl(bom, src1, 64);
-      const double om1 = t1 >= 0 ? om1v : 0.0;
-      double Tm[D][D];
-#pragma unroll
-      for (int r = 0; r < D; ++r)
-#pragma unroll
-        for (int s = 0; s < D; ++s) Tm[r][s] = 0.0;
-      const double kn = Kn[2 * i + L];
-      for (int t = tlo; t < thi; ++t) {
-        const int k = __shfl(bk, t, 64);
-        double c2[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) c2[q] = __shfl(bc[q], t, 64);
-        const double om2 = __shfl(bom, t, 64);
-        const double tv_k = __shfl(btv, t, 64);
-        const int toff = __shfl(off, t, 64), tnk = __shfl(nk, t, 64);
-        const bool direct = t1 >= 0 && k == j;
-        if (direct) {
-          // direct bond term (derivative.h:59-88 / 156-185)
-          const double* csum = cs_sum + (2 * k + L) * 3;
-#pragma unroll
-          for (int r = 0; r < D; ++r)
-#pragma unroll
-            for (int s = 0; s < D; ++s) {
-              const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
-              const double geo = (r == s) ? (kn + tv_i) * (c2[r] * c2[r])
-                                          : (kn + tv_i) * c2[hi_rs] * c2[lo_rs];
-              Tm[r][s] += om2 * ((-kn * c2[s] + tv_i * csum[s]) * c2[r] - geo);
-            }
-        } else {
-          if (t1 >= 0) {
-            // common-neighbour term through i's dilatation (derivative.h:91-99 / 188-196)
-            const double coef = 0.5 * (om1 * tv_i + om2 * tv_i);
-#pragma unroll
-            for (int r = 0; r < D; ++r)
-#pragma unroll
-              for (int s = 0; s < D; ++s) Tm[r][s] -= coef * c2[r] * c1[s];
-          }
-          // two-hop paths i->k->j through k's dilatation (derivative.h:101-115 /
-          // 122-139 and 198-212 / 219-236)
-          // four path ends read per round (reads past the list repeat its last
-          // entry and are skipped), matches added in ascending order
-          for (int u0 = 0; u0 < tnk; u0 += 4) {
-            int mm[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) mm[q] = Lm[toff + min(u0 + q, tnk - 1)];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-              if (u0 + q >= tnk || mm[q] != j) continue;
-              const int qq = toff + u0 + q;
-              found = true;
-              const double coef3 = 0.5 * (Lom[qq] * tv_k + om2 * tv_i);
-              const double c3[3] = {Lcs[3 * qq], Lcs[3 * qq + 1], Lcs[3 * qq + 2]};
-#pragma unroll
-              for (int r = 0; r < D; ++r)
-#pragma unroll
-                for (int s = 0; s < D; ++s) Tm[r][s] -= coef3 * c2[r] * c3[s];
-            }
-          }
-        }
-      }
-      if (found)
+      const bool direct = t1 >= 0 && k == j;
+      const bool path = !scan && ((mine >> t) & 1ull);
+      const int qp = path ? Llist[cursor] : 0;
+      cursor += path ? 1 : 0;
+      if (direct) {
+        // direct bond term (derivative.h:59-88 / 156-185)
+        const double* csum = cs_sum + (2 * k + L) * 3;
 #pragma unroll
         for (int r = 0; r < D; ++r)
 #pragma unroll
-          for (int s = 0; s < D; ++s) K[r][s] = K[r][s] + Tm[r][s];
-    }
-    if (j == i) {
-      // fdu2dxyz (derivative.h:7-45): upper triangle of the diagonal block
-#pragma unroll
-      for (int r = 0; r < D; ++r)
-#pragma unroll
-        for (int s = 0; s < D; ++s) K[r][s] = 0.0;
-      for (int L = 0; L < 2; ++L) {
-        const int lo = L == 0 ? b_lo : b_lo + n0, hi = L == 0 ? b_lo + n0 : b_lo + nbi;
-        const double kn = Kn[2 * i + L];
-        const double* csum = cs_sum + (2 * i + L) * 3;
-        for (int b = lo; b < hi; ++b) {
-          const double om = 1 - bdamage[b];
-          const double c[3] = {cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+          for (int s = 0; s < D; ++s) {
+            const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
+            const double geo = (r == s) ? (kn + tv_i) * (c2[r] * c2[r])
+                                        : (kn + tv_i) * c2[hi_rs] * c2[lo_rs];
+            Tm[r][s] += om2 * ((-kn * c2[s] + tv_i * csum[s]) * c2[r] - geo);
+          }
+      } else {
+        if (t1 >= 0) {
+          // common-neighbour term through i's dilatation (derivative.h:91-99 / 188-196)
+          const double coef = 0.5 * (om1 * tv_i + om2 * tv_i);
 #pragma unroll
           for (int r = 0; r < D; ++r)
 #pragma unroll
-            for (int s = r; s < D; ++s) {
-              const double geo = (r == s) ? (kn + tv_i) * (c[r] * c[r]) : (kn + tv_i) * c[s] * c[r];
-              K[r][s] += om * ((kn * c[s] + tv_i * csum[s]) * c[r] + geo);
-            }
+            for (int s = 0; s < D; ++s) Tm[r][s] -= coef * c2[r] * c1[s];
+        }
+        // two-hop paths i->k->j through k's dilatation (derivative.h:101-115 /
+        // 122-139 and 198-212 / 219-236)
+        if (path) {
+          const double coef3 = 0.5 * (Lom[qp] * tv_k + om2 * tv_i);
+          const double c3[3] = {Lcs[3 * qp], Lcs[3 * qp + 1], Lcs[3 * qp + 2]};
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) Tm[r][s] -= coef3 * c2[r] * c3[s];
+        }
+        if (scan) {  // every path end of bond t, matches in ascending order
+          for (int u = 0; u < tnk; ++u) {
+            const int qq = toff + u;
+            const int v = Lot[qq];  // the owner is the lane whose j is the path's end
+            if (v < 0 || (v >> 8) != lane) continue;
+            found = true;
+            const double coef3 = 0.5 * (Lom[qq] * tv_k + om2 * tv_i);
+            const double c3[3] = {Lcs[3 * qq], Lcs[3 * qq + 1], Lcs[3 * qq + 2]};
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+              for (int s = 0; s < D; ++s) Tm[r][s] -= coef3 * c2[r] * c3[s];
+          }
         }
       }
+    }
+    if (found)
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) K[r][s] = K[r][s] + Tm[r][s];
+  }
+  {
+    // fdu2dxyz (derivative.h:7-45): upper triangle of the diagonal block.
+    // Lane t forms bond t's term from the bond it holds; the sums over the
+    // bonds, in bond order, are formed in every lane and kept by lane j == i.
+    double G[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) G[r][s] = 0.0;
+    if (lane < nbi) {
+      const int L = lane < n0 ? 0 : 1;
+      const double kn = Kn[2 * i + L];
+      const double* csum = cs_sum + (2 * i + L) * 3;
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = r; s < D; ++s) {
+          const double geo = (r == s) ? (kn + tv_i) * (bc[r] * bc[r]) : (kn + tv_i) * bc[s] * bc[r];
+          G[r][s] = bom * ((kn * bc[s] + tv_i * csum[s]) * bc[r] + geo);
+        }
+    }
+    double Kd[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) Kd[r][s] = 0.0;
+    for (int t = 0; t < nbi; ++t)
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = r; s < D; ++s) Kd[r][s] += __shfl(G[r][s], t, 64);
+    if (j == i) {
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) K[r][s] = Kd[r][s];
       double dsq = 0;
 #pragma unroll
       for (int r = 0; r < D; ++r) dsq += K[r][r] * K[r][r];
       diag_sq[i] = dsq;
     }
-    if (active) {
-      double* out = H + (size_t)e * D * D;
+  }
+  if (active) {
+    double* out = H + (size_t)(c_lo + lane) * D * D;
 #pragma unroll
-      for (int r = 0; r < D; ++r)
+    for (int r = 0; r < D; ++r)
 #pragma unroll
-        for (int s = 0; s < D; ++s) out[r * D + s] = K[r][s];
-    }
+      for (int s = 0; s < D; ++s) out[r * D + s] = K[r][s];
   }
 }
 
@@ -656,8 +725,11 @@ int launch_assemble(Context* c, int want32) {
     if (int rc = update_floating(c)) return rc;
   float* k32 = want32 ? c->Kval32 : nullptr;
   // LDS-staged tangent when a row's two-hop path data fits 64 KB per workgroup
-  const int lds_E = (c->max_nb <= 64 && c->max_cand <= kTangentLdsMax) ? std::max(32, (c->max_cand + 31) / 32 * 32) : 0;
-  const size_t lds_bytes = (size_t)kWaves * ((size_t)4 * lds_E * sizeof(double) + (size_t)(lds_E + 128) * sizeof(int));
+  // and its connections one wavefront
+  const int lds_E = (c->max_nb <= 64 && c->max_nconn <= 64 && c->max_cand <= kTangentLdsMax)
+                        ? std::max(32, (c->max_cand + 31) / 32 * 32) : 0;
+  const size_t lds_bytes = (size_t)kWaves * ((size_t)(4 * lds_E + 64) * sizeof(double) + 256 * sizeof(int) +
+                                             (size_t)2 * lds_E * sizeof(short));
   if (c->dim == 3) {
     if (lds_E > 0)
       tangent_lds_kernel<3><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,

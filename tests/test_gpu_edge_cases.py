@@ -4,6 +4,7 @@
   a bond list), so rows of more than 64 bonds and more than 64 connections --
   the SpMV's multi-pass rows in FP64 and FP32 and the tangent's path-walking
   kernel;
+* a bond listed twice in a bond list (the tangent's scan over path ends);
 * the smallest systems: two particles and one bond; a particle without bonds
   is an input error (the reference's stiffness has no diagonal for it), reported
   by the creation call;
@@ -99,6 +100,40 @@ def test_rows_beyond_one_wavefront_match_oracle():
     g.solve_linear(**GPU_OPTS)
     o.solve_linear(**ORC_OPTS)
     assert _relerr(g.get("XYZ") - x0, o.get("XYZ") - x0) < 1e-10
+
+
+def test_repeated_bond_in_a_bond_list_matches_oracle():
+    """A bond file may list a bond twice (Assembly::readBond keeps both): two of
+    a particle's bonds then end at the same particle, and the tangent walks
+    those rows' paths one by one instead of through its per-connection lists."""
+    xyz = lat.sc_block(4, 4, 4, R)
+    o = Oracle(lat.SC3D, R, lat.ELASTIC, xyz)
+    bptr, nb0, p2 = o.bonds()
+    n = len(xyz)
+    p1 = np.repeat(np.arange(n), np.diff(bptr))
+    layer = np.concatenate([np.r_[np.zeros(nb0[i]), np.ones(bptr[i + 1] - bptr[i] - nb0[i])]
+                            for i in range(n)])
+    c = int(np.argmin(np.linalg.norm(xyz - xyz.mean(0), axis=1)))
+    b = bptr[c] + nb0[c]  # c's first layer-1 bond, listed again in both directions
+    k = int(p2[b])
+    bonds = (np.r_[p1, c, k].astype(np.int32), np.r_[p2, k, c].astype(np.int32),
+             np.r_[layer, 1, 1].astype(np.int32))
+    g = lpmg.System(lat.SC3D, R, lat.ELASTIC, xyz, bonds=bonds)
+    o = Oracle(lat.SC3D, R, lat.ELASTIC, xyz, bonds=bonds)
+    for a, b_ in zip(g.bonds(), o.bonds()):
+        np.testing.assert_array_equal(a, b_)
+    rng = np.random.default_rng(5)
+    x = xyz + 5e-4 * R * rng.standard_normal(xyz.shape)
+    for s in (g, o):
+        s.set_material(*MAT)
+        s.set("XYZ", x)
+        s.forward()
+    g.assemble()
+    o.assemble()
+    for a, b_ in zip(g.csr()[:3], o.csr()[:3]):
+        np.testing.assert_array_equal(a, b_)
+    Kg, Ko = g.csr()[3], o.csr()[3]
+    assert np.abs(Kg - Ko).max() <= 1e-13 * np.abs(Ko).max()
 
 
 def test_particle_without_bonds_is_an_input_error():
