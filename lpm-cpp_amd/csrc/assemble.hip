@@ -616,26 +616,39 @@ __global__ void __launch_bounds__(256) norm_diag_finish_kernel(const double* __r
   if (threadIdx.x == 0) sc->norm_diag = sqrt(tot.v[0]);
 }
 
+// Bit k of fixm[i]: DOF k of particle i is eliminated (dof_fixed).
 template <int D>
 __global__ void __launch_bounds__(256)
-symmetrize_bc_kernel(const int* __restrict__ brow, const int* __restrict__ conns,
-                     const int* __restrict__ tpos, int64_t nnzb, const int* __restrict__ cons,
-                     const double* __restrict__ dvis, const int* __restrict__ floating,
-                     const Scalars* __restrict__ sc,
-                     double* __restrict__ Kval, float* __restrict__ Kval32) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nnzb) return;
+fixed_dofs_kernel(const int* __restrict__ cons, const double* __restrict__ dvis, const int* __restrict__ floating,
+                  int N, int* __restrict__ fixm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  int m = 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) m |= dof_fixed(cons, dvis, floating, i, k) ? 1 << k : 0;
+  fixm[i] = m;
+}
+
+// One thread per block (i,j), j >= i, of the upper list.
+template <int D>
+__global__ void __launch_bounds__(256)
+symmetrize_bc_kernel(const int* __restrict__ uidx, int nup, const int* __restrict__ brow,
+                     const int* __restrict__ conns, const int* __restrict__ tpos, const int* __restrict__ fixm,
+                     const Scalars* __restrict__ sc, double* __restrict__ Kval, float* __restrict__ Kval32) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nup) return;
+  const int e = uidx[u];
   const int i = brow[e], j = conns[e];
-  if (i > j) return;
-  const double nd = sc->norm_diag;
+  const int mi = fixm[i], mj = fixm[j];
   bool fi[D], fj[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    fi[k] = dof_fixed(cons, dvis, floating, i, k);
-    fj[k] = dof_fixed(cons, dvis, floating, j, k);
+    fi[k] = (mi >> k) & 1;
+    fj[k] = (mj >> k) & 1;
   }
   double* Ke = Kval + (size_t)e * D * D;
   if (i == j) {
+    const double nd = sc->norm_diag;
     double v[D][D];
 #pragma unroll
     for (int r = 0; r < D; ++r)
@@ -715,7 +728,7 @@ int launch_assemble(Context* c, int want32) {
   const int nbw = (N + kWaves - 1) / kWaves;
   const int tb = 256;
   const int nbn = (N + tb - 1) / tb;
-  const int64_t nbe = (c->nnzb + tb - 1) / tb;
+  const int nbu = (c->nup + tb - 1) / tb;
   if (nbn > c->n_partials) {
     set_error("partials buffer too small");
     return -1;
@@ -745,8 +758,9 @@ int launch_assemble(Context* c, int want32) {
                                                            N, c->Kval, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
-    symmetrize_bc_kernel<3><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
-                                                      c->damage_visual, c->floating, c->sc, c->Kval, k32);
+    fixed_dofs_kernel<3><<<nbn, tb, 0, c->stream>>>(c->cons, c->damage_visual, c->floating, N, c->fixm);
+    symmetrize_bc_kernel<3><<<nbu, tb, 0, c->stream>>>(c->uidx, c->nup, c->brow, c->conns, c->tpos, c->fixm,
+                                                      c->sc, c->Kval, k32);
   } else {
     if (lds_E > 0)
       tangent_lds_kernel<2><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
@@ -762,8 +776,9 @@ int launch_assemble(Context* c, int want32) {
                                                            N, c->Kval, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
-    symmetrize_bc_kernel<2><<<nbe, tb, 0, c->stream>>>(c->brow, c->conns, c->tpos, c->nnzb, c->cons,
-                                                      c->damage_visual, c->floating, c->sc, c->Kval, k32);
+    fixed_dofs_kernel<2><<<nbn, tb, 0, c->stream>>>(c->cons, c->damage_visual, c->floating, N, c->fixm);
+    symmetrize_bc_kernel<2><<<nbu, tb, 0, c->stream>>>(c->uidx, c->nup, c->brow, c->conns, c->tpos, c->fixm,
+                                                      c->sc, c->Kval, k32);
   }
   LPMG_HIP(hipGetLastError());
   c->assembled64 = true;
