@@ -9,6 +9,10 @@
 // Solver::updateDisplacementBC / updateForceBC (solver.h:128-188),
 // Solver::solveLinearSystem's moveBy (solver.h:100-108).
 //
+// Rows of up to 32 bonds take 32 lanes per particle (geometry_force_lanes_kernel,
+// particle_force_lanes_kernel); longer rows one thread per particle.  Either
+// way the sums of a particle are formed in the reference's bond order.
+//
 // One thread owns one particle and walks its bonds in the reference's order, so
 // the per-layer dilatation sums, the forces and the stresses are summed in the
 // same order as the reference's serial loops (bit-identical FP64, the build uses
@@ -120,6 +124,133 @@ particle_force_kernel(const int* __restrict__ bptr, const int* __restrict__ rev,
   damage_visual[i] = dv / (double)(hi - lo);
 }
 
+// The same two kernels for particles of at most 32 bonds (every lattice here):
+// 32 lanes per particle, one per bond, so the bond arrays are read and written
+// in contiguous runs instead of one strided run per thread.  Each bond lane
+// forms its terms; the per-particle sums are then formed by one lane per sum,
+// reading the terms back from LDS in bond order -- the reference's summation
+// order, so the bits equal the thread-per-particle kernels'.  LDS is written
+// and read by the lanes of one wavefront only.
+constexpr int kFP = 32;  // lanes per particle
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void __launch_bounds__(256)
+geometry_force_lanes_kernel(const double4* __restrict__ xyz, const int* __restrict__ bptr,
+                            const int* __restrict__ nb0, const int* __restrict__ p2,
+                            const double* __restrict__ dis0, const double* __restrict__ dLp,
+                            double* __restrict__ bdamage, const double* __restrict__ Kn,
+                            const double* __restrict__ Tv, const double* __restrict__ damage, int ptype,
+                            int N, double* __restrict__ dis, double* __restrict__ bstrain,
+                            double* __restrict__ dLe, double* __restrict__ cs, double* __restrict__ bforce,
+                            double* __restrict__ dLe_tot, double* __restrict__ TdLe_tot,
+                            double* __restrict__ cs_sum) {
+  __shared__ double terms[256 / 64][64][5];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, base = lane & ~(kFP - 1), sl = lane & (kFP - 1);
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kFP;
+  const bool live = i < N;
+  const int ii = live ? i : N - 1;
+  const double4 xi = xyz[ii];
+  const double tv = Tv[ii];
+  const int b_lo = bptr[ii], n0 = nb0[ii], nb = bptr[ii + 1] - b_lo;
+  const bool has = live && sl < nb;
+  const int b = b_lo + (has ? sl : 0);
+  // Bond::updatebGeometry (bond.h:30-39)
+  const int j = p2[b];
+  const double4 xj = xyz[j];
+  const double bd0 = bdamage[b], d0 = dis0[b], dlp = dLp[b];
+  const double dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const bool broken = fabs(bd0 - 1.0) < kEPS;
+  const double dL = broken ? 0.0 : (d - d0);
+  const double le = broken ? 0.0 : (d - d0 - dlp);
+  const double cx = broken ? 0.0 : dx / d, cy = broken ? 0.0 : dy / d, cz = broken ? 0.0 : dz / d;
+  if (has) {
+    dis[b] = d;
+    bstrain[b] = broken ? 0.0 : (dL / d);
+    dLe[b] = le;
+    cs[3 * b] = cx;
+    cs[3 * b + 1] = cy;
+    cs[3 * b + 2] = cz;
+  }
+  terms[w][lane][0] = le;
+  terms[w][lane][1] = tv * le;
+  terms[w][lane][2] = cx;
+  terms[w][lane][3] = cy;
+  terms[w][lane][4] = cz;
+  wave_lds_sync();
+  // Particle::updateBondsGeometry (particle.h:163-181): lane q < 10 of the
+  // particle sums term q % 5 over the bonds of layer q / 5, in bond order
+  double tot = 0;
+  if (live && sl < 10) {
+    const int L = sl / 5, q = sl - 5 * L;
+    const int tlo = L == 0 ? 0 : n0, thi = L == 0 ? n0 : nb;
+    for (int t = tlo; t < thi; ++t) tot += terms[w][base + t][q];
+    if (q == 0) dLe_tot[2 * i + L] = tot;
+    else if (q == 1) TdLe_tot[2 * i + L] = tot;
+    else cs_sum[(2 * i + L) * 3 + (q - 2)] = tot;
+  }
+  // updateBondsForce (particle_elastic.h:38-50 and the damage models'):
+  // dLe_total of the bond's layer from the lane that summed it
+  const int L = sl < n0 ? 0 : 1;
+  const double dlt = __shfl(tot, base + 5 * L, 64);
+  if (has) {
+    const double kn = Kn[2 * i + L];
+    double f = 2. * kn * le + 2. * tv * dlt;
+    double bd = bd0;
+    if (ptype != kElastic) {
+      bd = std_max(bd, std_max(damage[i], damage[j]));
+      bdamage[b] = bd;
+    }
+    f *= (1.0 - bd);
+    bforce[b] = f;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+particle_force_lanes_kernel(const int* __restrict__ bptr, const int* __restrict__ rev,
+                            const double* __restrict__ bforce, const double* __restrict__ cs,
+                            const double* __restrict__ dis, const double* __restrict__ bdamage,
+                            double volume, double nneighbors, int N, double* __restrict__ Pin,
+                            double* __restrict__ stress, double* __restrict__ damage_visual) {
+  __shared__ double terms[256 / 64][64][10];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, base = lane & ~(kFP - 1), sl = lane & (kFP - 1);
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / kFP;
+  const bool live = i < N;
+  const int ii = live ? i : N - 1;
+  const int lo = bptr[ii], nb = bptr[ii + 1] - lo;
+  const bool has = live && sl < nb;
+  const int b = lo + (has ? sl : 0);
+  // Particle::updateParticleForce / Stress / DamageVisual (particle.h:150-161, 197-240)
+  const double V_m = volume * nb / nneighbors;
+  const double fsum = bforce[b] + bforce[rev[b]];
+  const double cx = cs[3 * b], cy = cs[3 * b + 1], cz = cs[3 * b + 2];
+  const double wv = 0.5 / V_m * dis[b] * 0.5 * fsum;
+  double* const tm = terms[w][lane];
+  tm[0] = cx * 0.5 * fsum;
+  tm[1] = cy * 0.5 * fsum;
+  tm[2] = cz * 0.5 * fsum;
+  tm[3] = wv * cx * cx;
+  tm[4] = wv * cy * cy;
+  tm[5] = wv * cz * cz;
+  tm[6] = wv * cy * cz;
+  tm[7] = wv * cx * cz;
+  tm[8] = wv * cx * cy;
+  tm[9] = bdamage[b];
+  wave_lds_sync();
+  if (live && sl < 10) {
+    double tot = 0;
+    for (int t = 0; t < nb; ++t) tot += terms[w][base + t][sl];
+    if (sl < 3) Pin[3 * i + sl] = tot;
+    else if (sl < 9) stress[6 * (size_t)i + (sl - 3)] = tot;
+    else damage_visual[i] = tot / (double)nb;
+  }
+}
+
 // Solver::updateRR: residual of free DOFs, and the sum of squares of the
 // reactions at constrained DOFs (the reference appends them to a vector that is
 // only ever read through its norm).
@@ -202,6 +333,18 @@ __global__ void move_by_kernel(double4* __restrict__ xyz, const double* __restri
 
 int launch_forward(Context* c) {
   const int N = c->N, tb = 256, nb = (N + tb - 1) / tb;
+  if (c->max_nb <= kFP) {
+    const int nbl = (int)(((int64_t)N * kFP + tb - 1) / tb);
+    geometry_force_lanes_kernel<<<nbl, tb, 0, c->stream>>>(
+        c->xyz, c->bptr, c->nb0, c->p2, c->dis0, c->dLp, c->bdamage, c->Kn, c->Tv, c->damage,
+        c->ptype, N, c->dis, c->bstrain, c->dLe, c->cs, c->bforce, c->dLe_tot, c->TdLe_tot, c->cs_sum);
+    particle_force_lanes_kernel<<<nbl, tb, 0, c->stream>>>(c->bptr, c->rev, c->bforce, c->cs, c->dis,
+                                                          c->bdamage, c->cell.volume,
+                                                          (double)c->cell.nneighbors, N, c->Pin, c->stress,
+                                                          c->damage_visual);
+    LPMG_HIP(hipGetLastError());
+    return 0;
+  }
   geometry_force_kernel<<<nb, tb, 0, c->stream>>>(
       c->xyz, c->bptr, c->nb0, c->p2, c->dis0, c->dLp, c->bdamage, c->Kn, c->Tv, c->damage,
       c->ptype, N, c->dis, c->bstrain, c->dLe, c->cs, c->bforce, c->dLe_tot, c->TdLe_tot, c->cs_sum);
