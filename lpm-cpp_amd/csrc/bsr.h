@@ -8,10 +8,11 @@
 namespace lpmg {
 
 constexpr int kBS = 256;         // vector kernels
-// SpMV kernels: 8 wavefronts per workgroup.  The CG SpMV ends in a workgroup
-// reduction of its dot product, whose barrier holds every wavefront until the
-// slowest row is done; 8 wavefronts wait less than 16 (SC 40^3: 51.9 vs
-// 58.2 us per CG SpMV).
+// SpMV kernels: 8 wavefronts (32 block rows) per workgroup.  The CG SpMV ends
+// in a workgroup reduction of its dot product, whose barrier holds every
+// wavefront until the slowest row is done.  Measured per CG SpMV with 16-lane
+// rows: 1024 threads 52.5 us on SC 40^3; 128 and 256 threads within 2% of 512
+// on SC 40^3 but 4% slower on the 250k CT plate (45.0 vs 43.2 us back to back).
 constexpr int kSpBS = 512;
 constexpr int kMaxGrid = 1024;   // workgroups of the grid-strided vector kernels
 constexpr int kMaxParts = 8192;  // SpMV partials summed directly by the consumer

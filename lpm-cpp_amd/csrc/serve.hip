@@ -14,7 +14,7 @@
 //            solve.  A worker's CG start (cg_begin) is enqueued on its own
 //            stream; the worker then hands the system to the batcher and
 //            sleeps.  Per chunk of kCgChunk iterations the batcher launches,
-//            for each batch class (dimension, precision, lanes per row) with
+//            for each batch class (dimension, precision, chunks per pass) with
 //            members, one fused SpMV, one fused reduction (large systems only)
 //            and one fused vector launch per iteration, then reads every
 //            member's done flag.  Systems join at chunk boundaries (after

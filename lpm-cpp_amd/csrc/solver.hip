@@ -29,7 +29,7 @@
 //
 // The same iteration bodies also run as fused launches over many systems (the
 // serving runtime): one launch per kernel kind covers every system of a batch
-// class (dimension, precision, lanes per row); workgroup g finds its system by
+// class (dimension, precision, chunks per pass); workgroup g finds its system by
 // a binary search over the systems' first workgroups and then computes exactly
 // what that system's own launch would compute in its workgroup g - first, so a
 // result does not depend on the batch it ran in.
