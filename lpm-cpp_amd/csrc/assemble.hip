@@ -29,9 +29,9 @@ namespace lpmg {
 namespace {
 
 constexpr int kWaves = 4;
-// Path entries per wavefront in tangent_lds_kernel's LDS slice (36 bytes each
-// plus 1.5 KB of per-bond and per-connection tables):
-// kWaves * (36 * 384 + 1536) B = 60 KB per workgroup
+// Path entries per wavefront in tangent_lds_kernel's LDS slice (36 bytes each,
+// plus 1.5 KB of per-bond and per-connection tables and 72 bytes per bond of
+// direct terms; a workgroup's kWaves slices stay within 64 KB)
 constexpr int kTangentLdsMax = 384;
 
 template <int D>
@@ -335,15 +335,18 @@ tangent_lanes_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
 // has at most one path per bond; the walk then takes the lane's next list entry
 // at each bond whose bit is set, all lanes together, instead of scanning every
 // path end for its own j.  A bond list in which two of k's bonds share an end
-// takes the scan (detected while ranking).  Walk order and expressions are those
-// of tangent_kernel, so the bits are the same.  Rows of up to 64 connections.
+// takes the scan (detected while ranking).  Each bond's direct term is formed
+// by its bond lane into LDS and added by the lane whose j is the bond's end at
+// that bond's step, instead of one lane computing it while the wavefront waits.
+// Walk order and expressions are those of tangent_kernel, so the bits are the
+// same.  Rows of up to 64 connections.
 template <int D>
 __global__ void __launch_bounds__(64 * kWaves)
 tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
                    const int* __restrict__ p2, const double* __restrict__ cs,
                    const double* __restrict__ bdamage, const double* __restrict__ cs_sum,
                    const double* __restrict__ Kn, const double* __restrict__ Tv,
-                   const int* __restrict__ cptr, const int* __restrict__ conns, int N, int E,
+                   const int* __restrict__ cptr, const int* __restrict__ conns, int N, int E, int NBM,
                    double* __restrict__ H, double* __restrict__ diag_sq) {
   extern __shared__ double lds_tangent[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -359,6 +362,8 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
   short* const Lot = (short*)(lds_tangent + (size_t)kWaves * (4 * E + 64) + (size_t)kWaves * 128) +
                      (size_t)w * 2 * E;  // [E] owner << 8 | bond, -1 without owner
   short* const Llist = Lot + E;          // [E] entries by owner, in bond order
+  double* const Ld = (double*)(lds_tangent + (size_t)kWaves * (4 * E + 64) + (size_t)kWaves * 128 +
+                               (size_t)kWaves * E / 2) + (size_t)w * NBM * D * D;  // [NBM][D][D] direct terms
   int c_lo = 0, nc = 0, b_lo = 0, n0 = 0, nbi = 0;
   double tv_i = 0;
   if (row) {
@@ -380,6 +385,19 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
     klo = lane < n0 ? k0 : k1;
     nk = lane < n0 ? k1 - k0 : k2 - k1;
     btv = Tv[bk];
+    // bond t's direct term (derivative.h:59-88 / 156-185), added at step t of
+    // the walk by the lane whose j is the bond's end
+    const int L = lane < n0 ? 0 : 1;
+    const double kn = Kn[2 * i + L];
+    const double* csum = cs_sum + (2 * bk + L) * 3;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
+        const double geo = (r == s) ? (kn + tv_i) * (bc[r] * bc[r]) : (kn + tv_i) * bc[hi_rs] * bc[lo_rs];
+        Ld[(lane * D + r) * D + s] = bom * ((-kn * bc[s] + tv_i * csum[s]) * bc[r] - geo);
+      }
   }
   const int incl = wave_incl_scan(nk);
   const int off = incl - nk;
@@ -480,7 +498,6 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
     for (int r = 0; r < D; ++r)
 #pragma unroll
       for (int s = 0; s < D; ++s) Tm[r][s] = 0.0;
-    const double kn = Kn[2 * i + L];
     for (int t = tlo; t < thi; ++t) {
       const int k = __shfl(bk, t, 64);
       double c2[3];
@@ -498,17 +515,10 @@ tangent_lds_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
       const int qp = path ? Llist[cursor] : 0;
       cursor += path ? 1 : 0;
       if (direct) {
-        // direct bond term (derivative.h:59-88 / 156-185)
-        const double* csum = cs_sum + (2 * k + L) * 3;
 #pragma unroll
         for (int r = 0; r < D; ++r)
 #pragma unroll
-          for (int s = 0; s < D; ++s) {
-            const int hi_rs = r > s ? r : s, lo_rs = r > s ? s : r;
-            const double geo = (r == s) ? (kn + tv_i) * (c2[r] * c2[r])
-                                        : (kn + tv_i) * c2[hi_rs] * c2[lo_rs];
-            Tm[r][s] += om2 * ((-kn * c2[s] + tv_i * csum[s]) * c2[r] - geo);
-          }
+          for (int s = 0; s < D; ++s) Tm[r][s] += Ld[(t * D + r) * D + s];
       } else {
         if (t1 >= 0) {
           // common-neighbour term through i's dilatation (derivative.h:91-99 / 188-196)
@@ -737,17 +747,20 @@ int launch_assemble(Context* c, int want32) {
   if (c->floating_dirty)
     if (int rc = update_floating(c)) return rc;
   float* k32 = want32 ? c->Kval32 : nullptr;
-  // LDS-staged tangent when a row's two-hop path data fits 64 KB per workgroup
-  // and its connections one wavefront
-  const int lds_E = (c->max_nb <= 64 && c->max_nconn <= 64 && c->max_cand <= kTangentLdsMax)
-                        ? std::max(32, (c->max_cand + 31) / 32 * 32) : 0;
+  // LDS-staged tangent when a row's two-hop path data and direct terms fit
+  // 64 KB per workgroup and its connections one wavefront
+  const int nbm = c->max_nb;
+  int lds_E = (c->max_nb <= 64 && c->max_nconn <= 64 && c->max_cand <= kTangentLdsMax)
+                  ? std::max(32, (c->max_cand + 7) / 8 * 8) : 0;
   const size_t lds_bytes = (size_t)kWaves * ((size_t)(4 * lds_E + 64) * sizeof(double) + 256 * sizeof(int) +
-                                             (size_t)2 * lds_E * sizeof(short));
+                                             (size_t)2 * lds_E * sizeof(short) +
+                                             (size_t)nbm * c->dim * c->dim * sizeof(double));
+  if (lds_bytes > 65536) lds_E = 0;
   if (c->dim == 3) {
     if (lds_E > 0)
       tangent_lds_kernel<3><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                        c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                                       N, lds_E, c->Kval, c->diag_sq);
+                                                                       N, lds_E, nbm, c->Kval, c->diag_sq);
     else if (c->max_nb <= 64)
       tangent_lanes_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                  c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
@@ -765,7 +778,7 @@ int launch_assemble(Context* c, int want32) {
     if (lds_E > 0)
       tangent_lds_kernel<2><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                        c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                                       N, lds_E, c->Kval, c->diag_sq);
+                                                                       N, lds_E, nbm, c->Kval, c->diag_sq);
     else if (c->max_nb <= 64)
       tangent_lanes_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                  c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
