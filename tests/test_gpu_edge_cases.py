@@ -4,6 +4,7 @@
   a bond list), so rows of more than 64 bonds and more than 64 connections --
   the SpMV's multi-pass rows in FP64 and FP32 and the tangent's path-walking
   kernel;
+* a row of exactly 32 bonds (the largest the lane-per-bond forward pass takes);
 * a bond listed twice in a bond list (the tangent's scan over path ends);
 * the smallest systems: two particles and one bond; a particle without bonds
   is an input error (the reference's stiffness has no diagonal for it), reported
@@ -100,6 +101,30 @@ def test_rows_beyond_one_wavefront_match_oracle():
     g.solve_linear(**GPU_OPTS)
     o.solve_linear(**ORC_OPTS)
     assert _relerr(g.get("XYZ") - x0, o.get("XYZ") - x0) < 1e-10
+
+
+def test_row_of_exactly_32_bonds_matches_oracle():
+    """A hub of 18 + 14 = 32 bonds: the largest row the lane-per-bond forward
+    pass takes (32 lanes per particle, every lane holding a bond)."""
+    xyz = lat.sc_block(6, 6, 6, R)
+    bonds, hub = _hub_bonds(xyz, n_extra=14)
+    g = lpmg.System(lat.SC3D, R, lat.ELASTIC, xyz, bonds=bonds)
+    o = Oracle(lat.SC3D, R, lat.ELASTIC, xyz, bonds=bonds)
+    bptr, _, _ = o.bonds()
+    assert np.diff(bptr).max() == 32
+    rng = np.random.default_rng(6)
+    x = xyz + 5e-4 * R * rng.standard_normal(xyz.shape)
+    for s in (g, o):
+        s.set_material(*MAT)
+        s.set("XYZ", x)
+        s.forward()
+    for f in ("PIN", "STRESS"):
+        a, b_ = g.get(f), o.get(f)
+        assert np.abs(a - b_).max() <= 1e-13 * np.abs(b_).max(), f
+    g.assemble()
+    o.assemble()
+    Kg, Ko = g.csr()[3], o.csr()[3]
+    assert np.abs(Kg - Ko).max() <= 1e-13 * np.abs(Ko).max()
 
 
 def test_repeated_bond_in_a_bond_list_matches_oracle():
