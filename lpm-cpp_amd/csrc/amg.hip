@@ -56,7 +56,9 @@ constexpr double kOmega = 2.0 / 3.0;
 
 // Row gather of A x for this lane's row, reduced over the row's lanes.  XM 0:
 // x_j = x[j]; XM 1: x_j = om * dinv[j] * b[j] (one Jacobi sweep from zero).
-template <int D, typename T, int LPR, int XM>
+// A pass covers U chunks of LPR blocks, issued as bsr_rows issues them (block
+// columns and values, then the gathers, each group together).
+template <int D, typename T, int LPR, int U, int XM>
 __device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            const T* __restrict__ dinv, const T* __restrict__ b, T om, int N,
@@ -71,27 +73,39 @@ __device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr
   const int lo = cptr[rw], hi = cptr[rw + 1];
 #pragma unroll
   for (int q = 0; q < D; ++q) acc[q] = T(0);
-  for (int e0 = lo; e0 < hi; e0 += LPR) {
-    const int e = e0 + sl;
-    const bool ok = e < hi;
-    const int ec = ok ? e : hi - 1;
-    const int col = conns[ec];
-    const T* vb = val + (size_t)ec * DD;
-    T v[DD], xv[D];
+  for (int e0 = lo; e0 < hi; e0 += U * LPR) {
+    int col[U], ec[U];
+    bool ok[U];
+    T v[U][DD], xv[U][D];
 #pragma unroll
-    for (int k = 0; k < DD; ++k) v[k] = vb[k];
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      const size_t q = (size_t)col * D + k;
-      xv[k] = XM == 0 ? x[q] : om * dinv[q] * b[q];
+    for (int c = 0; c < U; ++c) {
+      const int e = e0 + c * LPR + sl;
+      ok[c] = e < hi;
+      ec[c] = ok[c] ? e : hi - 1;
+      col[c] = conns[ec[c]];
     }
 #pragma unroll
-    for (int q = 0; q < D; ++q) {
-      T t = v[q * D] * xv[0];
+    for (int c = 0; c < U; ++c)
 #pragma unroll
-      for (int k = 1; k < D; ++k) t += v[q * D + k] * xv[k];
-      acc[q] += ok ? t : T(0);
-    }
+      for (int k = 0; k < DD; ++k) v[c][k] = val[(size_t)ec[c] * DD + k];
+    if constexpr (U > 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < U; ++c)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const size_t q = (size_t)col[c] * D + k;
+        xv[c][k] = XM == 0 ? x[q] : om * dinv[q] * b[q];
+      }
+    if constexpr (U > 1) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int c = 0; c < U; ++c)
+#pragma unroll
+      for (int q = 0; q < D; ++q) {
+        T t = v[c][q * D] * xv[c][0];
+#pragma unroll
+        for (int k = 1; k < D; ++k) t += v[c][q * D + k] * xv[c][k];
+        acc[q] += ok[c] ? t : T(0);
+      }
   }
 #pragma unroll
   for (int o = LPR / 2; o > 0; o >>= 1)
@@ -112,7 +126,7 @@ __device__ __forceinline__ T pick(const T acc[D], int sl) {
 }
 
 // r = b - A (om D^-1 b): the residual after one Jacobi sweep from zero
-template <int D, typename T, int LPR>
+template <int D, typename T, int LPR, int U>
 __global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* __restrict__ cptr,
                                                                     const int* __restrict__ conns,
                                                                     const T* __restrict__ val, int N,
@@ -123,7 +137,7 @@ __global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* _
   int row, sl;
   bool live;
   T acc[D];
-  row_gather<D, T, LPR, 1>(blockIdx.x, cptr, conns, val, nullptr, dinv, b, om, N, &row, &live, &sl, acc);
+  row_gather<D, T, LPR, U, 1>(blockIdx.x, cptr, conns, val, nullptr, dinv, b, om, N, &row, &live, &sl, acc);
   if (live && sl < D) {
     const size_t q = (size_t)row * D + sl;
     r[q] = b[q] - pick<D, T>(acc, sl);
@@ -133,7 +147,7 @@ __global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* _
 // z = x + om D^-1 (b - A x); with `mask` (finest level), z = 0 on the
 // eliminated DOFs, so the CG never moves them however inexact the solve; with
 // `part`, the workgroup's partial of (b, z)
-template <int D, typename T, int LPR>
+template <int D, typename T, int LPR, int U>
 __global__ void __launch_bounds__(kSpBS) amg_postsmooth_kernel(const int* __restrict__ cptr,
                                                                const int* __restrict__ conns,
                                                                const T* __restrict__ val, int N,
@@ -146,7 +160,7 @@ __global__ void __launch_bounds__(kSpBS) amg_postsmooth_kernel(const int* __rest
   int row, sl;
   bool live;
   T acc[D];
-  row_gather<D, T, LPR, 0>(blockIdx.x, cptr, conns, val, x, nullptr, nullptr, om, N, &row, &live, &sl, acc);
+  row_gather<D, T, LPR, U, 0>(blockIdx.x, cptr, conns, val, x, nullptr, nullptr, om, N, &row, &live, &sl, acc);
   double dot = 0;
   if (live && sl < D) {
     const size_t q = (size_t)row * D + sl;
@@ -301,7 +315,7 @@ __global__ void __launch_bounds__(1024) amg_invert_kernel(const int* __restrict_
 
 // ---- PCG kernels (one system, its own stream) ----
 
-template <int D, typename T, int LPR>
+template <int D, typename T, int LPR, int U>
 __global__ void __launch_bounds__(kSpBS) pcg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
                                                          const T* __restrict__ val, const T* __restrict__ p,
                                                          T* __restrict__ w, int N, double* __restrict__ part,
@@ -309,7 +323,7 @@ __global__ void __launch_bounds__(kSpBS) pcg_spmv_kernel(const int* __restrict__
   if (sc->done) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // read by the next kernels
   Vec<1> a;
-  a.v[0] = bsr_rows<D, T, LPR, 64 / LPR, true>(blockIdx.x, cptr, conns, val, p, w, N);
+  a.v[0] = bsr_rows<D, T, LPR, U, true>(blockIdx.x, cptr, conns, val, p, w, N);
   write_partial<kSpBS, 1>(part, a);
 }
 
@@ -448,6 +462,34 @@ int grid_n(int64_t work, int per) {
   int64_t g = (work + per - 1) / per;
   g = std::max<int64_t>(8, std::min<int64_t>(g, kMaxGrid));
   return (int)g;
+}
+
+// Row shape of the finest level: 16 lanes a row and chunks_per_pass(c) chunks
+// per pass (as the Jacobi CG's SpMV) when every row fits four chunks, else 64
+// lanes.  Coarser levels take 32 or 64 lanes, one chunk per pass.
+void fine_shape(const Context* c, int* lpr, int* u) {
+  if (c->max_nconn <= 4 * kLPR) {
+    *lpr = kLPR;
+    *u = chunks_per_pass(c);
+  } else {
+    *lpr = 64;
+    *u = 1;
+  }
+}
+
+// f(lanes per row, chunks per pass) with both as compile-time constants.
+template <class F>
+void with_shape(int lpr, int u, F&& f) {
+  using std::integral_constant;
+  if (lpr == 16) {
+    if (u == 2) f(integral_constant<int, 16>{}, integral_constant<int, 2>{});
+    else if (u == 3) f(integral_constant<int, 16>{}, integral_constant<int, 3>{});
+    else f(integral_constant<int, 16>{}, integral_constant<int, 4>{});
+  } else if (lpr == 32) {
+    f(integral_constant<int, 32>{}, integral_constant<int, 1>{});
+  } else {
+    f(integral_constant<int, 64>{}, integral_constant<int, 1>{});
+  }
 }
 
 int rows_grid(int N, int lpr) {
@@ -697,28 +739,28 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
   const int* conns = l == 0 ? c->conns : A->lv[l - 1].conns;
   const T* val = l == 0 ? (const T*)(P ? (const void*)c->Kval32 : (const void*)c->Kval) : (const T*)A->lv[l - 1].val[P];
   const T* dinv = l == 0 ? (const T*)(P ? (const void*)c->f_dinv : (const void*)c->dinv) : (const T*)A->lv[l - 1].dinv[P];
-  const int lpr = l == 0 ? (c->max_nconn <= 32 ? 32 : 64) : A->lv[l - 1].lpr;
+  int lpr = 64, u = 1;
+  if (l == 0) fine_shape(c, &lpr, &u);
+  else lpr = A->lv[l - 1].lpr;
   T* r = l == 0 ? scratch_r : (T*)A->lv[l - 1].vec[P][1];
   T* xp = l == 0 ? scratch_x : (T*)A->lv[l - 1].vec[P][2];
   const AmgLevel& C = A->lv[l];  // the next level
   T* bc = (T*)C.vec[P][0];
   T* xc = (l + 1 == L) ? (T*)C.vec[P][2] : (T*)C.vec[P][3];
   const int g = rows_grid(N, lpr);
-  if (lpr == 32)
-    amg_presmooth_resid_kernel<D, T, 32><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
-  else
-    amg_presmooth_resid_kernel<D, T, 64><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
+  with_shape(lpr, u, [&](auto LP, auto UC) {
+    amg_presmooth_resid_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
+        <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
+  });
   amg_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(C.mptr, C.midx, C.N, r, bc, c->sc);
   vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr, nullptr);
   amg_prolong_kernel<D, T><<<(unsigned)(((int64_t)N * D + 255) / 256), 256, 0, st>>>(C.agg, N, b, dinv, om, xc, xp,
                                                                                       c->sc);
   const T* mask = l == 0 ? (const T*)A->mask[P] : nullptr;
-  if (lpr == 32)
-    amg_postsmooth_kernel<D, T, 32><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, xp, dinv, om, mask, out,
-                                                        l == 0 ? rz_part : nullptr, c->sc);
-  else
-    amg_postsmooth_kernel<D, T, 64><<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, xp, dinv, om, mask, out,
-                                                        l == 0 ? rz_part : nullptr, c->sc);
+  with_shape(lpr, u, [&](auto LP, auto UC) {
+    amg_postsmooth_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
+        <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, xp, dinv, om, mask, out, l == 0 ? rz_part : nullptr, c->sc);
+  });
 }
 
 }  // namespace
@@ -761,7 +803,8 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
     if (int rc = amg_values<D, T>(c)) return rc;
     A->version[P] = c->asm_version;
   }
-  const int lpr = c->max_nconn <= 32 ? 32 : 64;
+  int lpr = 64, u = 1;
+  fine_shape(c, &lpr, &u);
   const int gs = rows_grid(N, lpr);
   const int gv = grid_n(n, kBS * 4);
   const bool two = gs > kMaxParts;
@@ -787,10 +830,10 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   LPMG_HIP(hipGetLastError());
   auto enqueue_chunk = [&]() -> int {
     for (int k = 0; k < kCgChunk; ++k) {
-      if (lpr == 32)
-        pcg_spmv_kernel<D, T, 32><<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
-      else
-        pcg_spmv_kernel<D, T, 64><<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
+      with_shape(lpr, u, [&](auto LP, auto UC) {
+        pcg_spmv_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
+            <<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
+      });
       if (two) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(pw_raw, gs, pw2, c->sc);
       pcg_vec1_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, p, w, n, two ? pw2 : pw_raw, npw, rr_part, c->sc);
       vcycle<D, T>(c, c->stream, 0, r, z, w, tx, rz_raw);

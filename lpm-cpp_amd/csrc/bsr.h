@@ -20,6 +20,14 @@ constexpr int kMaxParts = 8192;  // SpMV partials summed directly by the consume
 // Lanes per block row of the SpMV and CG kernels: four rows per wavefront.
 constexpr int kLPR = 16;
 
+// Chunks of kLPR blocks per pass over a row: enough for the system's longest
+// row (2 for the hexagonal lattice's 31 blocks, 3 for BCC's 41, 4 for SC's and
+// FCC's 61); longer rows take several passes.  Chunks past every row of a
+// wavefront cost issue slots, so the pass is sized to the longest row.
+inline int chunks_per_pass(const Context* c) {
+  return std::min(4, std::max(2, (c->max_nconn + kLPR - 1) / kLPR));
+}
+
 // y = A x for the rows of wavefronts of workgroup `blk` (64 / LPR rows per
 // wavefront, LPR lanes each, U chunks of LPR blocks per pass over a row).
 // With DOT, returns this lane's share of x.y (nonzero only in the row's first

@@ -371,19 +371,12 @@ int grid_for(int64_t work, int per) {
   return (int)g;
 }
 
-// Chunks of 16 blocks per pass over a row: enough for the longest row (2 for
-// the hexagonal 2D lattice's 31 blocks, 3 for BCC's 41, 4 for SC's and FCC's
-// 61); longer rows take several passes.  A row takes 16 lanes, four rows per
-// wavefront, in both precisions: a row's dependent loads (row start, block
-// columns, x) bound one row per wavefront, and 16 lanes keep four rows' bytes
-// in flight (back to back on SC 40^3: FP64 42.2 us with 16 lanes, 47.2 us with
-// 64, 57.0 us with 32; the BCC J2 bar: 58.4 vs 68.7 us with 64; the 250k CT
-// plate: 43.9 vs 50.4 us with 32).  Chunks past every row of a wavefront cost
-// issue slots, so the pass is sized to the system's longest row.
-int chunks_per_pass(const Context* c) {
-  return std::min(4, std::max(2, (c->max_nconn + kLPR - 1) / kLPR));
-}
-
+// A row takes 16 lanes, four rows per wavefront, in both precisions, and a pass
+// covers chunks_per_pass(c) chunks of 16 blocks: a row's dependent loads (row
+// start, block columns, x) bound one row per wavefront, and 16 lanes keep four
+// rows' bytes in flight (back to back on SC 40^3: FP64 42.2 us with 16 lanes,
+// 47.2 us with 64, 57.0 us with 32; the BCC J2 bar: 58.4 vs 68.7 us with 64;
+// the 250k CT plate: 43.9 vs 50.4 us with 32).
 int spmv_grid(const Context* c) {
   const int rows_per_block = (kSpBS / 64) * (64 / kLPR);
   return (c->N + rows_per_block - 1) / rows_per_block;
