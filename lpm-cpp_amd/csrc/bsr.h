@@ -28,31 +28,12 @@ inline int chunks_per_pass(const Context* c) {
   return std::min(4, std::max(2, (c->max_nconn + kLPR - 1) / kLPR));
 }
 
-// y = A x for the rows of wavefronts of workgroup `blk` (64 / LPR rows per
-// wavefront, LPR lanes each, U chunks of LPR blocks per pass over a row).
-// With DOT, returns this lane's share of x.y (nonzero only in the row's first
-// b lanes).
-//
-// A row's pass is latency-bound: row start -> block columns -> x gathers are
-// three dependent loads.  A pass issues the block columns and values of all U
-// chunks and all their x gathers before it waits for the products: one latency
-// for the gathers of the whole pass.  With LPR < 64, 64 / LPR rows per
-// wavefront keep that many times the bytes in flight.
-template <int D, typename T, int LPR, int U, bool DOT>
-__device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
-                                           const T* __restrict__ val, const T* __restrict__ x,
-                                           T* __restrict__ y, int N) {
-  constexpr int RPW = 64 / LPR;
-  const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
-  const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
-  const int row = wave * RPW + sub;
-  const bool live = row < N;
-  const int rw = live ? row : N - 1;  // dead rows shadow the last row: all lanes stay in the shuffles
-  const int lo = cptr[rw], hi = cptr[rw + 1];
-  // With DOT: this lane's own entry of x for the dot product, loaded with the
-  // row's extent.
-  const T xo = DOT ? x[(size_t)rw * D + min(sl, D - 1)] : T(0);
-  T acc[D];
+// acc = this lane's row's share of A x for blocks lo..hi-1 (LPR lanes per row,
+// U chunks of LPR blocks per pass), summed across the row's lanes: every lane
+// of the row ends with the row's D sums.
+template <int D, typename T, int LPR, int U>
+__device__ __forceinline__ void bsr_row_acc(int lo, int hi, int sl, const int* __restrict__ conns,
+                                            const T* __restrict__ val, const T* __restrict__ x, T acc[D]) {
 #pragma unroll
   for (int q = 0; q < D; ++q) acc[q] = T(0);
   for (int e0 = lo; e0 < hi; e0 += U * LPR) {
@@ -61,9 +42,10 @@ __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr
     // gathers the columns address, all chunks' gathers in flight together.
     // With three or four chunks the values are issued before the gathers (they
     // stream in while the gathers wait for their columns); with two, after
-    // them, skipping a chunk that no row of the wavefront reaches.  The scheduling barriers keep each group together (measured on the
-    // SC 40^3 CG SpMV: 44.1 us against 46.5 with the values last; FP32 CT
-    // plate, two chunks: 25.1 us with the values last against 27.3).
+    // them, skipping a chunk that no row of the wavefront reaches.  The
+    // scheduling barriers keep each group together (measured on the SC 40^3
+    // CG SpMV: 44.1 us against 46.5 with the values last; FP32 CT plate, two
+    // chunks: 25.1 us with the values last against 27.3).
     int col[U];
     bool ok[U];
     int ec[U];
@@ -117,6 +99,34 @@ __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr
   for (int o = LPR / 2; o > 0; o >>= 1)
 #pragma unroll
     for (int q = 0; q < D; ++q) acc[q] += __shfl_xor(acc[q], o, 64);
+}
+
+// y = A x for the rows of wavefronts of workgroup `blk` (64 / LPR rows per
+// wavefront, LPR lanes each, U chunks of LPR blocks per pass over a row).
+// With DOT, returns this lane's share of x.y (nonzero only in the row's first
+// b lanes).
+//
+// A row's pass is latency-bound: row start -> block columns -> x gathers are
+// three dependent loads.  A pass issues the block columns and values of all U
+// chunks and all their x gathers before it waits for the products: one latency
+// for the gathers of the whole pass.  With LPR < 64, 64 / LPR rows per
+// wavefront keep that many times the bytes in flight.
+template <int D, typename T, int LPR, int U, bool DOT>
+__device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
+                                           const T* __restrict__ val, const T* __restrict__ x,
+                                           T* __restrict__ y, int N) {
+  constexpr int RPW = 64 / LPR;
+  const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
+  const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
+  const int row = wave * RPW + sub;
+  const bool live = row < N;
+  const int rw = live ? row : N - 1;  // dead rows shadow the last row: all lanes stay in the shuffles
+  const int lo = cptr[rw], hi = cptr[rw + 1];
+  // With DOT: this lane's own entry of x for the dot product, loaded with the
+  // row's extent.
+  const T xo = DOT ? x[(size_t)rw * D + min(sl, D - 1)] : T(0);
+  T acc[D];
+  bsr_row_acc<D, T, LPR, U>(lo, hi, sl, conns, val, x, acc);
   double dot = 0;
   if (live && sl < D) {
     T yv = acc[0];
