@@ -25,8 +25,9 @@
 // Preconditioner: one V-cycle with one damped-Jacobi sweep (omega = 2/3) before
 // and after the coarse correction at every level.  Per level: a fused
 // residual kernel r = b - A (omega D^-1 b), the restriction (sum over each
-// aggregate's members), the prolongation fused with the pre-smoothed value,
-// and the post-smoothing sweep; the finest sweep also writes the partials of
+// aggregate's members), and the post-smoothing sweep, which forms the
+// prolonged correction plus the pre-smoothed value where it reads them; the
+// finest sweep also writes the partials of
 // (r, z) and zeroes z on the DOFs the Dirichlet elimination decoupled (the
 // coarse correction would otherwise move them: the preconditioner then acts
 // on the free DOFs only, where it stays symmetric positive definite).
@@ -55,14 +56,17 @@ constexpr double kOmega = 2.0 / 3.0;
 // ---- device kernels ------------------------------------------------------
 
 // Row gather of A x for this lane's row, reduced over the row's lanes.  XM 0:
-// x_j = x[j]; XM 1: x_j = om * dinv[j] * b[j] (one Jacobi sweep from zero).
+// x_j = x[j]; XM 1: x_j = om * dinv[j] * b[j] (one Jacobi sweep from zero);
+// XM 2: x_j = om * dinv[j] * b[j] + xc[agg[j]] (that sweep plus the coarse
+// correction, prolonged piecewise constant).
 // A pass covers U chunks of LPR blocks, issued as bsr_rows issues them (block
 // columns and values, then the gathers, each group together).
 template <int D, typename T, int LPR, int U, int XM>
 __device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            const T* __restrict__ dinv, const T* __restrict__ b, T om, int N,
-                                           int* row_out, bool* live_out, int* sl_out, T acc[D]) {
+                                           int* row_out, bool* live_out, int* sl_out, T acc[D],
+                                           const int* __restrict__ agg = nullptr, const T* __restrict__ xc = nullptr) {
   constexpr int DD = D * D;
   constexpr int RPW = 64 / LPR;
   const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
@@ -89,12 +93,18 @@ __device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr
 #pragma unroll
       for (int k = 0; k < DD; ++k) v[c][k] = val[(size_t)ec[c] * DD + k];
     if constexpr (U > 1) __builtin_amdgcn_sched_barrier(0);
+    int ag[U];
+    if constexpr (XM == 2)
+#pragma unroll
+      for (int c = 0; c < U; ++c) ag[c] = agg[col[c]];
 #pragma unroll
     for (int c = 0; c < U; ++c)
 #pragma unroll
       for (int k = 0; k < D; ++k) {
         const size_t q = (size_t)col[c] * D + k;
-        xv[c][k] = XM == 0 ? x[q] : om * dinv[q] * b[q];
+        if constexpr (XM == 0) xv[c][k] = x[q];
+        else if constexpr (XM == 1) xv[c][k] = om * dinv[q] * b[q];
+        else xv[c][k] = om * dinv[q] * b[q] + xc[(size_t)ag[c] * D + k];
       }
     if constexpr (U > 1) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -144,14 +154,18 @@ __global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* _
   }
 }
 
-// z = x + om D^-1 (b - A x); with `mask` (finest level), z = 0 on the
-// eliminated DOFs, so the CG never moves them however inexact the solve; with
-// `part`, the workgroup's partial of (b, z)
+// z = x + om D^-1 (b - A x) with x = om D^-1 b + P xc, the pre-smoothed
+// iterate plus the prolonged coarse correction, formed where it is read (for
+// the row itself and for every gathered column: no prolongation launch and
+// no vector for it); with `mask` (finest level), z = 0 on the eliminated
+// DOFs, so the CG never moves them however inexact the solve; with `part`,
+// the workgroup's partial of (b, z)
 template <int D, typename T, int LPR, int U>
 __global__ void __launch_bounds__(kSpBS) amg_postsmooth_kernel(const int* __restrict__ cptr,
                                                                const int* __restrict__ conns,
                                                                const T* __restrict__ val, int N,
-                                                               const T* __restrict__ b, const T* __restrict__ x,
+                                                               const T* __restrict__ b, const int* __restrict__ agg,
+                                                               const T* __restrict__ xc,
                                                                const T* __restrict__ dinv, T om,
                                                                const T* __restrict__ mask,
                                                                T* __restrict__ z, double* __restrict__ part,
@@ -160,11 +174,13 @@ __global__ void __launch_bounds__(kSpBS) amg_postsmooth_kernel(const int* __rest
   int row, sl;
   bool live;
   T acc[D];
-  row_gather<D, T, LPR, U, 0>(blockIdx.x, cptr, conns, val, x, nullptr, nullptr, om, N, &row, &live, &sl, acc);
+  row_gather<D, T, LPR, U, 2>(blockIdx.x, cptr, conns, val, nullptr, dinv, b, om, N, &row, &live, &sl, acc, agg,
+                              xc);
   double dot = 0;
   if (live && sl < D) {
     const size_t q = (size_t)row * D + sl;
-    T zq = x[q] + om * dinv[q] * (b[q] - pick<D, T>(acc, sl));
+    const T xq = om * dinv[q] * b[q] + xc[(size_t)agg[row] * D + sl];
+    T zq = xq + om * dinv[q] * (b[q] - pick<D, T>(acc, sl));
     if (mask) zq *= mask[q];
     z[q] = zq;
     dot = (double)b[q] * (double)zq;
@@ -204,18 +220,6 @@ __global__ void __launch_bounds__(256) amg_restrict_kernel(const int* __restrict
       if (lane == k) v = s[k];
     bc[(size_t)a * D + lane] = (T)v;
   }
-}
-
-// x = om D^-1 b + x_c[agg]: the pre-smoothed value plus the coarse correction
-template <int D, typename T>
-__global__ void amg_prolong_kernel(const int* __restrict__ agg, int N, const T* __restrict__ b,
-                                   const T* __restrict__ dinv, T om, const T* __restrict__ xc,
-                                   T* __restrict__ x, const Scalars* sc) {
-  if (sc->done) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)N * D) return;
-  const int i = (int)(t / D), k = (int)(t - (int64_t)i * D);
-  x[t] = om * dinv[t] * b[t] + xc[(size_t)agg[i] * D + k];
 }
 
 // x = Ainv b on the coarsest level (n <= kCoarsestMax)
@@ -724,7 +728,7 @@ int amg_values(Context* c) {
 // Enqueue z = M b at level l (l = 0: the handle's operator, b = the CG
 // residual, z written with partials of (b, z) into rz_part).
 template <int D, typename T>
-void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r, T* scratch_x, double* rz_part) {
+void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r, double* rz_part) {
   Amg* A = c->amg;
   const int P = prec_of<T>();
   const int L = (int)A->lv.size();
@@ -743,7 +747,6 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
   if (l == 0) fine_shape(c, &lpr, &u);
   else lpr = A->lv[l - 1].lpr;
   T* r = l == 0 ? scratch_r : (T*)A->lv[l - 1].vec[P][1];
-  T* xp = l == 0 ? scratch_x : (T*)A->lv[l - 1].vec[P][2];
   const AmgLevel& C = A->lv[l];  // the next level
   T* bc = (T*)C.vec[P][0];
   T* xc = (l + 1 == L) ? (T*)C.vec[P][2] : (T*)C.vec[P][3];
@@ -753,13 +756,12 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
         <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
   });
   amg_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(C.mptr, C.midx, C.N, r, bc, c->sc);
-  vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr, nullptr);
-  amg_prolong_kernel<D, T><<<(unsigned)(((int64_t)N * D + 255) / 256), 256, 0, st>>>(C.agg, N, b, dinv, om, xc, xp,
-                                                                                      c->sc);
+  vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr);
   const T* mask = l == 0 ? (const T*)A->mask[P] : nullptr;
   with_shape(lpr, u, [&](auto LP, auto UC) {
     amg_postsmooth_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
-        <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, xp, dinv, om, mask, out, l == 0 ? rz_part : nullptr, c->sc);
+        <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, C.agg, xc, dinv, om, mask, out, l == 0 ? rz_part : nullptr,
+                              c->sc);
   });
 }
 
@@ -789,11 +791,11 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   if (int rc = amg_alloc_prec<T>(c)) return rc;
   const T* val0 = (const T*)(P ? (const void*)c->Kval32 : (const void*)c->Kval);
   T* dinv0 = (T*)(P ? (void*)c->f_dinv : (void*)c->dinv);
-  T *x, *r, *z, *p, *w, *tx;
+  T *x, *r, *z, *p, *w;
   if (P) {
-    x = (T*)c->f_x; r = (T*)c->f_r; z = (T*)c->f_z; p = (T*)c->f_p; w = (T*)c->f_w; tx = (T*)c->f_q;
+    x = (T*)c->f_x; r = (T*)c->f_r; z = (T*)c->f_z; p = (T*)c->f_p; w = (T*)c->f_w;
   } else {
-    x = (T*)c->du; r = (T*)c->cg_r; z = (T*)c->cg_z; p = (T*)c->cg_p; w = (T*)c->cg_w; tx = (T*)c->cg_q;
+    x = (T*)c->du; r = (T*)c->cg_r; z = (T*)c->cg_z; p = (T*)c->cg_p; w = (T*)c->cg_w;
   }
   const int N = c->N;
   const int64_t n = (int64_t)D * N;
@@ -824,7 +826,7 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   const int64_t maxit = o.maxit > 0 ? o.maxit : n;
   amg_setup_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, o.rtol, o.atol, maxit);
   pcg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, n, rr_part);
-  vcycle<D, T>(c, c->stream, 0, r, z, w, tx, rz_raw);
+  vcycle<D, T>(c, c->stream, 0, r, z, w, rz_raw);
   if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
   pcg_init2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv, c->sc);
   LPMG_HIP(hipGetLastError());
@@ -836,7 +838,7 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
       });
       if (two) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(pw_raw, gs, pw2, c->sc);
       pcg_vec1_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, p, w, n, two ? pw2 : pw_raw, npw, rr_part, c->sc);
-      vcycle<D, T>(c, c->stream, 0, r, z, w, tx, rz_raw);
+      vcycle<D, T>(c, c->stream, 0, r, z, w, rz_raw);
       if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
       pcg_vec2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv,
                                                    c->sc);
