@@ -36,17 +36,20 @@ NEWTON_TOL = 1e-10
 # FP64 drop-in: Jacobi-PCG stopped where the reference stops MKL's CG (||r||^2 <=
 # 1e-12 ||r0||^2, solver.h:312), Newton tightened to 1e-10 so results meet 1e-8.
 FP64_OPTS = dict(precond=1, rtol=1e-12, atol=0.0, precision=0)
-# Mixed (the served configuration): FP32 stiffness and CG as an inexact inner
-# solve (||r||^2 <= 1e-4 ||r0||^2) of the FP64 modified Newton, whose FP64
+# Mixed (the served path's scheme): FP32 stiffness and CG as an inexact inner
+# solve (||r||^2 <= 3e-4 ||r0||^2) of the FP64 modified Newton, whose FP64
 # residual refines to the same 1e-10, each load step starting from the last
-# step's increment (solver option 6).
-MIXED_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
+# step's increment (solver option 6).  The inner tolerance is the fastest of
+# 1e-5 ... 1e-2 on this workload (looser: more Newton iterations; tighter: more
+# CG iterations per solve); the error stays near 2e-11 across that range.
+MIXED_RTOL = 3e-4
+MIXED_OPTS = dict(precond=1, rtol=MIXED_RTOL, atol=0.0, precision=1, predict=1.0)
 # Its parts, each measured alone: the mixed path without the predictor, the
 # FP64 path with the predictor, and the FP64 path at the mixed path's inexact
 # inner tolerance (the gain of FP32 alone is mixed_no_predictor over it).
-MIXED_NOPRED_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=1)
+MIXED_NOPRED_OPTS = dict(precond=1, rtol=MIXED_RTOL, atol=0.0, precision=1)
 FP64_PRED_OPTS = dict(precond=1, rtol=1e-12, atol=0.0, precision=0, predict=1.0)
-FP64_INEXACT_OPTS = dict(precond=1, rtol=1e-4, atol=0.0, precision=0)
+FP64_INEXACT_OPTS = dict(precond=1, rtol=MIXED_RTOL, atol=0.0, precision=0)
 # the reference's settings: MKL CG unpreconditioned, dpar[0] = dpar[1] = 1e-12
 # (solver.h:311-313), Newton tol_iter = 1e-5 and 30 iterations (examples)
 REF_OPTS = dict(precond=0, rtol=1e-12, atol=1e-12)
@@ -113,6 +116,9 @@ SERVE_OPENING = 7.5e-6  # pin opening per load step, in units of W
 # the requests of at most SERVE_FUSE_ROWS particles in fused Jacobi-CG launches
 # and the larger ones with the multigrid CG on their own streams (precond < 0:
 # the server's choice).
+# The CT stream keeps the inner tolerance at 1e-4: at 3e-4 its served
+# displacements moved 2.0e-8 from the plain FP64 ones (Newton tolerance 1e-9,
+# whose floor on these plates is about that), at 1e-4 1.6e-9.
 SERVED_OPTS = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
 
 
