@@ -781,6 +781,7 @@ int lpmg_set_field(lpmg_handle h, int f, const double* in) {
     pack_xyz_kernel<<<(c->N + 255) / 256, 256, 0, c->stream>>>(tmp, p, c->N);
   } else if (f == LPMG_F_CONSTRAINT) {
     d2i_kernel<<<(int)((n + 255) / 256), 256, 0, c->stream>>>(tmp, c->cons, n);
+    for (Group& g : c->groups) g.cons_set = 0;
   } else {
     API_HIP(hipMemcpyAsync(field_ptr(c, f), tmp, sizeof(double) * n, hipMemcpyDeviceToDevice, c->stream));
   }
@@ -799,6 +800,7 @@ int lpmg_restart(lpmg_handle h) {
   for (double* p : {c->Pex, c->Pin})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * 3 * N, c->stream));
   API_HIP(hipMemsetAsync(c->cons, 0, sizeof(int) * 3 * N, c->stream));
+  for (Group& g : c->groups) g.cons_set = 0;
   for (double* p : {c->damage, c->damage_last, c->Ddot_local, c->Ddot_nonlocal})
     API_HIP(hipMemsetAsync(p, 0, sizeof(double) * N, c->stream));
   if (int rc = launch_init_state_var(c)) return rc;

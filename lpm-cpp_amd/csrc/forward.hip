@@ -373,9 +373,12 @@ int launch_update_rr(Context* c, double* norm_res, double* reac_sq) {
   return 0;
 }
 
-int launch_disp_bc(Context* c, const Group& g, int mode, int k, double step) {
+int launch_disp_bc(Context* c, Group& g, int mode, int k, double step) {
   if (g.n == 0) return 0;
-  c->floating_dirty = true;  // the constrained set may grow
+  if (!(g.cons_set >> k & 1u)) {  // the constrained set may grow
+    c->floating_dirty = true;
+    g.cons_set |= 1u << k;
+  }
   disp_bc_kernel<<<(g.n + 255) / 256, 256, 0, c->stream>>>(c->xyz, c->cons, g.idx, g.n, mode, k, step);
   LPMG_HIP(hipGetLastError());
   return 0;

@@ -50,6 +50,10 @@ struct DArr {  // device array owned by the context
 struct Group {
   int* idx = nullptr;  // device
   int n = 0;
+  // Bit k: a displacement condition on component k has set cons[3*i+k] for
+  // every member since the constraint array was last cleared or overwritten,
+  // so another one cannot change the constrained set (or the fragment labels).
+  unsigned cons_set = 0;
 };
 
 // One system of a CG launch: its operator, vectors, reduction buffers, scalars
@@ -235,7 +239,7 @@ int exclusive_scan_i64(Context* c, const int64_t* in, int64_t* out, int n);
 // ---- forward.hip
 int launch_forward(Context* c);
 int launch_update_rr(Context* c, double* norm_res, double* norm_reac_step_sq);
-int launch_disp_bc(Context* c, const Group& g, int mode, int k, double step);
+int launch_disp_bc(Context* c, Group& g, int mode, int k, double step);
 int launch_force_bc(Context* c, const Group& g, int mode, double fx, double fy, double fz);
 int launch_move_by(Context* c, const double* du);
 
