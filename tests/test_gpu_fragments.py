@@ -49,3 +49,57 @@ def test_loose_fragment_is_held_and_rest_matches_oracle(precond):
     assert np.linalg.norm(ug - uo) <= 1e-8 * np.linalg.norm(uo)
     frag = np.repeat(ins_g, 3)
     assert np.all(ug[frag] == 0.0)
+
+
+def test_fragment_labels_follow_constraint_changes():
+    """The labels are recomputed only when the constrained set can change: a
+    displacement condition repeating one already applied keeps them, while an
+    overwritten constraint field or a restart (which clears the constraints)
+    makes the next assembly relabel, also when the same conditions follow."""
+    prob = P.hex_plate(nx=30, ny=26, steps=3)
+    xyz = prob.xyz
+    centre = (xyz[:, 0].mean(), xyz[:, 1].mean())
+    g, gg = P.build(lpmg.System, prob)
+    ins = _cut_out(g, xyz, centre, 0.45)
+    assert 2 <= ins.sum() < 40
+    held = lambda: g.get("FLOATING") > 0.5
+
+    def step_and_assemble():
+        P.apply_step(g, gg, prob.steps[0])
+        g.forward()
+        g.assemble()
+
+    step_and_assemble()
+    np.testing.assert_array_equal(held(), ins)
+    step_and_assemble()  # the same conditions again: labels kept
+    np.testing.assert_array_equal(held(), ins)
+
+    # one constrained DOF inside the fragment ties it down
+    p = int(np.flatnonzero(ins)[0])
+    cons = g.get("CONSTRAINT")
+    cons[3 * p] = 1.0
+    g.set("CONSTRAINT", cons)
+    step_and_assemble()
+    assert not held().any()
+    cons[3 * p] = 0.0
+    g.set("CONSTRAINT", cons)
+    step_and_assemble()
+    np.testing.assert_array_equal(held(), ins)
+
+    # a cleared constraint field, assembled, then the same conditions again
+    g.set("CONSTRAINT", np.zeros_like(cons))
+    g.forward()
+    g.assemble()
+    assert not held().any()
+    step_and_assemble()
+    np.testing.assert_array_equal(held(), ins)
+
+    # restart clears constraints and broken bonds; with nothing constrained
+    # nothing is held, and the first condition afterwards relabels
+    g.restart()
+    assert np.array_equal(_cut_out(g, xyz, centre, 0.45), ins)
+    g.forward()
+    g.assemble()
+    assert not held().any()
+    step_and_assemble()
+    np.testing.assert_array_equal(held(), ins)
