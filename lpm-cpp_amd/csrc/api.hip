@@ -232,7 +232,9 @@ int finish_create(lpmg_context* h) {
     set_error("stiffness exceeds 2^31 values per handle");
     return -1;
   }
-  if (dalloc(c, &c->Kval, (size_t)c->nnzb * c->dim * c->dim)) return -3;
+  if (dalloc(c, &c->Kval, (size_t)c->nnzb * c->dim * c->dim) ||
+      dalloc(c, &c->Kloc, (size_t)c->nnzb * c->dim * c->dim))
+    return -3;
   API_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -626,12 +626,6 @@ __global__ void __launch_bounds__(256) norm_diag_finish_kernel(const double* __r
   if (threadIdx.x == 0) sc->norm_diag = sqrt(tot.v[0]);
 }
 
-__global__ void __launch_bounds__(256) f64_to_f32_kernel(const double* __restrict__ a, float* __restrict__ b,
-                                                         int64_t n) {
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-    b[t] = (float)a[t];
-}
-
 // Bit k of fixm[i]: DOF k of particle i is eliminated (dof_fixed).
 template <int D>
 __global__ void __launch_bounds__(256)
@@ -645,64 +639,61 @@ fixed_dofs_kernel(const int* __restrict__ cons, const double* __restrict__ dvis,
   fixm[i] = m;
 }
 
-// One thread per block (i,j), j >= i, of the upper list.
+// One thread per block e = (i,j), in storage order: K(i,j) = 0.5 L(i,j) +
+// 0.5 L(j,i)^T from the tangent's unsymmetrised blocks L (the upper block's
+// term first, so a block and its transpose carry the same bits), then the
+// elimination of fixed DOFs (rows and columns zeroed, diagonal ||diag(K)||_2)
+// and the FP32 copy.  Every write is coalesced; only the L(j,i) reads scatter.
 template <int D>
 __global__ void __launch_bounds__(256)
-symmetrize_bc_kernel(const int* __restrict__ uidx, int nup, const int* __restrict__ brow,
-                     const int* __restrict__ conns, const int* __restrict__ tpos, const int* __restrict__ fixm,
-                     const Scalars* __restrict__ sc, double* __restrict__ Kval, float* __restrict__ Kval32) {
-  const int u = blockIdx.x * blockDim.x + threadIdx.x;
-  if (u >= nup) return;
-  const int e = uidx[u];
+symmetrize_bc_kernel(int nnzb, const int* __restrict__ brow, const int* __restrict__ conns,
+                     const int* __restrict__ tpos, const int* __restrict__ fixm, const Scalars* __restrict__ sc,
+                     const double* __restrict__ L, double* __restrict__ Kval, float* __restrict__ Kval32) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nnzb) return;
   const int i = brow[e], j = conns[e];
   const int mi = fixm[i], mj = fixm[j];
-  bool fi[D], fj[D];
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    fi[k] = (mi >> k) & 1;
-    fj[k] = (mj >> k) & 1;
-  }
-  double* Ke = Kval + (size_t)e * D * D;
+  const double* Le = L + (size_t)e * D * D;
+  double v[D][D];
   if (i == j) {
     const double nd = sc->norm_diag;
-    double v[D][D];
-#pragma unroll
-    for (int r = 0; r < D; ++r)
-#pragma unroll
-      for (int s = 0; s < D; ++s) v[r][s] = (s >= r) ? Ke[r * D + s] : Ke[s * D + r];
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
       for (int s = 0; s < D; ++s) {
-        if (fi[r] || fi[s]) v[r][s] = (r == s) ? nd : 0.0;
-        Ke[r * D + s] = v[r][s];
-        if (Kval32) Kval32[(size_t)e * D * D + r * D + s] = (float)v[r][s];
+        v[r][s] = (s >= r) ? Le[r * D + s] : Le[s * D + r];
+        if (((mi >> r) & 1) || ((mi >> s) & 1)) v[r][s] = (r == s) ? nd : 0.0;
       }
-    return;
+  } else {
+    const double* Lt = L + (size_t)tpos[e] * D * D;
+    double a[D][D], b[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        a[r][s] = Le[r * D + s];
+        b[r][s] = Lt[r * D + s];
+      }
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        v[r][s] = i < j ? 0.5 * a[r][s] + 0.5 * b[s][r] : 0.5 * b[s][r] + 0.5 * a[r][s];
+        if (((mi >> r) & 1) || ((mj >> s) & 1)) v[r][s] = 0.0;
+      }
   }
-  const int t = tpos[e];
-  double* Kt = Kval + (size_t)t * D * D;
-  double a[D][D], b[D][D];
+  double* Ke = Kval + (size_t)e * D * D;
 #pragma unroll
   for (int r = 0; r < D; ++r)
 #pragma unroll
-    for (int s = 0; s < D; ++s) {
-      a[r][s] = Ke[r * D + s];
-      b[r][s] = Kt[r * D + s];
-    }
+    for (int s = 0; s < D; ++s) Ke[r * D + s] = v[r][s];
+  if (Kval32) {
+    float* K32 = Kval32 + (size_t)e * D * D;
 #pragma unroll
-  for (int r = 0; r < D; ++r)
+    for (int r = 0; r < D; ++r)
 #pragma unroll
-    for (int s = 0; s < D; ++s) {
-      double v = 0.5 * a[r][s] + 0.5 * b[s][r];
-      if (fi[r] || fj[s]) v = 0.0;
-      Ke[r * D + s] = v;
-      Kt[s * D + r] = v;
-      if (Kval32) {
-        Kval32[(size_t)e * D * D + r * D + s] = (float)v;
-        Kval32[(size_t)t * D * D + s * D + r] = (float)v;
-      }
-    }
+      for (int s = 0; s < D; ++s) K32[r * D + s] = (float)v[r][s];
+  }
 }
 
 // Reference upper-triangular 1-based CSR (IK/JK/K_global) from the BSR.
@@ -744,7 +735,7 @@ int launch_assemble(Context* c, int want32) {
   const int nbw = (N + kWaves - 1) / kWaves;
   const int tb = 256;
   const int nbn = (N + tb - 1) / tb;
-  const int nbu = (c->nup + tb - 1) / tb;
+  const int nbk = (int)((c->nnzb + tb - 1) / tb);
   if (nbn > c->n_partials) {
     set_error("partials buffer too small");
     return -1;
@@ -766,45 +757,38 @@ int launch_assemble(Context* c, int want32) {
     if (lds_E > 0)
       tangent_lds_kernel<3><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                        c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                                       N, lds_E, nbm, c->Kval, c->diag_sq);
+                                                                       N, lds_E, nbm, c->Kloc, c->diag_sq);
     else if (c->max_nb <= 64)
       tangent_lanes_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                  c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                                 N, c->Kval, c->diag_sq);
+                                                                 N, c->Kloc, c->diag_sq);
     else
       tangent_kernel<3><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                            c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                           N, c->Kval, c->diag_sq);
+                                                           N, c->Kloc, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     fixed_dofs_kernel<3><<<nbn, tb, 0, c->stream>>>(c->cons, c->damage_visual, c->floating, N, c->fixm);
-    symmetrize_bc_kernel<3><<<nbu, tb, 0, c->stream>>>(c->uidx, c->nup, c->brow, c->conns, c->tpos, c->fixm,
-                                                      c->sc, c->Kval, nullptr);
+    symmetrize_bc_kernel<3><<<nbk, tb, 0, c->stream>>>((int)c->nnzb, c->brow, c->conns, c->tpos, c->fixm, c->sc,
+                                                      c->Kloc, c->Kval, k32);
   } else {
     if (lds_E > 0)
       tangent_lds_kernel<2><<<nbw, 64 * kWaves, lds_bytes, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                        c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                                       N, lds_E, nbm, c->Kval, c->diag_sq);
+                                                                       N, lds_E, nbm, c->Kloc, c->diag_sq);
     else if (c->max_nb <= 64)
       tangent_lanes_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                                  c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                                 N, c->Kval, c->diag_sq);
+                                                                 N, c->Kloc, c->diag_sq);
     else
       tangent_kernel<2><<<nbw, 64 * kWaves, 0, c->stream>>>(c->bptr, c->nb0, c->p2, c->cs, c->bdamage,
                                                            c->cs_sum, c->Kn, c->Tv, c->cptr, c->conns,
-                                                           N, c->Kval, c->diag_sq);
+                                                           N, c->Kloc, c->diag_sq);
     norm_diag_kernel<<<nbn, tb, 0, c->stream>>>(c->diag_sq, N, c->partials, c->sc);
     norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
     fixed_dofs_kernel<2><<<nbn, tb, 0, c->stream>>>(c->cons, c->damage_visual, c->floating, N, c->fixm);
-    symmetrize_bc_kernel<2><<<nbu, tb, 0, c->stream>>>(c->uidx, c->nup, c->brow, c->conns, c->tpos, c->fixm,
-                                                      c->sc, c->Kval, nullptr);
-  }
-  // the FP32 copy in one coalesced pass over the finished FP64 values (written
-  // inside the symmetrisation, its transposed half was scattered: 388 vs 244 us
-  // on SC 40^3)
-  if (k32) {
-    const int64_t nv = c->nnzb * c->dim * c->dim;
-    f64_to_f32_kernel<<<(unsigned)std::min<int64_t>((nv + tb - 1) / tb, 8192), tb, 0, c->stream>>>(c->Kval, k32, nv);
+    symmetrize_bc_kernel<2><<<nbk, tb, 0, c->stream>>>((int)c->nnzb, c->brow, c->conns, c->tpos, c->fixm, c->sc,
+                                                      c->Kloc, c->Kval, k32);
   }
   LPMG_HIP(hipGetLastError());
   c->assembled64 = true;

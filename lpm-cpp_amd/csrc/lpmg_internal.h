@@ -135,11 +135,10 @@ struct Context {
   // conns / BSR pattern
   int *cptr = nullptr, *conns = nullptr, *ncl = nullptr, *tpos = nullptr, *brow = nullptr,
       *dpos = nullptr;
-  int* uidx = nullptr;      // blocks (i,j) with j >= i, row by row (symmetrisation's work list)
-  int nup = 0;
   int* fixm = nullptr;      // per particle: bit k set when DOF k is eliminated (assembly scratch)
   int64_t* Kp = nullptr;
   double* Kval = nullptr;   // nnzb*b*b
+  double* Kloc = nullptr;   // nnzb*b*b, the tangent's blocks before symmetrisation
   float* Kval32 = nullptr;  // nnzb*b*b (mixed precision)
   double* diag_sq = nullptr;
   // nonlocal neighbours

@@ -443,7 +443,7 @@ int64_t estimate_bytes(const lpmg_request* r) {
   const double n_partials = std::max(std::ceil(N / 256), 4096.0) + std::ceil(N / 8) + 4096 + 64;
   double b = N * (680.0 + 92.0 * dim) + 16.0 * n_partials;     // particle and solver arrays, partials
   b += NB * 120.0;                                              // bond arrays
-  b += nnzb * (14.0 + DD * 8.0 + (mixed ? DD * 4.0 : 0.0));    // conns, BSR values
+  b += nnzb * (12.0 + DD * 16.0 + (mixed ? DD * 4.0 : 0.0));   // conns, BSR values (tangent + symmetrised)
   b += N * 48.0;                                                // setup scratch
   for (int g = 0; g < r->n_groups; ++g) b += 4.0 * std::max(1, (int)r->group_size[g]) + 256;
   if (r->nonlocal_ratio > 0 && r->n_params > 0) {

@@ -353,16 +353,6 @@ __global__ void kp_incr_kernel(const int* __restrict__ ncl, int N, int d, int64_
   if (i < N) inc[i] = (int64_t)d * d * ncl[i] - (d == 2 ? 1 : 3);
 }
 
-// Row i's blocks (i,j) with j >= i -- from its diagonal block to the row's end
-// -- at uoff[i] of the upper list.
-__global__ void upper_list_kernel(const int* __restrict__ cptr, const int* __restrict__ dpos,
-                                  const int* __restrict__ uoff, int N, int* __restrict__ uidx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const int d = dpos[i], n = cptr[i + 1] - d, o = uoff[i];
-  for (int t = 0; t < n; ++t) uidx[o + t] = d + t;
-}
-
 // Block-row index and transpose position of every BSR block.
 __global__ void bsr_aux_kernel(const int* __restrict__ cptr, const int* __restrict__ conns, int N,
                                int* __restrict__ brow, int* __restrict__ tpos, int* __restrict__ err) {
@@ -585,19 +575,7 @@ int build_topology(Context* c) {
   bsr_aux_kernel<<<nb, tb, 0, c->stream>>>(c->cptr, c->conns, N, c->brow, c->tpos, c->err_flag);
   LPMG_HIP(hipGetLastError());
   if (check_err(c, "BSR transpose lookup")) return -1;
-  {
-    // ncl[i] counts row i's connections j >= i: their blocks run from the
-    // diagonal block to the row's end
-    int* uoff = nullptr;
-    LPMG_HIP(hipMallocAsync((void**)&uoff, sizeof(int) * (N + 1), c->stream));
-    int nup = 0;
-    if (int rc = exclusive_scan_i32(c, c->ncl, uoff, N, &nup)) return rc;
-    c->nup = nup;
-    if (dalloc(c, &c->uidx, std::max(nup, 1)) || dalloc(c, &c->fixm, N)) return -3;
-    upper_list_kernel<<<nb, tb, 0, c->stream>>>(c->cptr, c->dpos, uoff, N, c->uidx);
-    LPMG_HIP(hipGetLastError());
-    LPMG_HIP(hipFreeAsync(uoff, c->stream));
-  }
+  if (dalloc(c, &c->fixm, N)) return -3;
   int64_t* inc = nullptr;
   LPMG_HIP(hipMallocAsync((void**)&inc, sizeof(int64_t) * N, c->stream));
   if (dalloc(c, &c->Kp, N + 1)) return -3;
