@@ -16,7 +16,10 @@
 // cblas_dgemm for the 3x3*3x1 el_mapping product.
 //
 // How each part is pinned (tests/test_oracle_*.py):
-//   unit cell / KnTv          closed form of the el_mapping algebra
+//   unit cell data            el_mapping, cutoffs, volume, nneighbors bit for bit
+//                             against tests/golden/unit_cells.json, evaluated from
+//                             the reference header's expressions
+//   KnTv                      closed form of the el_mapping algebra
 //   bonds, layers, conns      brute-force O(N^2) search; shell-count invariants
 //   geometry/force/stress     rigid motion -> zero force; affine stretch closed
 //                             form; global force balance; pair antisymmetry
@@ -26,7 +29,12 @@
 //   CSR indices               closed form from conns (row starts, columns)
 //   CG / Newton               scipy sparse direct solve on small systems;
 //                             residual of the converged state
-//   J2 return / damage        textbook radial return; energy closed form
+//   J2 return                 textbook radial return
+//   damage laws               ElasticDamage rate law gated both ways on closed-form
+//                             energies; damage += Ddot_nonlocal, threshold (met with
+//                             equality on exact binary sums) and the 1e-3 return;
+//                             J2's zero threshold; bdamage = max(bdamage, D_i, D_j)
+//                             and the (1 - bdamage) force scale on hand-set damages
 //   CT bond exclusion         hand-placed bonds across a slit / a hole rim with
 //                             known outcome; brute force vs the grid search
 // CG iteration counts are parity unpinned (MKL's internals are closed).
