@@ -184,6 +184,12 @@ int lpmg_store_state(lpmg_handle h);
 int lpmg_reset_state(lpmg_handle h, int reset_xyz);
 /* Assembly::searchNonlocalNeighbors (assembly.h:117-130) */
 int lpmg_search_nonlocal(lpmg_handle h, double cutoff_ratio);
+/* Particle::neighbors_nonlocal (particle.h:36) as the last lpmg_search_nonlocal
+ * left them: nlptr[n+1] (nlptr[n] = total entries) and nlidx[total], each
+ * particle's list in id order with the particle itself included, as the
+ * reference's loop over pt_sys pushes them (assembly.h:120-127).  Either pointer
+ * may be NULL (fetch nlptr first to size nlidx); -1 before any search. */
+int lpmg_export_nonlocal(lpmg_handle h, int32_t* nlptr, int32_t* nlidx);
 
 /* A particle group for boundary conditions (DispBC/ForceBC::group, boundary.h:12,31);
  * returns the group id. */

@@ -670,6 +670,15 @@ int lpmg_search_nonlocal(lpmg_handle h, double ratio) {
   return search_nonlocal(&h->c, ratio);
 }
 
+int lpmg_export_nonlocal(lpmg_handle h, int32_t* nlptr, int32_t* nlidx) {
+  CHECK_HANDLE(h);
+  Context* c = &h->c;
+  if (!c->nlptr || !c->nlidx) { set_error("no nonlocal neighbours: lpmg_search_nonlocal first"); return -1; }
+  if (nlptr) API_HIP(copy_sync(c, nlptr, c->nlptr, sizeof(int) * (c->N + 1), hipMemcpyDeviceToHost));
+  if (nlidx && c->n_nl > 0) API_HIP(copy_sync(c, nlidx, c->nlidx, sizeof(int) * c->n_nl, hipMemcpyDeviceToHost));
+  return 0;
+}
+
 int lpmg_define_group(lpmg_handle h, int n, const int32_t* idx) {
   CHECK_HANDLE(h);
   Context* c = &h->c;

@@ -30,6 +30,7 @@ EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "
            "lpmg_set_types", "lpmg_set_material", "lpmg_forward", "lpmg_update_rr", "lpmg_assemble",
            "lpmg_spmv", "lpmg_solve_linear", "lpmg_newton", "lpmg_update_state_var",
            "lpmg_update_static_damage", "lpmg_update_fatigue_damage", "lpmg_store_state", "lpmg_reset_state", "lpmg_search_nonlocal",
+           "lpmg_export_nonlocal",
            "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_restart", "lpmg_field_len",
            "lpmg_get_field", "lpmg_set_field", "lpmg_profile", "lpmg_profile_read", "lpmg_time_kernel",
            "lpmg_synchronize", "lpmg_request_bytes", "lpmg_server_create", "lpmg_server_submit",
@@ -70,6 +71,7 @@ def lib():
         L.lpmg_sizes.argtypes = [vp, i64p]
         L.lpmg_export_bonds.argtypes = [vp, i32p, i32p, i32p]
         L.lpmg_export_conns.argtypes = [vp, i32p, i32p, i32p]
+        L.lpmg_export_nonlocal.argtypes = [vp, i32p, i32p]
         L.lpmg_export_csr.argtypes = [vp, i64p, i64p, i64p, dp]
         L.lpmg_set_types.argtypes = [vp, i32p]
         L.lpmg_set_material.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int, dp]
@@ -261,6 +263,14 @@ class System:
 
     def search_nonlocal(self, ratio):
         self._chk(lib().lpmg_search_nonlocal(self.h, float(ratio)))
+
+    def nonlocal_neighbors(self):
+        """(nlptr, nlidx): Particle::neighbors_nonlocal of every particle, id order."""
+        nlptr = np.zeros(self.N + 1, np.int32)
+        self._chk(lib().lpmg_export_nonlocal(self.h, _p(nlptr, ctypes.c_int32), None))
+        nlidx = np.zeros(max(int(nlptr[-1]), 1), np.int32)
+        self._chk(lib().lpmg_export_nonlocal(self.h, None, _p(nlidx, ctypes.c_int32)))
+        return nlptr, nlidx[:int(nlptr[-1])]
 
     def synchronize(self):
         self._chk(lib().lpmg_synchronize(self.h))

@@ -1210,6 +1210,14 @@ int lpmo_store_state(void* h) { store_state(static_cast<Ctx*>(h)); return 0; }
 int lpmo_reset_state(void* h, int reset_xyz) { reset_state(static_cast<Ctx*>(h), reset_xyz); return 0; }
 int lpmo_search_nonlocal(void* h, double ratio) { search_nonlocal(static_cast<Ctx*>(h), ratio); return 0; }
 
+int lpmo_export_nonlocal(void* h, int32_t* nlptr, int32_t* nlidx) {
+  Ctx* c = static_cast<Ctx*>(h);
+  if (c->nlptr.empty()) { g_err = "no nonlocal neighbours"; return -1; }
+  if (nlptr) std::copy(c->nlptr.begin(), c->nlptr.end(), nlptr);
+  if (nlidx) std::copy(c->nlidx.begin(), c->nlidx.end(), nlidx);
+  return 0;
+}
+
 int lpmo_define_group(void* h, int n, const int32_t* idx) {
   Ctx* c = static_cast<Ctx*>(h);
   Group g;

@@ -48,7 +48,7 @@ def lib():
                      "lpmo_spmv", "lpmo_solve_linear", "lpmo_newton", "lpmo_update_state_var",
                      "lpmo_update_static_damage", "lpmo_update_fatigue_damage", "lpmo_store_state",
                      "lpmo_reset_state",
-                     "lpmo_search_nonlocal", "lpmo_define_group", "lpmo_disp_bc", "lpmo_force_bc",
+                     "lpmo_search_nonlocal", "lpmo_export_nonlocal", "lpmo_define_group", "lpmo_disp_bc", "lpmo_force_bc",
                      "lpmo_solve_step", "lpmo_get_field", "lpmo_set_field", "lpmo_unit_cell"):
             getattr(L, name).restype = ctypes.c_int
         L.lpmo_set_material.argtypes = [vp, ctypes.c_int, i32p, ctypes.c_int, dp]
@@ -58,6 +58,7 @@ def lib():
         L.lpmo_force_bc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                     ctypes.c_double]
         L.lpmo_search_nonlocal.argtypes = [vp, ctypes.c_double]
+        L.lpmo_export_nonlocal.argtypes = [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]
         L.lpmo_update_fatigue_damage.argtypes = [vp, ctypes.c_int, ctypes.c_double]
         L.lpmo_unit_cell.argtypes = [ctypes.c_int, ctypes.c_double, dp]
         L.lpmo_set_threads.argtypes = [ctypes.c_int]
@@ -275,6 +276,14 @@ class Oracle:
 
     def search_nonlocal(self, ratio):
         lib().lpmo_search_nonlocal(self.h, float(ratio))
+
+    def nonlocal_neighbors(self):
+        """(nlptr, nlidx) of the last search: every particle's list in id order."""
+        nlptr = np.zeros(self.N + 1, np.int32)
+        self._chk(lib().lpmo_export_nonlocal(self.h, _p(nlptr, ctypes.c_int32), None))
+        nlidx = np.zeros(max(int(nlptr[-1]), 1), np.int32)
+        self._chk(lib().lpmo_export_nonlocal(self.h, None, _p(nlidx, ctypes.c_int32)))
+        return nlptr, nlidx[:int(nlptr[-1])]
 
     # -- fields
     def get(self, name):

@@ -289,3 +289,31 @@ def test_elastic_particles_keep_their_bond_damage():
     o.set("BDAMAGE", bd0)
     o.forward()
     np.testing.assert_array_equal(o.get("BDAMAGE"), bd0)
+
+
+# --------------------------------------------------------- nonlocal neighbour search
+
+@pytest.mark.parametrize("L,gen", [(lat.HEXAGON2D, lambda: lat.hex_plate(16, 16, R)),
+                                   (lat.BCC3D, lambda: lat.bcc_block(5, 5, 5, R))], ids=["hex", "bcc"])
+def test_oracle_nonlocal_search_is_the_reference_loop(L, gen):
+    # Assembly::searchNonlocalNeighbors (assembly.h:117-130): for every p1,
+    # every p2 of pt_sys (p1 itself too) with distanceTo < ratio * p1's
+    # nonlocal_L, pushed in pt_sys order.  distanceTo is sqrt of the pow(d, 2)
+    # sum (particle.h:277-280); pow(d, 2) rounds as d * d.  Checked against a
+    # numpy loop with per-particle lengths: a list sorted by distance, without
+    # self, or using p2's length fails.
+    xyz = gen() + np.random.default_rng(2).uniform(-0.005, 0.005, (len(gen()), 3)) * (
+        [1, 1, 0] if L == lat.HEXAGON2D else 1)
+    o = Oracle(L, R, lat.ELASTIC_DAMAGE, xyz)
+    n = o.N
+    lens = np.where(np.arange(n) % 2 == 0, 0.25, 0.4)
+    for nl in (0.25, 0.4):
+        idx = np.nonzero(lens == nl)[0].astype(np.int32)
+        o.set_material(P.MAT_DAMAGE, [nl, 1 if L < 2 else 0, 200e3, 0.3, 1e-3, 1.0, 1.0, 1.0], idx=idx)
+    o.search_nonlocal(1.5)
+    ptr, idx = o.nonlocal_neighbors()
+    d = xyz[:, None, :] - xyz[None, :, :]
+    dist = np.sqrt(d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1] + d[..., 2] * d[..., 2])
+    for i in range(n):
+        want = np.nonzero(dist[i] < 1.5 * lens[i])[0]
+        np.testing.assert_array_equal(idx[ptr[i]:ptr[i + 1]], want)
