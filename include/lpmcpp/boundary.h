@@ -1,0 +1,4 @@
+// boundary.h -- LPM-CPP's include/boundary.h for drivers built against the GPU adapter;
+// its classes are those of lpm_gpu.hpp (see lpm.h in this directory).
+#pragma once
+#include "lpm.h"

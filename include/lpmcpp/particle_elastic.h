@@ -1,0 +1,4 @@
+// particle_elastic.h -- LPM-CPP's include/particle_elastic.h for drivers built against the GPU adapter;
+// its classes are those of lpm_gpu.hpp (see lpm.h in this directory).
+#pragma once
+#include "lpm.h"

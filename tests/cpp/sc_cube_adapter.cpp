@@ -1,81 +1,98 @@
-// Drives the GPU path through the C++ adapter (include/lpm_gpu.hpp) the way
-// LPM-CPP's examples drive the reference (examples/ex1_isoelasticity_3d_sc.cpp:
-// build a lattice, set the material, define BC groups and load steps, then
-// SolverStatic::solveProblem).  Prints the final displacement field.
-#include <cmath>
-#include <cstdio>
-#include <cstdlib>
-#include <string>
+// A driver written the way LPM-CPP's examples drive the reference
+// (examples/ex1_isoelasticity_3d_sc.cpp): only the include path differs
+// (include/lpmcpp instead of LPM-CPP's include/).  It builds a simple-cubic
+// cuboid with createCuboidSC3D, walks pt_sys to form the boundary groups and
+// set every particle's material through dynamic_cast, loads the top face in
+// displacement steps through DispBC over Particle* groups, solves with
+// SolverStatic::solveProblem and reads the result back from the particle views
+// and Stiffness's public CSR arrays.
+//
+// Usage: sc_cube_adapter <n_steps> <out_dir>; writes u.bin (N x 3 displacements
+// from p->xyz - p->xyz_initial), pin.bin (p->Pin), Kp.bin, IK.bin, JK.bin,
+// K.bin (solv.stiffness.*) and the dump file the solver appends per step.
+#include "lpm.h"
+#include "particle.h"
+#include "assembly.h"
+#include "utilities.h"
+#include "stiffness.h"
+#include "load_step.h"
+#include "solver_static.h"
 
-#include "lpm_gpu.hpp"
-
-using namespace lpm_gpu;
+template <typename T>
+static void dump(const std::string& path, const T* data, size_t n) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  std::fwrite(data, sizeof(T), n, f);
+  std::fclose(f);
+}
 
 int main(int argc, char** argv) {
-  const int n = argc > 1 ? std::atoi(argv[1]) : 6;
-  const int n_steps = argc > 2 ? std::atoi(argv[2]) : 3;
-  const double radius = 0.1, total = 0.05;
+  if (argc < 3) return 2;
+  const int n_steps = std::atoi(argv[1]);
+  const std::string out = argv[2];
+
+  const int n_layer = 2;
+  double radius = 0.1;
   UnitCell cell(LatticeType::SimpleCubic3D, radius);
-  std::vector<std::array<double, 3>> xyz;
-  for (int k = 0; k < n; ++k)
-    for (int j = 0; j < n; ++j)
-      for (int i = 0; i < n; ++i) xyz.push_back({2 * radius * i, 2 * radius * j, 2 * radius * k});
-  std::array<double, 6> box{0, 2 * radius * (n - 1), 0, 2 * radius * (n - 1), 0, 2 * radius * (n - 1)};
-  Assembly<2> ass(xyz, box, cell, ParticleType::Elastic);
-  ass.setElastic(0.0, false, 70e3, 0.33, 1.0);
-  std::vector<int> top, bottom;
-  for (int p = 0; p < (int)xyz.size(); ++p) {
-    if (xyz[p][2] < 0.5 * radius) bottom.push_back(p);
-    if (xyz[p][2] > box[5] - 0.5 * radius) top.push_back(p);
+  int eulerflag = 0;
+  double angles[] = {PI / 180.0 * 0.0, PI / 180.0 * 0.0, PI / 180.0 * 0.0};
+  double* R_matrix = createRMatrix(eulerflag, angles);
+  std::array<double, 2 * NDIM> box{0.0, 1.0, -0.1, 1.0, -0.1, 1.6};
+  std::vector<std::array<double, NDIM>> sc_xyz = createCuboidSC3D(box, cell, R_matrix);
+  Assembly<n_layer> pt_ass{sc_xyz, box, cell, ParticleType::Elastic};
+
+  bool is_plane_stress = false;
+  double E0 = 69e3, mu0 = 0.3, critical_bstrain = 1;
+  double step_size = 0.002;
+  int max_iter = 30, start_index = 0;
+  double tol_iter = 1e-7;
+  int undamaged_pt_type = -1;
+
+  std::vector<Particle<n_layer>*> top_group, bottom_group;
+  for (Particle<n_layer>* p1 : pt_ass.pt_sys) {
+    if (p1->xyz[2] > box[5] - 2 * radius) {
+      top_group.push_back(p1);
+      p1->type = 1;
+    }
+    if (p1->xyz[2] < box[4] + 2 * radius) {
+      bottom_group.push_back(p1);
+      p1->type = 2;
+    }
+    ParticleElastic<n_layer>* elpt = dynamic_cast<ParticleElastic<n_layer>*>(p1);
+    elpt->setParticleProperty(0.0, is_plane_stress, E0, mu0, critical_bstrain);
   }
-  std::vector<LoadStep<2>> load;
-  for (int s = 0; s < n_steps; ++s) {
-    LoadStep<2> step;
-    step.dispBCs.push_back(DispBC<2>(bottom, LoadMode::Relative, 'x', 0.0));
-    step.dispBCs.push_back(DispBC<2>(bottom, LoadMode::Relative, 'y', 0.0));
-    step.dispBCs.push_back(DispBC<2>(bottom, LoadMode::Relative, 'z', 0.0));
-    step.dispBCs.push_back(DispBC<2>(top, LoadMode::Relative, 'z', total / n_steps));
+
+  std::vector<LoadStep<n_layer>> load;
+  for (int i = 0; i < n_steps; i++) {
+    LoadStep<n_layer> step;
+    step.dispBCs.push_back(DispBC<n_layer>(bottom_group, LoadMode::Relative, 'x', 0.0));
+    step.dispBCs.push_back(DispBC<n_layer>(bottom_group, LoadMode::Relative, 'y', 0.0));
+    step.dispBCs.push_back(DispBC<n_layer>(bottom_group, LoadMode::Relative, 'z', 0.0));
+    step.dispBCs.push_back(DispBC<n_layer>(top_group, LoadMode::Relative, 'z', step_size));
     load.push_back(step);
   }
-  if (argc > 3 && std::string(argv[3]) == "serve") {
-    // the same solve as a request to the serving runtime, next to a second copy
-    std::vector<double> flat, params{0.0, 0, 70e3, 0.33, 1.0}, out0(3 * xyz.size()), out1(3 * xyz.size());
-    for (auto& q : xyz) flat.insert(flat.end(), q.begin(), q.end());
-    std::vector<int32_t> gsize{(int32_t)bottom.size(), (int32_t)top.size()}, gidx(bottom.begin(), bottom.end());
-    gidx.insert(gidx.end(), top.begin(), top.end());
-    lpmg_bc bcs[4] = {{0, 0, LPMG_RELATIVE, 'x', {0, 0, 0}}, {0, 0, LPMG_RELATIVE, 'y', {0, 0, 0}},
-                      {0, 0, LPMG_RELATIVE, 'z', {0, 0, 0}}, {0, 1, LPMG_RELATIVE, 'z', {total / n_steps, 0, 0}}};
-    lpmg_request r{};
-    r.lattice = LPMG_SC3D; r.radius = radius; r.ptype = LPMG_ELASTIC;
-    r.n = (int)xyz.size(); r.xyz = flat.data();
-    r.model = 0; r.n_params = 5; r.params = params.data();
-    r.n_groups = 2; r.group_size = gsize.data(); r.group_idx = gidx.data();
-    r.n_bc = 4; r.bc = bcs; r.n_steps = n_steps;
-    r.undamaged = -1; r.max_iter = 30; r.tol = 1e-11; r.max_halvings = 8;
-    const double o[8] = {1, 1e-24, 0.0, 0, 0, 16, 0, 0};
-    for (int k = 0; k < 8; ++k) r.opts[k] = o[k];
-    lpmg_out_field f0{LPMG_F_XYZ, (int64_t)out0.size(), out0.data()}, f1{LPMG_F_XYZ, (int64_t)out1.size(), out1.data()};
-    Server srv(2);
-    r.n_out = 1; r.out = &f0;
-    const int64_t t0 = srv.submit(r);
-    r.out = &f1;
-    const int64_t t1 = srv.submit(r);
-    srv.wait(t0);
-    srv.wait(t1);
-    for (size_t k = 0; k < out0.size(); ++k)
-      if (out0[k] != out1[k]) { std::fprintf(stderr, "served copies differ\n"); return 1; }
-    for (size_t p = 0; p < xyz.size(); ++p)
-      std::printf("%.17g %.17g %.17g\n", out0[3 * p] - xyz[p][0], out0[3 * p + 1] - xyz[p][1], out0[3 * p + 2] - xyz[p][2]);
-    return 0;
-  }
-  ass.updateGeometry();
-  ass.updateForceState();
-  SolverStatic<2> solv(-1, ass, StiffnessMode::Analytical, SolverMode::CG, "unused.dump", 30, 1e-11);
-  solv.opts = {1, 1e-24, 0.0, 0, 0, 16, 0, 0};
-  int start = 0;
-  solv.solveProblem(load, start);
-  auto now = ass.positions();
-  for (size_t p = 0; p < xyz.size(); ++p)
-    std::printf("%.17g %.17g %.17g\n", now[p][0] - xyz[p][0], now[p][1] - xyz[p][1], now[p][2] - xyz[p][2]);
+
+  pt_ass.updateGeometry();
+  pt_ass.updateForceState();
+
+  SolverStatic<n_layer> solv{undamaged_pt_type, pt_ass, StiffnessMode::Analytical, SolverMode::CG,
+                             out + "/sc_position_3d.dump", max_iter, tol_iter};
+  solv.solveProblem(load, start_index);
+
+  const size_t n = pt_ass.pt_sys.size();
+  std::vector<double> u(3 * n), pin(3 * n);
+  for (Particle<n_layer>* p : pt_ass.pt_sys)
+    for (int k = 0; k < 3; ++k) {
+      u[3 * p->id + k] = p->xyz[k] - p->xyz_initial[k];
+      pin[3 * p->id + k] = p->Pin[k];
+    }
+  dump(out + "/u.bin", u.data(), u.size());
+  dump(out + "/pin.bin", pin.data(), pin.size());
+  const int dim = pt_ass.pt_sys[0]->cell.dim;
+  const long long nnz = solv.stiffness.K_pointer[n];
+  dump(out + "/Kp.bin", solv.stiffness.K_pointer, n + 1);
+  dump(out + "/IK.bin", solv.stiffness.IK, dim * n + 1);
+  dump(out + "/JK.bin", solv.stiffness.JK, nnz);
+  dump(out + "/K.bin", solv.stiffness.K_global, nnz);
+  std::printf("%zu %lld %d\n", n, nnz, start_index);
   return 0;
 }

@@ -1,8 +1,16 @@
-"""The C++ adapter (include/lpm_gpu.hpp) keeps LPM-CPP's class names over the C
-ABI: it compiles as C++17 against the header (CPU), and on the GPU it yields the
-same displacements as the same problem driven through the Python binding --
-through SolverStatic::solveProblem, as two concurrent requests to the serving
-runtime (lpm_gpu::Server), and through SolverFatigue::solveProblemCyclic."""
+"""The C++ adapter (include/lpm_gpu.hpp behind the reference-named headers of
+include/lpmcpp) is a drop-in for LPM-CPP's drivers.
+
+CPU: the drivers under tests/cpp compile against it; LPM-CPP's own example
+drivers compile unmodified against it (when the reference tree is present);
+the lattice generators of include/lpmcpp/utilities.h equal lpmg.lattice bit for
+bit for rotated lattices and reduce to the closed forms for the identity.
+GPU: a driver in the reference's idiom (pt_sys loops, dynamic_cast,
+setParticleProperty, DispBC over Particle* groups, SolverStatic::solveProblem,
+Stiffness's public CSR arrays) yields the same bits as the same problem driven
+through the Python binding; so do two concurrent requests through
+lpm_gpu::Server and SolverFatigue::solveProblemCyclic."""
+import math
 import os
 import subprocess
 
@@ -10,41 +18,154 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "tests", "cpp", "sc_cube_adapter.cpp")
-FATIGUE_SRC = os.path.join(ROOT, "tests", "cpp", "fatigue_adapter.cpp")
+CPP = os.path.join(ROOT, "tests", "cpp")
 PKG = os.path.join(ROOT, "lpm-cpp_amd")
+REF_EXAMPLES = "/root/reference/examples"
 
 
-def _compile(out, src=SRC):
-    subprocess.check_call(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
-                           src, "-o", out, "-L", PKG, "-llpmg", "-Wl,-rpath," + PKG])
-
-
-def test_adapter_compiles(tmp_path):
+def _lib():
     import lpmg
 
     if not os.path.exists(lpmg.LIB_PATH):
         subprocess.check_call(["make", "-j8", "-C", lpmg.CSRC])
-    _compile(str(tmp_path / "adapter"))
-    _compile(str(tmp_path / "fatigue_adapter"), FATIGUE_SRC)
+
+
+def _compile(out, name, extra=()):
+    _lib()
+    subprocess.check_call(["g++", "-std=c++17", "-O2", "-Wall", "-Werror", "-fopenmp",
+                           "-I", os.path.join(ROOT, "include", "lpmcpp"), "-I", os.path.join(ROOT, "include"),
+                           *extra, os.path.join(CPP, name + ".cpp"), "-o", out, "-L", PKG, "-llpmg",
+                           "-Wl,-rpath," + PKG])
+    return out
+
+
+@pytest.mark.parametrize("name", ["sc_cube_adapter", "fatigue_adapter", "lattice_gen", "server_adapter"])
+def test_drivers_compile(tmp_path, name):
+    _compile(str(tmp_path / name), name)
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_EXAMPLES), reason="the reference tree is not present")
+@pytest.mark.parametrize("example", ["ex1_isoelasticity_3d_sc", "ex1_isoelasticity_2d_hex", "ex2_polycrystal_3d_bcc",
+                                     "ex3_singlecrystal_3d_fcc", "ex4_isoelasticity_2d_crack",
+                                     "ex5_isoelasticity_2d_fatigue_gage", "ex6_isoelasticity_2d_CT_fatigue"])
+def test_reference_examples_compile_unmodified(tmp_path, example):
+    # the reference's own drivers, read where they lie, compiled (not linked:
+    # they define run() for src/lpmcpp.cpp) against the adapter's headers
+    subprocess.check_call(["g++", "-std=c++17", "-fopenmp", "-fsyntax-only",
+                           "-I", os.path.join(ROOT, "include", "lpmcpp"), "-I", os.path.join(ROOT, "include"),
+                           os.path.join(REF_EXAMPLES, example + ".cpp")])
+
+
+def _gen(exe, kind, flag, angles, radius, box):
+    out = subprocess.check_output([exe, kind, str(flag), *map(repr, angles), repr(radius), *map(repr, box)], text=True)
+    lines = out.strip().splitlines()
+    R = np.array([float.fromhex(v) for v in lines[0].split()])
+    pts = np.array([[float.fromhex(v) for v in ln.split()] for ln in lines[1:]]).reshape(-1, 3)
+    return R, pts
+
+
+GEN_CASES = [("sq", 0, (0, 0, 0), 0.1, (0, 2, 0, 1.5, 0, 0)), ("sq", 0, (0, 0, 17), 0.1, (0, 2, 0, 1.5, 0, 0)),
+             ("hex", 0, (0, 0, 0), 0.1, (0, 2, 0, 1.5, 0, 0)), ("hex", 0, (0, 0, 30), 0.12, (0, 2, 0, 2, 0, 0)),
+             ("hex", 2, (10, 0, 25), 0.1, (0, 1.5, 0, 1.5, 0, 0)),
+             ("sc", 0, (0, 0, 0), 0.1, (0, 1, -0.1, 1, -0.1, 1.4)), ("sc", 0, (15, 20, 35), 0.1, (0, 1, 0, 1, 0, 1)),
+             ("sc", 1, (40, 30, 110), 0.1, (0, 1, 0, 1, 0, 1)), ("fcc", 0, (0, 0, 0), 0.1, (0, 1, 0, 1, 0, 1)),
+             ("fcc", 2, (30, 45, 60), 0.1, (0, 1, 0, 1, 0, 1))]
+
+
+@pytest.mark.parametrize("kind,flag,angles,radius,box", GEN_CASES)
+def test_lattice_generators_match_python(tmp_path, kind, flag, angles, radius, box):
+    from lpmg import lattice as lat
+
+    exe = _compile(str(tmp_path / "lattice_gen"), "lattice_gen")
+    R, pts = _gen(exe, kind, flag, angles, radius, box)
+    Rp = lat.create_rmatrix(flag, [math.pi / 180.0 * a for a in angles])
+    np.testing.assert_array_equal(R, Rp)
+    make = dict(sq=lat.create_plate_sq2d, hex=lat.create_plate_hex2d, sc=lat.create_cuboid_sc3d,
+                fcc=lat.create_cuboid_fcc3d)[kind]
+    np.testing.assert_array_equal(pts, make(list(box), radius, Rp))
+    assert len(pts) > 10
+    # R is a rotation: orthonormal with determinant 1, so a rotated lattice
+    # keeps its nearest-neighbour distance
+    M = R.reshape(3, 3)
+    np.testing.assert_allclose(M @ M.T, np.eye(3), atol=1e-15)
+    assert abs(np.linalg.det(M) - 1) < 1e-14
+    d = np.linalg.norm(pts[:, None, :] - pts[None, :, :], axis=2)
+    d[np.diag_indices_from(d)] = np.inf
+    want = {"sq": 2 * radius, "hex": 2 * radius, "sc": 2 * radius, "fcc": 2 * radius}[kind]
+    np.testing.assert_allclose(d.min(), want, rtol=1e-12)
+
+
+def test_identity_generators_are_the_closed_forms():
+    # R = identity: createCuboidSC3D's x starts at box[0], y and z at -a
+    # (the box diagonal) in steps of 2r; createPlateHEX2D's rows are sqrt(3) r
+    # apart from -a with the even rows shifted by r
+    from lpmg import lattice as lat
+
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, 1.0]
+    p = lat.create_cuboid_sc3d(box, 0.1)
+    a = math.sqrt(3.0)
+    xs = np.unique(p[:, 0])
+    ys = np.unique(p[:, 1])
+    np.testing.assert_array_equal(xs, 0.0 + 0.2 * np.arange(len(xs)))
+    k = np.arange(100)
+    grid = -a + 0.2 * k
+    np.testing.assert_array_equal(ys, grid[(grid >= 0) & (grid <= 1)])
+    h = lat.create_plate_hex2d([0.0, 2.0, 0.0, 2.0, 0.0, 0.0], 0.1)
+    rows = np.unique(h[:, 1])
+    np.testing.assert_allclose(np.diff(rows), 0.2 * math.sqrt(3.0) / 2.0, rtol=1e-12)
+
+
+def _sc_problem():
+    """The problem tests/cpp/sc_cube_adapter.cpp solves, through the Python binding."""
+    from lpmg import lattice as lat
+    from lpmg import problems as P
+
+    box = [0.0, 1.0, -0.1, 1.0, -0.1, 1.6]
+    r = 0.1
+    xyz = lat.create_cuboid_sc3d(box, r, lat.create_rmatrix(0, [0.0, 0.0, 0.0]))
+    z = xyz[:, 2]
+    top = np.nonzero(z > box[5] - 2 * r)[0]
+    bottom = np.nonzero(z < box[4] + 2 * r)[0]
+    types = np.zeros(len(xyz), dtype=np.int32)
+    types[top] = 1
+    types[bottom] = 2
+    step = [("disp", "bottom", P.RELATIVE, "x", 0.0), ("disp", "bottom", P.RELATIVE, "y", 0.0),
+            ("disp", "bottom", P.RELATIVE, "z", 0.0), ("disp", "top", P.RELATIVE, "z", 0.002)]
+    return P.Problem("sc_adapter", lat.SC3D, r, lat.ELASTIC, xyz, (P.MAT_ELASTIC, [0.0, 0, 69e3, 0.3, 1.0]),
+                     dict(bottom=bottom, top=top), [list(step) for _ in range(3)], types=types, undamaged=-1,
+                     max_iter=30, tol=1e-7)
 
 
 @pytest.mark.gpu
-def test_adapter_matches_python_binding(tmp_path):
+def test_reference_idiom_driver_matches_python_binding(tmp_path):
     import lpmg
     from lpmg import problems as P
 
-    exe = str(tmp_path / "adapter")
-    _compile(exe)
-    out = subprocess.check_output([exe, "6", "3"], text=True)
-    u_cpp = np.array([[float(v) for v in line.split()] for line in out.strip().splitlines()])
-    prob = P.sc_cube(n=6, steps=3, total=0.05, E=70e3, nu=0.33)
+    exe = _compile(str(tmp_path / "sc_cube_adapter"), "sc_cube_adapter")
+    out = subprocess.check_output([exe, "3", str(tmp_path)], text=True).split()
+    n, nnz, start = int(out[0]), int(out[1]), int(out[2])
+    assert start == 3
+    prob = _sc_problem()
     s, gids = P.build(lpmg.System, prob)
     s.forward()
-    P.solve_problem(s, gids, P.Problem(**{**prob.__dict__, "tol": 1e-11}),
-                    dict(precond=1, rtol=1e-24, atol=0.0))
+    P.solve_problem(s, gids, prob, dict(precond=1, rtol=1e-12, atol=1e-12))
+    assert s.N == n
     u_py = (s.get("XYZ") - s.get("XYZ_INITIAL")).reshape(-1, 3)
-    assert np.linalg.norm(u_cpp - u_py) <= 1e-12 * np.linalg.norm(u_py)
+    u_cpp = np.fromfile(tmp_path / "u.bin").reshape(-1, 3)
+    np.testing.assert_array_equal(u_cpp, u_py)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "pin.bin").reshape(-1, 3), s.get("PIN").reshape(-1, 3))
+    Kp, IK, JK, K = s.csr()
+    assert nnz == len(K) == Kp[-1]
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "Kp.bin", dtype=np.int64), Kp)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "IK.bin", dtype=np.int64), IK)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "JK.bin", dtype=np.int64), JK)
+    np.testing.assert_array_equal(np.fromfile(tmp_path / "K.bin"), K)
+    # solveProblem appended one dump frame per converged step, in the
+    # reference's writeDump layout (17 columns)
+    text = open(tmp_path / "sc_position_3d.dump").read().splitlines()
+    assert text.count("ITEM: TIMESTEP") == 3
+    first = text.index("ITEM: ATOMS id type x y z dx dy dz s11 s22 s33 s23 s13 s12 damage damage_visual state_var[0] ")
+    assert len(text[first + 1].split()) == 17
 
 
 @pytest.mark.gpu
@@ -52,9 +173,8 @@ def test_adapter_server_matches_python_binding(tmp_path):
     import lpmg
     from lpmg import problems as P
 
-    exe = str(tmp_path / "adapter")
-    _compile(exe)
-    out = subprocess.check_output([exe, "6", "3", "serve"], text=True)
+    exe = _compile(str(tmp_path / "server_adapter"), "server_adapter")
+    out = subprocess.check_output([exe, "6", "3"], text=True)
     u_srv = np.array([[float(v) for v in line.split()] for line in out.strip().splitlines()])
     prob = P.sc_cube(n=6, steps=3, total=0.05, E=70e3, nu=0.33)
     s, gids = P.build(lpmg.System, prob)
@@ -71,8 +191,7 @@ def test_adapter_fatigue_matches_python_binding(tmp_path):
     from lpmg import fatigue as F
     from lpmg import problems as P
 
-    exe = str(tmp_path / "fatigue_adapter")
-    _compile(exe, FATIGUE_SRC)
+    exe = _compile(str(tmp_path / "fatigue_adapter"), "fatigue_adapter")
     n_sim, tau = 3, 1e-4
     prob = P.fatigue_gage(hole=1.0, nonlocal_L=0.6, tau=tau, n_cycles=n_sim * 10000)
     prob.tol = 1e-11
