@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -382,14 +383,17 @@ class Assembly {
     lpmg_destroy(h);
   }
 
-  // updateGeometry and updateForceState are one fused device pass; the
-  // reference always calls them as a pair (assembly.h:132-191, solver.h:58-59, 82-83)
+  // updateGeometry and updateForceState are one fused device pass (geometry
+  // from the current bond damage, then forces with the damage models' bond
+  // damage update); the reference always calls them as a pair
+  // (assembly.h:132-191, solver.h:58-59, 82-83).  The pass runs exactly once
+  // per pair: a second one would see the raised bond damage and treat bonds
+  // of fully damaged particles as broken, which the reference does only at
+  // its next updateGeometry.
   void updateGeometry() { pending_forward_ = true; }
   void updateForceState() {
-    push();
-    check(lpmg_forward(h), "updateForceState");
     pending_forward_ = false;
-    pull();
+    run([&] { return lpmg_forward(h); }, "updateForceState");
   }
   void updateStateVar() { run([&] { return lpmg_update_state_var(h); }, "updateStateVar"); }
   void storeStateVar() { run([&] { return lpmg_store_state(h); }, "storeStateVar"); }
@@ -463,21 +467,24 @@ class Assembly {
     if (t != sent_type_) {
       check(lpmg_set_types(h, t.data()), "types");
       sent_type_ = t;
+      ++sends_[0];
     }
-    send3(LPMG_F_XYZ, &Particle<nlayer>::xyz, sent_xyz_);
-    send3(LPMG_F_PEX, &Particle<nlayer>::Pex, sent_pex_);
+    sends_[1] += send3(LPMG_F_XYZ, &Particle<nlayer>::xyz, sent_xyz_);
+    sends_[2] += send3(LPMG_F_PEX, &Particle<nlayer>::Pex, sent_pex_);
     std::vector<double> d(nparticle);
     for (int i = 0; i < nparticle; ++i) d[i] = pt_sys[i]->damage;
-    if (d != sent_damage_) {
+    if (!same(d, sent_damage_)) {
       check(lpmg_set_field(h, LPMG_F_DAMAGE, d.data()), "damage");
       sent_damage_ = d;
+      ++sends_[3];
     }
     std::vector<double> c(3 * (size_t)nparticle);
     for (int i = 0; i < nparticle; ++i)
       for (int k = 0; k < 3; ++k) c[3 * i + k] = pt_sys[i]->disp_constraint[k];
-    if (c != sent_cons_) {
+    if (!same(c, sent_cons_)) {
       check(lpmg_set_field(h, LPMG_F_CONSTRAINT, c.data()), "disp_constraint");
       sent_cons_ = c;
+      ++sends_[4];
     }
     if (pending_forward_) {
       check(lpmg_forward(h), "updateGeometry/updateForceState");
@@ -555,6 +562,10 @@ class Assembly {
     pending_[params].push_back(idx);
   }
 
+  // no reference counterpart: how often push() sent types, positions,
+  // external forces, damage and constraints to the device
+  const std::array<int64_t, 5>& sends() const { return sends_; }
+
   std::vector<double> field(int f) const {
     std::vector<double> out((size_t)lpmg_field_len(h, f));
     check(lpmg_get_field(h, f, out.data()), "field");
@@ -584,15 +595,20 @@ class Assembly {
     pending_.clear();
   }
 
-  void send3(int f, std::array<double, 3> Particle<nlayer>::*m, std::vector<double>& sent) {
+  // bitwise equality (a NaN the device reported is not a host edit)
+  static bool same(const std::vector<double>& a, const std::vector<double>& b) {
+    return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(double)) == 0);
+  }
+  int send3(int f, std::array<double, 3> Particle<nlayer>::*m, std::vector<double>& sent) {
     std::vector<double> v(3 * (size_t)nparticle);
     for (int i = 0; i < nparticle; ++i)
       for (int k = 0; k < 3; ++k) v[3 * i + k] = (pt_sys[i]->*m)[k];
-    if (v != sent) {
-      check(lpmg_set_field(h, f, v.data()), "field");
-      sent = v;
-    }
+    if (same(v, sent)) return 0;
+    check(lpmg_set_field(h, f, v.data()), "field");
+    sent = v;
+    return 1;
   }
+  std::array<int64_t, 5> sends_{};
 
   void writeHeader(FILE* fpt, int step) {
     std::fprintf(fpt, "ITEM: TIMESTEP\n%d\nITEM: NUMBER OF ATOMS\n%d\nITEM: BOX BOUNDS pp pp pp\n", step, nparticle);

@@ -61,8 +61,10 @@ constexpr double kOmega = 2.0 / 3.0;
 // correction, prolonged piecewise constant).
 // A pass covers U chunks of LPR blocks, issued as bsr_rows issues them (block
 // columns and values, then the gathers, each group together).
+// `wave` is the wavefront's index over the level's rows (RPW = 64 / LPR rows
+// each); a row's sums do not depend on which wavefront forms them.
 template <int D, typename T, int LPR, int U, int XM>
-__device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
+__device__ __forceinline__ void row_gather(int wave, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
                                            const T* __restrict__ dinv, const T* __restrict__ b, T om, int N,
                                            int* row_out, bool* live_out, int* sl_out, T acc[D],
@@ -70,7 +72,6 @@ __device__ __forceinline__ void row_gather(int blk, const int* __restrict__ cptr
   constexpr int DD = D * D;
   constexpr int RPW = 64 / LPR;
   const int lane = threadIdx.x & 63, sub = lane / LPR, sl = lane - sub * LPR;
-  const int wave = blk * (kSpBS / 64) + (threadIdx.x >> 6);
   const int row = wave * RPW + sub;
   const bool live = row < N;
   const int rw = live ? row : N - 1;
@@ -147,7 +148,8 @@ __global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* _
   int row, sl;
   bool live;
   T acc[D];
-  row_gather<D, T, LPR, U, 1>(blockIdx.x, cptr, conns, val, nullptr, dinv, b, om, N, &row, &live, &sl, acc);
+  row_gather<D, T, LPR, U, 1>(blockIdx.x * (kSpBS / 64) + (threadIdx.x >> 6), cptr, conns, val, nullptr, dinv, b, om,
+                              N, &row, &live, &sl, acc);
   if (live && sl < D) {
     const size_t q = (size_t)row * D + sl;
     r[q] = b[q] - pick<D, T>(acc, sl);
@@ -174,8 +176,8 @@ __global__ void __launch_bounds__(kSpBS) amg_postsmooth_kernel(const int* __rest
   int row, sl;
   bool live;
   T acc[D];
-  row_gather<D, T, LPR, U, 2>(blockIdx.x, cptr, conns, val, nullptr, dinv, b, om, N, &row, &live, &sl, acc, agg,
-                              xc);
+  row_gather<D, T, LPR, U, 2>(blockIdx.x * (kSpBS / 64) + (threadIdx.x >> 6), cptr, conns, val, nullptr, dinv, b, om,
+                              N, &row, &live, &sl, acc, agg, xc);
   double dot = 0;
   if (live && sl < D) {
     const size_t q = (size_t)row * D + sl;
@@ -526,6 +528,7 @@ struct Amg {
   void* mask[2] = {nullptr, nullptr};  // finest-level free-DOF mask per precision
   int64_t version[2] = {-1, -1};
   hipGraphExec_t graph[2] = {nullptr, nullptr};
+  std::vector<uintptr_t> graph_sig[2];  // pointers and shapes captured into graph[P]
 };
 
 namespace {
@@ -847,7 +850,23 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
     return 0;
   };
   hipGraphExec_t* gexec = &A->graph[P];
-  if (!*gexec) {
+  std::vector<uintptr_t> sig;
+  for (const void* q : {(const void*)c->cptr, (const void*)c->conns, (const void*)val0, (const void*)dinv0,
+                        (const void*)x, (const void*)r, (const void*)z, (const void*)p, (const void*)w,
+                        (const void*)c->partials, (const void*)c->sc, (const void*)A->inv[P],
+                        (const void*)A->mask[P]})
+    graph_sig_push(&sig, q);
+  for (const auto& L : A->lv)
+    for (const void* q : {(const void*)L.cptr, (const void*)L.conns, (const void*)L.mptr, (const void*)L.midx,
+                          (const void*)L.agg, (const void*)L.val[P], (const void*)L.dinv[P],
+                          (const void*)L.vec[P][0], (const void*)L.vec[P][1], (const void*)L.vec[P][2],
+                          (const void*)L.vec[P][3]})
+      graph_sig_push(&sig, q);
+  for (int64_t v : {(int64_t)N, (int64_t)gs, (int64_t)gv, (int64_t)lpr, (int64_t)u, (int64_t)two, (int64_t)A->nc})
+    graph_sig_push(&sig, v);
+  if (*gexec) {
+    if (int rc = check_graph_signature(A->graph_sig[P], sig, "multigrid PCG chunk")) return rc;
+  } else {
     hipGraph_t graph = nullptr;
     LPMG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue_chunk();
@@ -856,6 +875,7 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
     LPMG_HIP(e2);
     LPMG_HIP(hipGraphInstantiate(gexec, graph, nullptr, nullptr, 0));
     LPMG_HIP(hipGraphDestroy(graph));
+    A->graph_sig[P] = sig;
   }
   for (;;) {
     LPMG_HIP(hipGraphLaunch(*gexec, c->stream));

@@ -164,8 +164,10 @@ struct Context {
   double prof_spmv_ms = 0;
   int64_t prof_spmv_n = 0;
   int prof_spmv_prec = -1;
-  // captured CG chunk per precision (FP64, FP32)
+  // captured CG chunk per precision (FP64, FP32), with the device pointers
+  // and launch shapes captured into it (checked at every launch)
   hipGraphExec_t cg_graph[2] = {nullptr, nullptr};
+  std::vector<uintptr_t> cg_graph_sig[2];
   int sc_host_iter_launched = 0;
   // CG executor (serving runtime); null: the context's own stream
   CgExec cg_exec = nullptr;
@@ -222,6 +224,16 @@ inline hipError_t copy_sync(Context* c, void* dst, const void* src, size_t bytes
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(c->stream);
 }
+
+// A captured graph bakes device pointers and launch shapes into its nodes;
+// these record them at capture and compare them at every launch, so a graph
+// whose buffers were replaced (a reallocation, a freed temporary) is never
+// replayed: the launch fails with -1 instead.
+inline void graph_sig_push(std::vector<uintptr_t>* sig, const void* p) { sig->push_back((uintptr_t)p); }
+inline void graph_sig_push(std::vector<uintptr_t>* sig, int64_t v) { sig->push_back((uintptr_t)v); }
+std::vector<uintptr_t> cg_graph_signature(const CgSys& q, int chunks);
+int check_graph_signature(const std::vector<uintptr_t>& captured, const std::vector<uintptr_t>& now,
+                          const char* which);
 
 // the context behind a C-ABI handle (api.hip)
 Context* handle_context(struct lpmg_context* h);

@@ -480,11 +480,15 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
     return 0;
   };
   std::vector<hipEvent_t> evs;
-  hipGraphExec_t* gexec = &c->cg_graph[sizeof(T) == 4 ? 1 : 0];
+  const int P = sizeof(T) == 4 ? 1 : 0;
+  hipGraphExec_t* gexec = &c->cg_graph[P];
+  const std::vector<uintptr_t> sig = cg_graph_signature(q, chunks);
   if (c->prof) {
     evs.resize(2 * kCgChunk);
     for (auto& e : evs) LPMG_HIP(hipEventCreate(&e));
-  } else if (!*gexec) {
+  } else if (*gexec) {
+    if (int rc = check_graph_signature(c->cg_graph_sig[P], sig, "CG chunk")) return rc;
+  } else {
     // capture one chunk once per handle and precision; every pointer, size and
     // launch shape in it is fixed for the handle's lifetime
     hipGraph_t graph = nullptr;
@@ -495,6 +499,7 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
     LPMG_HIP(e2);
     LPMG_HIP(hipGraphInstantiate(gexec, graph, nullptr, nullptr, 0));
     LPMG_HIP(hipGraphDestroy(graph));
+    c->cg_graph_sig[P] = sig;
   }
   for (;;) {
     const int i0 = c->sc_host_iter_launched;
@@ -535,6 +540,25 @@ void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
 }
 
 }  // namespace
+
+std::vector<uintptr_t> cg_graph_signature(const CgSys& q, int chunks) {
+  std::vector<uintptr_t> s;
+  for (const void* p : {(const void*)q.cptr, (const void*)q.conns, q.val, q.dinv, (const void*)q.x, (const void*)q.r,
+                        (const void*)q.u, (const void*)q.p, (const void*)q.s, (const void*)q.w, (const void*)q.partV0,
+                        (const void*)q.partV1, (const void*)q.partS_raw, (const void*)q.partS, (const void*)q.sc})
+    graph_sig_push(&s, p);
+  for (int64_t v : {(int64_t)q.n, (int64_t)q.N, (int64_t)q.gs, (int64_t)q.gv, (int64_t)q.nS, (int64_t)q.two_level,
+                    (int64_t)chunks})
+    graph_sig_push(&s, v);
+  return s;
+}
+
+int check_graph_signature(const std::vector<uintptr_t>& captured, const std::vector<uintptr_t>& now,
+                          const char* which) {
+  if (captured == now) return 0;
+  set_error(std::string("captured ") + which + " graph no longer matches its buffers or launch shape");
+  return -1;
+}
 
 int cg_class(int dim, int fp32, int chunks) {
   return (dim == 3 ? 6 : 0) + (fp32 ? 3 : 0) + (chunks - 2);

@@ -71,6 +71,9 @@ int main(int argc, char** argv) {
   }
   const int N = solv.solveProblemCyclic(FatigueLoadType::LoadUniaxialDisp, 1, groups);
   if (N != n_sim * dN) return 4;
+  const auto& sent = pt_ass.sends();
+  std::fprintf(stderr, "host edits sent: types %lld, xyz %lld, Pex %lld, damage %lld, constraints %lld\n",
+               (long long)sent[0], (long long)sent[1], (long long)sent[2], (long long)sent[3], (long long)sent[4]);
   for (Particle<n_layer>* p : pt_ass.pt_sys)
     std::printf("%.17g %.17g %.17g\n", p->xyz[0] - p->xyz_initial[0], p->xyz[1] - p->xyz_initial[1], p->damage);
   return 0;
