@@ -103,3 +103,57 @@ def test_polycrystal_served_with_ct_requests(poly):
     for a, b in zip(mix[1:], ct_alone):
         np.testing.assert_array_equal(a.out["XYZ"], b.out["XYZ"])
         np.testing.assert_array_equal(a.out["BDAMAGE"], b.out["BDAMAGE"])
+
+
+# CT requests of the serving stream's workload (2D hex CT, a/W in [0.2, 0.5],
+# critical bond stretch 1e-3, 30 load steps of pin opening 7.5e-6 W, Newton tol 1e-9)
+CT_TOL = 1e-9
+SERVED = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
+
+
+def test_large_served_ct_matches_plain_fp64():
+    # A 100k-particle CT through the server's mixed path (multigrid CG, FP32
+    # inner solves, predictor) against the plain FP64 path on its own handle
+    # (Jacobi-PCG at the reference's CG tolerance, the reference's step
+    # start): displacements within the stream's bar, the same broken bonds.
+    prob = P.ct_plate(100500, 0.29, steps=30, opening=7.5e-6, crit=1e-3, tol=CT_TOL)
+    assert len(prob.xyz) >= 100000
+    s, gids = P.build(lpmg.System, prob)
+    s.forward()
+    P.solve_problem(s, gids, prob, FP64)
+    u64 = (s.get("XYZ") - s.get("XYZ_INITIAL")).reshape(-1, 3)
+    bd64 = s.get("BDAMAGE") > 0.5
+    s.close()
+    r = _req(prob, SERVED)
+    with Server(max_active=2, batching=50000) as srv:
+        status, st = srv.wait(srv.submit(r))
+    assert status == 0, st.get("error")
+    assert st["bytes"] <= st["bytes_est"]
+    u = r.out["XYZ"].reshape(-1, 3) - prob.xyz
+    assert _rel(u, u64) <= 1e-8
+    assert bd64.sum() > 0
+    np.testing.assert_array_equal(r.out["BDAMAGE"] > 0.5, bd64)
+
+
+def test_stream_sized_ct_matches_oracle():
+    # A CT of the stream's smallest size class (>= 5k particles) served with
+    # the mixed options against the CPU oracle's FP64 solve of the same
+    # request: displacements within 1e-8, the same broken bonds.
+    from oracle import Oracle, set_threads
+
+    prob = P.ct_plate(5200, 0.35, steps=30, opening=7.5e-6, crit=1e-3, tol=CT_TOL)
+    assert len(prob.xyz) >= 5000
+    set_threads(16)
+    o, gids = P.build(Oracle, prob)
+    o.forward()
+    P.solve_problem(o, gids, prob, dict(precond=1, rtol=1e-12, atol=0.0))
+    u_ref = (o.get("XYZ") - o.get("XYZ_INITIAL")).reshape(-1, 3)
+    bd_ref = o.get("BDAMAGE") > 0.5
+    r = _req(prob, SERVED)
+    with Server(max_active=1, batching=50000) as srv:
+        status, st = srv.wait(srv.submit(r))
+    assert status == 0, st.get("error")
+    u = r.out["XYZ"].reshape(-1, 3) - prob.xyz
+    assert _rel(u, u_ref) <= 1e-8
+    assert bd_ref.sum() > 0
+    np.testing.assert_array_equal(r.out["BDAMAGE"] > 0.5, bd_ref)
