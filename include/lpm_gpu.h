@@ -160,6 +160,14 @@ int lpmg_assemble(lpmg_handle h, int precision);
  * symmetric upper CSR, solver.h:334).  x, y: n*dim host buffers. */
 int lpmg_spmv(lpmg_handle h, int precision, const double* x, double* y);
 
+/* lpmg_spmv without any device allocation: no counterpart in the reference.
+ * For callers that cannot allocate (tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and bit-identical result; x and y are staged in one
+ * 24 MiB static device array per GPU, so n*dim must be at most 1,048,576
+ * (larger systems return -1 with a message: use lpmg_spmv).  Calls that overlap
+ * on one GPU share that array and race. */
+int lpmg_spmv_noalloc(lpmg_handle h, int precision, const double* x, double* y);
+
 /* Solver::solveLinearSystem (solver.h:92-109): solve K du = residual by
  * (P)CG (replaces LPM_CG solver.h:276-354 and LPM_PARDISO solver.h:190-274),
  * then move every particle by du.  Returns -4 if maxit was reached. */

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -402,6 +403,43 @@ double forward_bytes(const Context* c) {
 
 }  // namespace
 
+// Allocation-free SpMV: the same kernels as lpmg_spmv, with the staged x and y
+// (and their FP32 copies) in one file-scope device array instead of
+// stream-ordered allocations.  Each GPU holds its own copy of the array from
+// module load until the process ends; its address on a device comes from
+// hipGetSymbolAddress on the first call there (the array's name is a host-side
+// stand-in that cannot be passed to a kernel).  Every call on one GPU shares
+// the array, so calls that overlap on a GPU race; nothing in it needs clearing
+// (x is copied in whole and every row of y is written).
+namespace {
+constexpr int64_t kScratchMaxDofs = int64_t(1) << 20;
+constexpr size_t kScratchBytes = (size_t)kScratchMaxDofs * (2 * sizeof(double) + 2 * sizeof(float));
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+int scratch_on_device(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= kScratchDevices) {
+    set_error("allocation-free entry points: device ordinal out of range");
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = -1;
+    API_HIP(hipGetDevice(&cur));
+    if (cur != dev) API_HIP(hipSetDevice(dev));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) API_HIP(hipSetDevice(cur));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
 extern "C" {
 
 int lpmg_version(void) { return 1; }
@@ -620,6 +658,39 @@ int lpmg_spmv(lpmg_handle h, int precision, const double* x, double* y) {
   if (fx) (void)hipFreeAsync(fx, c->stream);
   if (fy) (void)hipFreeAsync(fy, c->stream);
   return rc;
+}
+
+int lpmg_spmv_noalloc(lpmg_handle h, int precision, const double* x, double* y) {
+  CHECK_HANDLE(h);
+  Context* c = &h->c;
+  const int64_t n = (int64_t)c->dim * c->N;
+  if (n > kScratchMaxDofs) {
+    set_error("lpmg_spmv_noalloc: " + std::to_string(n) + " unknowns exceed the " +
+              std::to_string(kScratchMaxDofs) + " its static scratch holds (use lpmg_spmv)");
+    return -1;
+  }
+  if (precision == 0 && !c->assembled64) { set_error("stiffness not assembled"); return -1; }
+  if (precision == 1 && !c->assembled32) { set_error("FP32 stiffness not assembled"); return -1; }
+  unsigned char* base = nullptr;
+  if (int rc = scratch_on_device(c->device, &base)) return rc;
+  double* dx = (double*)base;
+  double* dy = dx + kScratchMaxDofs;
+  float* fx = (float*)(dy + kScratchMaxDofs);
+  float* fy = fx + kScratchMaxDofs;
+  API_HIP(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  int rc;
+  const int nb = (int)((n + 255) / 256);
+  if (precision == 0) {
+    rc = launch_spmv(c, 0, dx, dy);
+  } else {
+    d2f_kernel<<<nb, 256, 0, c->stream>>>(dx, fx, n);
+    rc = launch_spmv(c, 1, fx, fy);
+    f2d_kernel<<<nb, 256, 0, c->stream>>>(fy, dy, n);
+  }
+  if (rc) return rc;
+  API_HIP(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  API_HIP(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int lpmg_solve_linear(lpmg_handle h, const double* opts, double* stats) {

@@ -28,7 +28,7 @@ FIELDS = dict(XYZ=0, XYZ_INITIAL=1, XYZ_LAST=2, PIN=3, PEX=4, STRESS=5, DAMAGE=6
 EXPORTS = ("lpmg_version", "lpmg_last_error", "lpmg_unit_cell", "lpmg_create", "lpmg_create_with_bonds",
            "lpmg_destroy", "lpmg_sizes", "lpmg_export_bonds", "lpmg_export_conns", "lpmg_export_csr",
            "lpmg_set_types", "lpmg_set_material", "lpmg_forward", "lpmg_update_rr", "lpmg_assemble",
-           "lpmg_spmv", "lpmg_solve_linear", "lpmg_newton", "lpmg_update_state_var",
+           "lpmg_spmv", "lpmg_spmv_noalloc", "lpmg_solve_linear", "lpmg_newton", "lpmg_update_state_var",
            "lpmg_update_static_damage", "lpmg_update_fatigue_damage", "lpmg_store_state", "lpmg_reset_state", "lpmg_search_nonlocal",
            "lpmg_export_nonlocal",
            "lpmg_define_group", "lpmg_disp_bc", "lpmg_force_bc", "lpmg_solve_step", "lpmg_restart", "lpmg_field_len",
@@ -81,6 +81,7 @@ def lib():
         L.lpmg_update_rr.argtypes = [vp, dp]
         L.lpmg_assemble.argtypes = [vp, ctypes.c_int]
         L.lpmg_spmv.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.lpmg_spmv_noalloc.argtypes = [vp, ctypes.c_int, dp, dp]
         L.lpmg_solve_linear.argtypes = [vp, dp, dp]
         L.lpmg_newton.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, dp]
         L.lpmg_update_static_damage.argtypes = [vp, ctypes.c_int]
@@ -224,6 +225,14 @@ class System:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         self._chk(lib().lpmg_spmv(self.h, int(precision), _p(x, ctypes.c_double), _p(y, ctypes.c_double)))
+        return y
+
+    def spmv_noalloc(self, x, precision=0):
+        """spmv through the allocation-free entry point (static device staging)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._chk(lib().lpmg_spmv_noalloc(self.h, int(precision), _p(x, ctypes.c_double),
+                                          _p(y, ctypes.c_double)))
         return y
 
     def solve_linear(self, **kw):
