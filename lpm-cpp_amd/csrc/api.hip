@@ -177,6 +177,7 @@ int alloc_context(lpmg_context* h, const double* xyz) {
   rc |= dalloc(c, &c->dLe_tot, 2 * N);
   rc |= dalloc(c, &c->TdLe_tot, 2 * N);
   rc |= dalloc(c, &c->cs_sum, 6 * N);
+  rc |= dalloc(c, &c->cs_wsum, 6 * N);
   rc |= dalloc(c, &c->Kn, 2 * N);
   rc |= dalloc(c, &c->type, N);
   rc |= dalloc(c, &c->floating, N);
@@ -233,9 +234,7 @@ int finish_create(lpmg_context* h) {
     set_error("stiffness exceeds 2^31 values per handle");
     return -1;
   }
-  if (dalloc(c, &c->Kval, (size_t)c->nnzb * c->dim * c->dim) ||
-      dalloc(c, &c->Kloc, (size_t)c->nnzb * c->dim * c->dim))
-    return -3;
+  if (dalloc(c, &c->Kval, (size_t)c->nnzb * c->dim * c->dim)) return -3;
   API_HIP(hipStreamSynchronize(c->stream));
   return 0;
 }

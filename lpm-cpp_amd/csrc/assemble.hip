@@ -728,6 +728,458 @@ __global__ void export_csr_kernel(const int64_t* __restrict__ Kp, const int* __r
   if (i == N - 1) IK[(size_t)D * N] = Kp[N] + 1;
 }
 
+// ------------------------------------------------------------ one-pass assembly
+//
+// The assembled block is K(i,j) = 0.5 L(i,j) + 0.5 L(j,i)^T (stiffness.h:241-249,
+// 267-275), L the unsymmetrised local tangent.  Every term of L(j,i) runs
+// through the same particles as a term of L(i,j): the direct bond j->i is the
+// reverse of i->j, and a two-hop path j->k->i uses the reverses of the bonds of
+// a path i->k->j (bond lists are symmetric within each layer).  So the lane of
+// block (i,j) forms both halves and writes the final block -- symmetrised,
+// eliminated, and its FP32 copy -- once, with no unsymmetrised blocks in HBM:
+//   direct + common-neighbour term of L(i,j) in layer L through bond b = i->j
+//   (derivative.h:59-99 / 156-196), with the common-neighbour sum over the
+//   layer's other bonds in closed form from per-particle sums
+//     Q_i = sum_t c_t (cs_sum),  S_i = sum_t (1 - bdamage_t) c_t,
+//     om(b) ((-Kn_i c_b[s] + Tv_i Q_j[s]) c_b[r] - (Kn_i + Tv_i) c_b[r] c_b[s]) - v_i[r] c_b[s],
+//     v_i = 0.5 Tv_i (om(b) (Q_i - c_b) + (S_i - om(b) c_b)),
+//   and the same with i and j exchanged through the reverse bond for L(j,i);
+//   path i->k->j (bonds t = i->k, u = k->j; derivative.h:101-139 / 198-236)
+//   with the path j->k->i (bonds rev u, rev t) of L(j,i):
+//     -0.5 (0.5 (om(u) Tv_k + om(t) Tv_i) c_t[r] c_u[s]
+//           + 0.5 (om(rev t) Tv_k + om(rev u) Tv_j) c_rev t[r] c_rev u[s]).
+// Each term is formed so that the lanes of (i,j) and (j,i) compute it from the
+// same values with operands exchanged, and the terms are added in the same order
+// (direct term, then paths by ascending k, which is bond order for bond lists
+// sorted by particle id): the two blocks are exact transposes of each other.
+// The rounding differs from the two-pass assembly by a few ulps of each term.
+// Used when no particle has two same-layer bonds to one particle, rows have at
+// most 64 bonds and 64 connections, and the path data fits LDS.
+
+// Per particle: eliminated DOFs (fixm), S_i per layer, the diagonal block
+// (fdu2dxyz, derivative.h:7-45, in the reference's bond order) written into the
+// stiffness before elimination, and the workgroup's share of ||diag(K)||^2.
+template <int D>
+__global__ void __launch_bounds__(256)
+assembly_prep_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, const double* __restrict__ cs,
+                     const double* __restrict__ bdamage, const double* __restrict__ cs_sum,
+                     const double* __restrict__ Kn, const double* __restrict__ Tv, const int* __restrict__ dpos,
+                     const int* __restrict__ cons, const double* __restrict__ dvis,
+                     const int* __restrict__ floating, int N, int* __restrict__ fixm,
+                     double* __restrict__ cs_wsum, double* __restrict__ diag_sq, double* __restrict__ Kval,
+                     double* __restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  Vec<1> a;
+  a.v[0] = 0.0;
+  if (i < N) {
+    int m = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) m |= dof_fixed(cons, dvis, floating, i, k) ? 1 << k : 0;
+    fixm[i] = m;
+    const int b_lo = bptr[i], b_mid = b_lo + nb0[i], b_hi = bptr[i + 1];
+    const double tv_i = Tv[i];
+    double K[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+    for (int L = 0; L < 2; ++L) {
+      const int lo = L == 0 ? b_lo : b_mid, hi = L == 0 ? b_mid : b_hi;
+      const double kn = Kn[2 * i + L];
+      const double* csum = cs_sum + (2 * i + L) * 3;
+      double w[3] = {0.0, 0.0, 0.0};
+      for (int b = lo; b < hi; ++b) {
+        const double om = 1 - bdamage[b];
+        const double c[3] = {cs[3 * b], cs[3 * b + 1], cs[3 * b + 2]};
+#pragma unroll
+        for (int q = 0; q < 3; ++q) w[q] += om * c[q];
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int s = r; s < D; ++s) {
+            const double geo = (r == s) ? (kn + tv_i) * (c[r] * c[r]) : (kn + tv_i) * c[s] * c[r];
+            K[r][s] += om * ((kn * c[s] + tv_i * csum[s]) * c[r] + geo);
+          }
+      }
+#pragma unroll
+      for (int q = 0; q < 3; ++q) cs_wsum[(2 * i + L) * 3 + q] = w[q];
+    }
+    double dsq = 0;
+#pragma unroll
+    for (int r = 0; r < D; ++r) dsq += K[r][r] * K[r][r];
+    diag_sq[i] = dsq;
+    a.v[0] = dsq;
+    double* out = Kval + (size_t)dpos[i] * D * D;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) out[r * D + s] = s >= r ? K[r][s] : K[s][r];
+  }
+  write_partial<256, 1>(partials, a);
+}
+
+// Direct and common-neighbour terms of L(a,b) in one layer through bond a->b:
+// direction c, om = 1 - bdamage, a's Kn, Tv, Q, S and b's Q.
+template <int D>
+__device__ __forceinline__ void direct_common_half(double kn, double tv, const double* Qa, const double* Sa,
+                                                   const double* Qb, const double* c, double om,
+                                                   double H[D][D]) {
+  double v[D];
+#pragma unroll
+  for (int r = 0; r < D; ++r) v[r] = 0.5 * tv * (om * (Qa[r] - c[r]) + (Sa[r] - om * c[r]));
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      const double cc = r >= s ? c[r] * c[s] : c[s] * c[r];
+      H[r][s] = om * ((-kn * c[s] + tv * Qb[s]) * c[r] - (kn + tv) * cc) - v[r] * c[s];
+    }
+}
+
+// Doubles per bond in the path table: c_t[3], om(t), c_rev t[3], om(rev t), Tv(k).
+constexpr int kSymTable = 9;
+// Path entries per lane whose loads are issued before the first barrier (rows
+// of more entries take further rounds after it).
+constexpr int kSymGather = 3;
+// Flag of a path entry's slot word: a bond of the path is broken on one side
+// only (the two halves then have different directions).
+constexpr int kOneSided = 1 << 8;
+
+// The path plan depends on the bond topology only, so it is built once per
+// handle.  For row i (a wavefront), path entry e (bond u = klo_t + v of the end
+// k of i's bond t; entries ordered by t, then v) is owned by the lane whose
+// connection is u's end m; entries with m == i belong to the diagonal block and
+// are left out.  Each owner's entries are ranked in bond order into slots:
+//   plan_e[i*E + e] = u | (t | owner << 6 | slot << 12) << 32, or -1,
+//   plan_l[i*64 + lane] = first slot | count << 10 | (direct bond of layer 0
+//       + 1) << 17 | (direct bond of layer 1 + 1) << 24,
+//   plan_n[i] = number of entries.
+__host__ __device__ constexpr size_t plan_slice_bytes(int E) {
+  return ((size_t)64 * 8 + 6 * 64 * 4 + (size_t)2 * E + 15) / 16 * 16;
+}
+
+__global__ void __launch_bounds__(64 * kWaves)
+tangent_plan_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, const int* __restrict__ p2,
+                    const int* __restrict__ cptr, const int* __restrict__ conns, int N, int E,
+                    long long* __restrict__ plan_e, int* __restrict__ plan_l, int* __restrict__ plan_n) {
+  extern __shared__ double lds_plan[];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + w);
+  const bool row = i < N;  // wave-uniform; every wavefront still reaches the barriers
+  char* const slice = reinterpret_cast<char*>(lds_plan) + (size_t)w * plan_slice_bytes(E);
+  unsigned long long* const Lmask = reinterpret_cast<unsigned long long*>(slice);  // [64] per owner
+  int* const Lconn = reinterpret_cast<int*>(Lmask + 64);  // [64] connections, padded with INT_MAX
+  int* const Loff = Lconn + 64;                           // [64] first entry of bond t
+  int* const Lklo = Loff + 64;                            // [64] k's first bond of the layer
+  int* const Lbase = Lklo + 64;                           // [64] first slot of each owner
+  int* const Ldir = Lbase + 64;                           // [2][64] direct bond of each owner per layer
+  short* const Lot = reinterpret_cast<short*>(Ldir + 128);  // [E] owner << 8 | bond, -1 without owner
+  int c_lo = 0, nc = 0, b_lo = 0, n0 = 0, nbi = 0;
+  if (row) {
+    c_lo = cptr[i]; nc = cptr[i + 1] - c_lo;
+    b_lo = bptr[i]; n0 = nb0[i]; nbi = bptr[i + 1] - b_lo;
+  }
+  const bool active = lane < nc;
+  const int j = active ? conns[c_lo + lane] : -1;
+  Lconn[lane] = active ? j : INT_MAX;
+  Lmask[lane] = 0ull;
+  Ldir[lane] = -1;
+  Ldir[64 + lane] = -1;
+  int bk = 0, bL = 0, klo = 0, nk = 0;
+  if (lane < nbi) {
+    bk = p2[b_lo + lane];
+    bL = lane < n0 ? 0 : 1;
+    const int k0 = bptr[bk], k1 = k0 + nb0[bk], k2 = bptr[bk + 1];
+    klo = bL == 0 ? k0 : k1;
+    nk = bL == 0 ? k1 - k0 : k2 - k1;
+  }
+  const int incl = wave_incl_scan(nk);
+  const int off = incl - nk;
+  const int T = __builtin_amdgcn_readfirstlane(__shfl(incl, 63, 64));
+  if (lane < nbi) {
+    Loff[lane] = off;
+    Lklo[lane] = klo;
+  }
+  __syncthreads();
+  if (lane < nbi) {  // the owner of bond t's end takes it as its direct bond
+    int pos = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1)
+      if (Lconn[pos + st] <= bk) pos += st;
+    if (Lconn[pos] == bk) Ldir[bL * 64 + pos] = lane;
+  }
+  for (int e = lane; e < T; e += 64) {
+    int lo = 0, hi = nbi - 1;  // last t with Loff[t] <= e
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (Loff[mid] <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const int m = p2[Lklo[lo] + (e - Loff[lo])];
+    int pos = 0;
+#pragma unroll
+    for (int st = 32; st > 0; st >>= 1)
+      if (Lconn[pos + st] <= m) pos += st;
+    const bool own = Lconn[pos] == m && m != i;
+    Lot[e] = (short)(own ? (pos << 8 | lo) : -1);
+    if (own) atomicOr(&Lmask[pos], 1ull << lo);
+  }
+  __syncthreads();
+  const unsigned long long mine = active ? Lmask[lane] : 0ull;
+  const int cnt = __popcll(mine);
+  const int base = wave_incl_scan(cnt) - cnt;
+  Lbase[lane] = base;
+  __syncthreads();
+  if (!row) return;  // no barrier below
+  for (int e = lane; e < T; e += 64) {
+    const int v = Lot[e];
+    long long rec = -1;
+    if (v >= 0) {
+      const int o = v >> 8, t = v & 255;
+      const int slot = Lbase[o] + __popcll(Lmask[o] & ((1ull << t) - 1ull));
+      const int u = Lklo[t] + (e - Loff[t]);
+      rec = (long long)(unsigned)u | ((long long)(t | o << 6 | slot << 12) << 32);
+    }
+    plan_e[(size_t)i * E + e] = rec;
+  }
+  if (active)
+    plan_l[(size_t)i * 64 + lane] = base | cnt << 10 | (Ldir[lane] + 1) << 17 | (Ldir[64 + lane] + 1) << 24;
+  if (lane == 0) plan_n[i] = T;
+}
+
+// Per wavefront: the bond table, a region holding first the path entries'
+// coefficients and second directions (slot order) and then the row's finished
+// blocks on their way to HBM, Tv of each connection, and per slot the bond
+// index and the bond t.
+__host__ __device__ constexpr size_t sym_slice_bytes(int E, int NBM, int DD) {
+  return ((size_t)kSymTable * NBM * 8 + (size_t)8 * (4 * E > 64 * DD ? 4 * E : 64 * DD) + 64 * 8 +
+          (size_t)4 * E + (size_t)2 * E + 15) / 16 * 16;
+}
+
+template <int D>
+__global__ void __launch_bounds__(64 * kWaves)
+tangent_sym_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, const int* __restrict__ p2,
+                   const int* __restrict__ rev, const double* __restrict__ cs, const double* __restrict__ bdamage,
+                   const double* __restrict__ cs_sum, const double* __restrict__ cs_wsum,
+                   const double* __restrict__ Kn, const double* __restrict__ Tv, const int* __restrict__ cptr,
+                   const int* __restrict__ conns, const int* __restrict__ fixm, const Scalars* __restrict__ sc,
+                   const long long* __restrict__ plan_e, const int* __restrict__ plan_l,
+                   const int* __restrict__ plan_n, int N, int E, int NBM, double* __restrict__ Kval,
+                   float* __restrict__ Kval32) {
+  constexpr int DD = D * D;
+  extern __shared__ double lds_sym[];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + w);
+  const bool row = i < N;  // wave-uniform; every wavefront still reaches the barriers
+  char* const slice = reinterpret_cast<char*>(lds_sym) + (size_t)w * sym_slice_bytes(E, NBM, DD);
+  double* const Lt = reinterpret_cast<double*>(slice);  // [kSymTable][NBM]
+  double* const Lreg = Lt + kSymTable * NBM;            // [max(4E, 64 DD)]
+  double* const Lcc = Lreg;                             // [E] combined path coefficient, by slot
+  double* const Lc3 = Lreg + E;                         // [E][3] direction of the path's second bond
+  double* const Ltvj = Lreg + (4 * E > 64 * DD ? 4 * E : 64 * DD);  // [64] Tv of each connection
+  int* const Lu = reinterpret_cast<int*>(Ltvj + 64);    // [E] the path's second bond, by slot
+  short* const Lst = reinterpret_cast<short*>(Lu + E);  // [E] bond t (| kOneSided), by slot
+  int c_lo = 0, nc = 0, b_lo = 0, n0 = 0, nbi = 0, T = 0;
+  double tv_i = 0;
+  if (row) {
+    c_lo = cptr[i]; nc = cptr[i + 1] - c_lo;
+    b_lo = bptr[i]; n0 = nb0[i]; nbi = bptr[i + 1] - b_lo;
+    tv_i = Tv[i];
+    T = plan_n[i];
+  }
+  const bool active = row && lane < nc;
+  const int pw = active ? plan_l[(size_t)i * 64 + lane] : 0;
+  const int j = active ? conns[c_lo + lane] : -1;
+  // the first round of path entries, loaded before the barrier
+  long long rec[kSymGather];
+#pragma unroll
+  for (int q = 0; q < kSymGather; ++q) {
+    const int e = lane + 64 * q;
+    rec[q] = e < T ? plan_e[(size_t)i * E + e] : -1ll;
+  }
+  int u3[kSymGather], r3[kSymGather];
+  double c3[kSymGather][3], cr3[kSymGather][3], om3[kSymGather], omr3[kSymGather];
+#pragma unroll
+  for (int q = 0; q < kSymGather; ++q) {
+    u3[q] = rec[q] >= 0 ? (int)(unsigned)(rec[q] & 0xffffffffll) : 0;
+    r3[q] = rev[u3[q]];
+#pragma unroll
+    for (int u = 0; u < 3; ++u) c3[q][u] = cs[3 * u3[q] + u];
+    om3[q] = 1 - bdamage[u3[q]];
+  }
+#pragma unroll
+  for (int q = 0; q < kSymGather; ++q) {
+#pragma unroll
+    for (int u = 0; u < 3; ++u) cr3[q][u] = cs[3 * r3[q] + u];
+    omr3[q] = 1 - bdamage[r3[q]];
+  }
+  // lane t: bond t of i and its reverse
+  if (lane < nbi) {
+    const int b = b_lo + lane, rb = rev[b];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      Lt[q * NBM + lane] = cs[3 * b + q];
+      Lt[(4 + q) * NBM + lane] = cs[3 * rb + q];
+    }
+    Lt[3 * NBM + lane] = 1 - bdamage[b];
+    Lt[7 * NBM + lane] = 1 - bdamage[rb];
+    Lt[8 * NBM + lane] = Tv[p2[b]];
+  }
+  Ltvj[lane] = active ? Tv[j] : 0.0;
+  const int mj = active ? fixm[j] : 0;
+  __syncthreads();
+  // Each entry's coefficient 0.5 (cA + cB) and second direction go to its slot.
+  auto place = [&](long long r, const double* c3e, const double* cr3e, double om3e, double omr3e) {
+    const int word = (int)(r >> 32);
+    const int t = word & 63, o = (word >> 6) & 63, slot = word >> 12;
+    const double tvk = Lt[8 * NBM + t];
+    const double cA = 0.5 * (om3e * tvk + Lt[3 * NBM + t] * tv_i);
+    const double cB = 0.5 * (Lt[7 * NBM + t] * tvk + omr3e * Ltvj[o]);
+    bool anti = true;
+#pragma unroll
+    for (int u = 0; u < D; ++u)
+      anti = anti && Lt[(4 + u) * NBM + t] == -Lt[u * NBM + t] && cr3e[u] == -c3e[u];
+    Lcc[slot] = 0.5 * (cA + cB);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) Lc3[3 * slot + u] = c3e[u];
+    Lu[slot] = (int)(unsigned)(r & 0xffffffffll);
+    Lst[slot] = (short)(t | (anti ? 0 : kOneSided));
+  };
+#pragma unroll
+  for (int q = 0; q < kSymGather; ++q)
+    if (rec[q] >= 0) place(rec[q], c3[q], cr3[q], om3[q], omr3[q]);
+  for (int e = lane + 64 * kSymGather; e < T; e += 64) {  // rows with more entries
+    const long long r = plan_e[(size_t)i * E + e];
+    if (r < 0) continue;
+    const int u = (int)(unsigned)(r & 0xffffffffll), ru = rev[u];
+    double a[3], b[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      a[k] = cs[3 * u + k];
+      b[k] = cs[3 * ru + k];
+    }
+    place(r, a, b, 1 - bdamage[u], 1 - bdamage[ru]);
+  }
+  __syncthreads();
+  if (!row) return;  // no barrier below
+  const int mi = fixm[i];
+  double K[D][D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int s = 0; s < D; ++s) K[r][s] = 0.0;
+  if (active && j == i) {  // the diagonal block from the per-particle pass, eliminated
+    const double* in = Kval + (size_t)(c_lo + lane) * DD;
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) K[r][s] = in[r * D + s];
+    if (mi) {
+      const double nd = sc->norm_diag;
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s)
+          if (((mi >> r) & 1) || ((mi >> s) & 1)) K[r][s] = (r == s) ? nd : 0.0;
+    }
+  } else if (active) {
+    const double tv_j = Ltvj[lane];
+    int q = pw & 1023;
+    const int qend = q + ((pw >> 10) & 127);
+    for (int L = 0; L < 2; ++L) {
+      double acc[D][D];
+#pragma unroll
+      for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int s = 0; s < D; ++s) acc[r][s] = 0.0;
+      const int t1 = ((pw >> (L == 0 ? 17 : 24)) & 127) - 1;
+      bool found = t1 >= 0;
+      if (found) {
+        double Qi[3], Si[3], Qj[3], Sj[3], c1[3], cr1[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          Qi[u] = cs_sum[(2 * i + L) * 3 + u];
+          Si[u] = cs_wsum[(2 * i + L) * 3 + u];
+          Qj[u] = cs_sum[(2 * j + L) * 3 + u];
+          Sj[u] = cs_wsum[(2 * j + L) * 3 + u];
+          c1[u] = Lt[u * NBM + t1];
+          cr1[u] = Lt[(4 + u) * NBM + t1];
+        }
+        double A[D][D], B[D][D];
+        direct_common_half<D>(Kn[2 * i + L], tv_i, Qi, Si, Qj, c1, Lt[3 * NBM + t1], A);
+        direct_common_half<D>(Kn[2 * j + L], tv_j, Qj, Sj, Qi, cr1, Lt[7 * NBM + t1], B);
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int s = 0; s < D; ++s) acc[r][s] = 0.5 * A[r][s] + 0.5 * B[s][r];
+      }
+      const int thi = L == 0 ? n0 : nbi;
+      while (q < qend) {
+        const int st = Lst[q];
+        const int t = st & 63;
+        if (t >= thi) break;
+        found = true;
+        double ct[3], cu[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          ct[u] = Lt[u * NBM + t];
+          cu[u] = Lc3[3 * q + u];
+        }
+        if (!(st & kOneSided)) {  // c_rev t = -c_t, c_rev u = -c_u: the two products are equal
+          const double cc = Lcc[q];
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) acc[r][s] -= cc * (ct[r] * cu[s]);
+        } else {  // a bond broken on one side only: the halves separately
+          const int u = Lu[q], ru = rev[u];
+          double crt[3], cru[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k) {
+            crt[k] = Lt[(4 + k) * NBM + t];
+            cru[k] = cs[3 * ru + k];
+          }
+          const double tvk = Lt[8 * NBM + t];
+          const double cA = 0.5 * ((1 - bdamage[u]) * tvk + Lt[3 * NBM + t] * tv_i);
+          const double cB = 0.5 * (Lt[7 * NBM + t] * tvk + (1 - bdamage[ru]) * tv_j);
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int s = 0; s < D; ++s) acc[r][s] -= 0.5 * (cA * (ct[r] * cu[s]) + cB * (crt[r] * cru[s]));
+        }
+        ++q;
+      }
+      if (found)
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+          for (int s = 0; s < D; ++s) K[r][s] = K[r][s] + acc[r][s];
+    }
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s)
+        if (((mi >> r) & 1) || ((mj >> s) & 1)) K[r][s] = 0.0;
+  }
+  // The row's blocks are contiguous in HBM: stage them in this wavefront's
+  // slice (its path data is no longer read: the wavefront's LDS operations
+  // complete in order) and store them with consecutive lanes on consecutive
+  // values.
+  double* const Ks = Lreg;
+  if (active)
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+      for (int s = 0; s < D; ++s) Ks[lane * DD + r * D + s] = K[r][s];
+  const int nv = nc * DD;
+  double* const out = Kval + (size_t)c_lo * DD;
+  for (int x = lane; x < nv; x += 64) out[x] = Ks[x];
+  if (Kval32) {
+    float* const o32 = Kval32 + (size_t)c_lo * DD;
+    for (int x = lane; x < nv; x += 64) o32[x] = (float)Ks[x];
+  }
+}
+
 }  // namespace
 
 int launch_assemble(Context* c, int want32) {
@@ -744,6 +1196,50 @@ int launch_assemble(Context* c, int want32) {
   if (c->floating_dirty)
     if (int rc = update_floating(c)) return rc;
   float* k32 = want32 ? c->Kval32 : nullptr;
+  // one pass (tangent_sym_kernel) when every row fits a wavefront and LDS
+  {
+    const int E = std::max(32, (c->max_cand + 7) / 8 * 8);
+    const int NBM = std::max(1, c->max_nb);
+    const int DD = c->dim * c->dim;
+    const size_t lds = (size_t)kWaves * sym_slice_bytes(E, NBM, DD);
+    if (c->bonds_unique && c->max_nb <= 64 && c->max_nconn <= 64 && E <= 1023 && lds <= 65536) {
+      if (c->plan_E != E) {  // the path plan, once per topology
+        if (dalloc(c, &c->plan_e, (size_t)N * E) || dalloc(c, &c->plan_l, (size_t)N * 64) ||
+            dalloc(c, &c->plan_n, (size_t)N))
+          return -3;
+        tangent_plan_kernel<<<nbw, 64 * kWaves, (size_t)kWaves * plan_slice_bytes(E), c->stream>>>(
+            c->bptr, c->nb0, c->p2, c->cptr, c->conns, N, E, c->plan_e, c->plan_l, c->plan_n);
+        LPMG_HIP(hipGetLastError());
+        c->plan_E = E;
+      }
+      if (c->dim == 3) {
+        assembly_prep_kernel<3><<<nbn, tb, 0, c->stream>>>(c->bptr, c->nb0, c->cs, c->bdamage, c->cs_sum, c->Kn,
+                                                          c->Tv, c->dpos, c->cons, c->damage_visual, c->floating,
+                                                          N, c->fixm, c->cs_wsum, c->diag_sq, c->Kval,
+                                                          c->partials);
+        norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
+        tangent_sym_kernel<3><<<nbw, 64 * kWaves, lds, c->stream>>>(
+            c->bptr, c->nb0, c->p2, c->rev, c->cs, c->bdamage, c->cs_sum, c->cs_wsum, c->Kn, c->Tv, c->cptr,
+            c->conns, c->fixm, c->sc, c->plan_e, c->plan_l, c->plan_n, N, E, NBM, c->Kval, k32);
+      } else {
+        assembly_prep_kernel<2><<<nbn, tb, 0, c->stream>>>(c->bptr, c->nb0, c->cs, c->bdamage, c->cs_sum, c->Kn,
+                                                          c->Tv, c->dpos, c->cons, c->damage_visual, c->floating,
+                                                          N, c->fixm, c->cs_wsum, c->diag_sq, c->Kval,
+                                                          c->partials);
+        norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
+        tangent_sym_kernel<2><<<nbw, 64 * kWaves, lds, c->stream>>>(
+            c->bptr, c->nb0, c->p2, c->rev, c->cs, c->bdamage, c->cs_sum, c->cs_wsum, c->Kn, c->Tv, c->cptr,
+            c->conns, c->fixm, c->sc, c->plan_e, c->plan_l, c->plan_n, N, E, NBM, c->Kval, k32);
+      }
+      LPMG_HIP(hipGetLastError());
+      c->assembled64 = true;
+      c->assembled32 = want32 != 0;
+      c->asm_version += 1;
+      return 0;
+    }
+  }
+  // two passes: unsymmetrised blocks, then symmetrisation and elimination
+  if (!c->Kloc && dalloc(c, &c->Kloc, (size_t)c->nnzb * c->dim * c->dim)) return -3;
   // LDS-staged tangent when a row's two-hop path data and direct terms fit
   // 64 KB per workgroup and its connections one wavefront
   const int nbm = c->max_nb;

@@ -107,6 +107,7 @@ struct Context {
   int64_t NB = 0, nnzb = 0, nnz_upper = 0;
   int max_nb = 0, max_nconn = 0;
   int max_cand = 0;  // largest sum over a particle's bonds of (1 + the bond end's same-layer bonds)
+  bool bonds_unique = true;  // no particle has two same-layer bonds to one particle, or a bond to itself
 
   // particles
   double4 *xyz = nullptr, *xyz0 = nullptr, *xyz_last = nullptr;
@@ -120,6 +121,7 @@ struct Context {
   double *sv = nullptr, *sv_last = nullptr;                 // 7 per particle
   double *dLe_tot = nullptr, *TdLe_tot = nullptr;           // [2i+L]
   double *cs_sum = nullptr;                                 // [(2i+L)*3+c]
+  double *cs_wsum = nullptr;                                // [(2i+L)*3+c], sum of (1 - bdamage) * cs
   int *type = nullptr, *cons = nullptr;                     // cons[3i+k]
   // material
   double *Kn = nullptr, *Tv = nullptr;                      // Kn[2i+L], Tv[i]
@@ -138,7 +140,13 @@ struct Context {
   int* fixm = nullptr;      // per particle: bit k set when DOF k is eliminated (assembly scratch)
   int64_t* Kp = nullptr;
   double* Kval = nullptr;   // nnzb*b*b
-  double* Kloc = nullptr;   // nnzb*b*b, the tangent's blocks before symmetrisation
+  double* Kloc = nullptr;   // nnzb*b*b, unsymmetrised tangent blocks (two-pass assembly only; lazy)
+  // one-pass assembly's path plan (topology only; built on first use):
+  // per row, path entries at stride plan_E, per connection lane words at stride 64
+  long long* plan_e = nullptr;
+  int* plan_l = nullptr;
+  int* plan_n = nullptr;
+  int plan_E = 0;
   float* Kval32 = nullptr;  // nnzb*b*b (mixed precision)
   double* diag_sq = nullptr;
   // nonlocal neighbours

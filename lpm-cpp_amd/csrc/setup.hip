@@ -250,20 +250,26 @@ __global__ void bond_init_kernel(const double4* __restrict__ xyz, const int* __r
 constexpr int kConnCap4 = 3072;
 constexpr int kConnCap1 = 12288;
 
+// cmax[0]: the largest candidate count; cmax[1] set to 1 when some particle has
+// two bonds of one layer to the same particle, or a bond to itself.
 __global__ void cand_count_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0,
                                   const int* __restrict__ p2, int N, int* __restrict__ cmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
   int C = 0;
+  bool dup = false;
   for (int L = 0; L < 2; ++L) {
     const int lo = L == 0 ? bptr[i] : bptr[i] + nb0[i];
     const int hi = L == 0 ? bptr[i] + nb0[i] : bptr[i + 1];
     for (int b = lo; b < hi; ++b) {
       const int k = p2[b];
       C += 1 + (L == 0 ? nb0[k] : bptr[k + 1] - bptr[k] - nb0[k]);
+      dup = dup || k == i;
+      for (int q = lo; q < b; ++q) dup = dup || p2[q] == k;
     }
   }
   atomicMax(cmax, C);
+  if (dup) atomicOr(cmax + 1, 1);
 }
 
 // Assembly::updateConnections: one wavefront per particle gathers the 1- and
@@ -544,13 +550,15 @@ int build_topology(Context* c) {
   LPMG_HIP(hipMallocAsync((void**)&nconn, sizeof(int) * N, c->stream));
   if (dalloc(c, &c->ncl, N) || dalloc(c, &c->cptr, N + 1) || dalloc(c, &c->dpos, N)) return -3;
   int* cmax = nullptr;
-  int hmax = 0;
-  LPMG_HIP(hipMallocAsync((void**)&cmax, sizeof(int), c->stream));
-  LPMG_HIP(hipMemsetAsync(cmax, 0, sizeof(int), c->stream));
+  int hcm[2] = {0, 0};
+  LPMG_HIP(hipMallocAsync((void**)&cmax, 2 * sizeof(int), c->stream));
+  LPMG_HIP(hipMemsetAsync(cmax, 0, 2 * sizeof(int), c->stream));
   cand_count_kernel<<<nb, tb, 0, c->stream>>>(c->bptr, c->nb0, c->p2, N, cmax);
   LPMG_HIP(hipGetLastError());
-  LPMG_HIP(copy_sync(c, &hmax, cmax, sizeof(int), hipMemcpyDeviceToHost));
+  LPMG_HIP(copy_sync(c, hcm, cmax, 2 * sizeof(int), hipMemcpyDeviceToHost));
   LPMG_HIP(hipFreeAsync(cmax, c->stream));
+  const int hmax = hcm[0];
+  c->bonds_unique = hcm[1] == 0;
   if (hmax > kConnCap1) {
     set_error("a particle has " + std::to_string(hmax) + " connection candidates, more than " +
               std::to_string(kConnCap1));

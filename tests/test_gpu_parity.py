@@ -174,6 +174,39 @@ def test_assembly_matches_oracle(name, L, gen):
 
 
 @pytest.mark.parametrize("name,L,gen", LATTICES)
+def test_stiffness_exactly_symmetric_with_one_sided_damage(name, L, gen):
+    # Bond damage set per direction (a bond broken on one side only, partial
+    # damage differing between a bond and its reverse), constrained DOFs: the
+    # assembled blocks match the oracle at 1e-13, and every block below the
+    # diagonal is bit for bit the transpose of the one above.  Column a of K is
+    # read exactly as K e_a (every other product is an exact zero).
+    xyz = _jitter(gen(), 4e-4 * R, seed=5)
+    g, o = _pair(L, xyz)
+    bptr, _, p2 = o.bonds()
+    rng = np.random.default_rng(17)
+    nb = len(p2)
+    bd = np.where(rng.random(nb) < 0.25, rng.random(nb) * 0.8, 0.0)
+    bd[rng.random(nb) < 0.04] = 1.0
+    cons = (rng.random((o.N, 3)) < 0.05).astype(float)
+    for s in (g, o):
+        s.set("BDAMAGE", bd)
+        s.set("CONSTRAINT", cons)
+        s.forward()
+        s.assemble()
+    assert _csr_close(g, o).all()
+    Kp, IK, JK, K = g.csr()
+    import scipy.sparse as sp
+    n = g.dim * g.N
+    U = sp.csr_matrix((K, (np.repeat(np.arange(n), np.diff(IK)), JK - 1)), shape=(n, n)).toarray()
+    for a in rng.choice(n, size=min(n, 24), replace=False):
+        x = np.zeros(n)
+        x[a] = 1.0
+        col = g.spmv(x, 0)
+        np.testing.assert_array_equal(col[a + 1:], U[a, a + 1:])
+        np.testing.assert_array_equal(col[:a + 1], U[:a + 1, a])
+
+
+@pytest.mark.parametrize("name,L,gen", LATTICES)
 def test_spmv_matches_oracle(name, L, gen):
     xyz = gen()
     g, o = _pair(L, xyz)
