@@ -836,8 +836,10 @@ __device__ __forceinline__ void direct_common_half(double kn, double tv, const d
     }
 }
 
-// Doubles per bond in the path table: c_t[3], om(t), c_rev t[3], om(rev t), Tv(k).
-constexpr int kSymTable = 9;
+// Doubles per bond t = i->k in the bond table: c_t[3], om(t), c_rev t[3],
+// om(rev t), Tv(k), and k's Q[3], S[3] and Kn of t's layer (for the direct term
+// of the lane whose connection is k).
+constexpr int kSymTable = 16;
 // Path entries per lane whose loads are issued before the first barrier (rows
 // of more entries take further rounds after it).
 constexpr int kSymGather = 3;
@@ -851,17 +853,20 @@ constexpr int kOneSided = 1 << 8;
 // connection is u's end m; entries with m == i belong to the diagonal block and
 // are left out.  Each owner's entries are ranked in bond order into slots:
 //   plan_e[i*E + e] = u | (t | owner << 6 | slot << 12) << 32, or -1,
+//   plan_r[i*E + e] = the reverse of bond u,
 //   plan_l[i*64 + lane] = first slot | count << 10 | (direct bond of layer 0
 //       + 1) << 17 | (direct bond of layer 1 + 1) << 24,
-//   plan_n[i] = number of entries.
+//   plan_n[i] = number of entries;
+// plan_smax[0] is raised to the largest number of slots of a row.
 __host__ __device__ constexpr size_t plan_slice_bytes(int E) {
   return ((size_t)64 * 8 + 6 * 64 * 4 + (size_t)2 * E + 15) / 16 * 16;
 }
 
 __global__ void __launch_bounds__(64 * kWaves)
 tangent_plan_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, const int* __restrict__ p2,
-                    const int* __restrict__ cptr, const int* __restrict__ conns, int N, int E,
-                    long long* __restrict__ plan_e, int* __restrict__ plan_l, int* __restrict__ plan_n) {
+                    const int* __restrict__ rev, const int* __restrict__ cptr, const int* __restrict__ conns,
+                    int N, int E, long long* __restrict__ plan_e, int* __restrict__ plan_r,
+                    int* __restrict__ plan_l, int* __restrict__ plan_n, int* __restrict__ plan_smax) {
   extern __shared__ double lds_plan[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + w);
@@ -927,20 +932,25 @@ tangent_plan_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, c
   __syncthreads();
   const unsigned long long mine = active ? Lmask[lane] : 0ull;
   const int cnt = __popcll(mine);
-  const int base = wave_incl_scan(cnt) - cnt;
+  const int incl_c = wave_incl_scan(cnt);
+  const int base = incl_c - cnt;
   Lbase[lane] = base;
   __syncthreads();
   if (!row) return;  // no barrier below
+  if (lane == 63) atomicMax(plan_smax, incl_c);
   for (int e = lane; e < T; e += 64) {
     const int v = Lot[e];
     long long rec = -1;
+    int ru = 0;
     if (v >= 0) {
       const int o = v >> 8, t = v & 255;
       const int slot = Lbase[o] + __popcll(Lmask[o] & ((1ull << t) - 1ull));
       const int u = Lklo[t] + (e - Loff[t]);
       rec = (long long)(unsigned)u | ((long long)(t | o << 6 | slot << 12) << 32);
+      ru = rev[u];
     }
     plan_e[(size_t)i * E + e] = rec;
+    plan_r[(size_t)i * E + e] = ru;
   }
   if (active)
     plan_l[(size_t)i * 64 + lane] = base | cnt << 10 | (Ldir[lane] + 1) << 17 | (Ldir[64 + lane] + 1) << 24;
@@ -948,12 +958,13 @@ tangent_plan_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, c
 }
 
 // Per wavefront: the bond table, a region holding first the path entries'
-// coefficients and second directions (slot order) and then the row's finished
-// blocks on their way to HBM, Tv of each connection, and per slot the bond
-// index and the bond t.
-__host__ __device__ constexpr size_t sym_slice_bytes(int E, int NBM, int DD) {
-  return ((size_t)kSymTable * NBM * 8 + (size_t)8 * (4 * E > 64 * DD ? 4 * E : 64 * DD) + 64 * 8 +
-          (size_t)4 * E + (size_t)2 * E + 15) / 16 * 16;
+// coefficients and second directions (S slots) and then the row's finished
+// blocks on their way to HBM, Tv of each connection, the diagonal block, and per
+// slot the bond index and the bond t.
+__host__ __device__ constexpr int sym_region(int S, int DD) { return 4 * S > 64 * DD ? 4 * S : 64 * DD; }
+__host__ __device__ constexpr size_t sym_slice_bytes(int S, int NBM, int DD) {
+  return ((size_t)kSymTable * NBM * 8 + (size_t)8 * sym_region(S, DD) + 64 * 8 + 16 * 8 + (size_t)4 * S +
+          (size_t)2 * S + 15) / 16 * 16;
 }
 
 template <int D>
@@ -963,70 +974,89 @@ tangent_sym_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, co
                    const double* __restrict__ cs_sum, const double* __restrict__ cs_wsum,
                    const double* __restrict__ Kn, const double* __restrict__ Tv, const int* __restrict__ cptr,
                    const int* __restrict__ conns, const int* __restrict__ fixm, const Scalars* __restrict__ sc,
-                   const long long* __restrict__ plan_e, const int* __restrict__ plan_l,
-                   const int* __restrict__ plan_n, int N, int E, int NBM, double* __restrict__ Kval,
+                   const long long* __restrict__ plan_e, const int* __restrict__ plan_r,
+                   const int* __restrict__ plan_l, const int* __restrict__ plan_n, int N, int E, int S,
+                   int NBM, double* __restrict__ Kval,
                    float* __restrict__ Kval32) {
   constexpr int DD = D * D;
   extern __shared__ double lds_sym[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + w);
   const bool row = i < N;  // wave-uniform; every wavefront still reaches the barriers
-  char* const slice = reinterpret_cast<char*>(lds_sym) + (size_t)w * sym_slice_bytes(E, NBM, DD);
+  char* const slice = reinterpret_cast<char*>(lds_sym) + (size_t)w * sym_slice_bytes(S, NBM, DD);
   double* const Lt = reinterpret_cast<double*>(slice);  // [kSymTable][NBM]
-  double* const Lreg = Lt + kSymTable * NBM;            // [max(4E, 64 DD)]
-  double* const Lcc = Lreg;                             // [E] combined path coefficient, by slot
-  double* const Lc3 = Lreg + E;                         // [E][3] direction of the path's second bond
-  double* const Ltvj = Lreg + (4 * E > 64 * DD ? 4 * E : 64 * DD);  // [64] Tv of each connection
-  int* const Lu = reinterpret_cast<int*>(Ltvj + 64);    // [E] the path's second bond, by slot
-  short* const Lst = reinterpret_cast<short*>(Lu + E);  // [E] bond t (| kOneSided), by slot
-  int c_lo = 0, nc = 0, b_lo = 0, n0 = 0, nbi = 0, T = 0;
-  double tv_i = 0;
+  double* const Lreg = Lt + kSymTable * NBM;            // [sym_region]
+  double* const Lcc = Lreg;                             // [S] combined path coefficient, by slot
+  double* const Lc3 = Lreg + S;                         // [S][3] direction of the path's second bond
+  double* const Ltvj = Lreg + sym_region(S, DD);        // [64] Tv of each connection
+  double* const Ldg = Ltvj + 64;                        // [16] the diagonal block
+  int* const Lu = reinterpret_cast<int*>(Ldg + 16);     // [S] the path's second bond, by slot
+  short* const Lst = reinterpret_cast<short*>(Lu + S);  // [S] bond t (| kOneSided), by slot
+  int c_lo = 0, nc = 0, b_lo = 0, n0 = 0, nbi = 0, T = 0, mi = 0;
+  double tv_i = 0, kni[2] = {0, 0}, Qi[2][3] = {}, Si[2][3] = {};
   if (row) {
     c_lo = cptr[i]; nc = cptr[i + 1] - c_lo;
     b_lo = bptr[i]; n0 = nb0[i]; nbi = bptr[i + 1] - b_lo;
     tv_i = Tv[i];
     T = plan_n[i];
+    mi = fixm[i];
+#pragma unroll
+    for (int L = 0; L < 2; ++L) {
+      kni[L] = Kn[2 * i + L];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        Qi[L][u] = cs_sum[(2 * i + L) * 3 + u];
+        Si[L][u] = cs_wsum[(2 * i + L) * 3 + u];
+      }
+    }
   }
   const bool active = row && lane < nc;
   const int pw = active ? plan_l[(size_t)i * 64 + lane] : 0;
   const int j = active ? conns[c_lo + lane] : -1;
   // the first round of path entries, loaded before the barrier
   long long rec[kSymGather];
+  int u3[kSymGather], r3[kSymGather];
 #pragma unroll
   for (int q = 0; q < kSymGather; ++q) {
     const int e = lane + 64 * q;
     rec[q] = e < T ? plan_e[(size_t)i * E + e] : -1ll;
+    r3[q] = e < T ? plan_r[(size_t)i * E + e] : 0;
   }
-  int u3[kSymGather], r3[kSymGather];
   double c3[kSymGather][3], cr3[kSymGather][3], om3[kSymGather], omr3[kSymGather];
 #pragma unroll
   for (int q = 0; q < kSymGather; ++q) {
     u3[q] = rec[q] >= 0 ? (int)(unsigned)(rec[q] & 0xffffffffll) : 0;
-    r3[q] = rev[u3[q]];
 #pragma unroll
-    for (int u = 0; u < 3; ++u) c3[q][u] = cs[3 * u3[q] + u];
+    for (int u = 0; u < 3; ++u) {
+      c3[q][u] = cs[3 * u3[q] + u];
+      cr3[q][u] = cs[3 * r3[q] + u];
+    }
     om3[q] = 1 - bdamage[u3[q]];
-  }
-#pragma unroll
-  for (int q = 0; q < kSymGather; ++q) {
-#pragma unroll
-    for (int u = 0; u < 3; ++u) cr3[q][u] = cs[3 * r3[q] + u];
     omr3[q] = 1 - bdamage[r3[q]];
   }
-  // lane t: bond t of i and its reverse
+  // lane t: bond t of i, its reverse, and its end's direct-term inputs
   if (lane < nbi) {
-    const int b = b_lo + lane, rb = rev[b];
+    const int b = b_lo + lane, rb = rev[b], k = p2[b];
+    const int L = lane < n0 ? 0 : 1;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
       Lt[q * NBM + lane] = cs[3 * b + q];
       Lt[(4 + q) * NBM + lane] = cs[3 * rb + q];
+      Lt[(9 + q) * NBM + lane] = cs_sum[(2 * k + L) * 3 + q];
+      Lt[(12 + q) * NBM + lane] = cs_wsum[(2 * k + L) * 3 + q];
     }
     Lt[3 * NBM + lane] = 1 - bdamage[b];
     Lt[7 * NBM + lane] = 1 - bdamage[rb];
-    Lt[8 * NBM + lane] = Tv[p2[b]];
+    Lt[8 * NBM + lane] = Tv[k];
+    Lt[15 * NBM + lane] = Kn[2 * k + L];
   }
   Ltvj[lane] = active ? Tv[j] : 0.0;
   const int mj = active ? fixm[j] : 0;
+  if (active && j == i) {  // the diagonal block from the per-particle pass
+    const double* in = Kval + (size_t)(c_lo + lane) * DD;
+#pragma unroll
+    for (int q = 0; q < DD; ++q) Ldg[q] = in[q];
+  }
   __syncthreads();
   // Each entry's coefficient 0.5 (cA + cB) and second direction go to its slot.
   auto place = [&](long long r, const double* c3e, const double* cr3e, double om3e, double omr3e) {
@@ -1051,7 +1081,7 @@ tangent_sym_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, co
   for (int e = lane + 64 * kSymGather; e < T; e += 64) {  // rows with more entries
     const long long r = plan_e[(size_t)i * E + e];
     if (r < 0) continue;
-    const int u = (int)(unsigned)(r & 0xffffffffll), ru = rev[u];
+    const int u = (int)(unsigned)(r & 0xffffffffll), ru = plan_r[(size_t)i * E + e];
     double a[3], b[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -1062,18 +1092,16 @@ tangent_sym_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, co
   }
   __syncthreads();
   if (!row) return;  // no barrier below
-  const int mi = fixm[i];
   double K[D][D];
 #pragma unroll
   for (int r = 0; r < D; ++r)
 #pragma unroll
     for (int s = 0; s < D; ++s) K[r][s] = 0.0;
   if (active && j == i) {  // the diagonal block from the per-particle pass, eliminated
-    const double* in = Kval + (size_t)(c_lo + lane) * DD;
 #pragma unroll
     for (int r = 0; r < D; ++r)
 #pragma unroll
-      for (int s = 0; s < D; ++s) K[r][s] = in[r * D + s];
+      for (int s = 0; s < D; ++s) K[r][s] = Ldg[r * D + s];
     if (mi) {
       const double nd = sc->norm_diag;
 #pragma unroll
@@ -1094,20 +1122,18 @@ tangent_sym_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, co
         for (int s = 0; s < D; ++s) acc[r][s] = 0.0;
       const int t1 = ((pw >> (L == 0 ? 17 : 24)) & 127) - 1;
       bool found = t1 >= 0;
-      if (found) {
-        double Qi[3], Si[3], Qj[3], Sj[3], c1[3], cr1[3];
+      if (found) {  // bond t1 = i -> j; the table holds j's inputs for t1's layer
+        double Qj[3], Sj[3], c1[3], cr1[3];
 #pragma unroll
         for (int u = 0; u < 3; ++u) {
-          Qi[u] = cs_sum[(2 * i + L) * 3 + u];
-          Si[u] = cs_wsum[(2 * i + L) * 3 + u];
-          Qj[u] = cs_sum[(2 * j + L) * 3 + u];
-          Sj[u] = cs_wsum[(2 * j + L) * 3 + u];
+          Qj[u] = Lt[(9 + u) * NBM + t1];
+          Sj[u] = Lt[(12 + u) * NBM + t1];
           c1[u] = Lt[u * NBM + t1];
           cr1[u] = Lt[(4 + u) * NBM + t1];
         }
         double A[D][D], B[D][D];
-        direct_common_half<D>(Kn[2 * i + L], tv_i, Qi, Si, Qj, c1, Lt[3 * NBM + t1], A);
-        direct_common_half<D>(Kn[2 * j + L], tv_j, Qj, Sj, Qi, cr1, Lt[7 * NBM + t1], B);
+        direct_common_half<D>(kni[L], tv_i, Qi[L], Si[L], Qj, c1, Lt[3 * NBM + t1], A);
+        direct_common_half<D>(Lt[15 * NBM + t1], tv_j, Qj, Sj, Qi[L], cr1, Lt[7 * NBM + t1], B);
 #pragma unroll
         for (int r = 0; r < D; ++r)
 #pragma unroll
@@ -1201,16 +1227,28 @@ int launch_assemble(Context* c, int want32) {
     const int E = std::max(32, (c->max_cand + 7) / 8 * 8);
     const int NBM = std::max(1, c->max_nb);
     const int DD = c->dim * c->dim;
-    const size_t lds = (size_t)kWaves * sym_slice_bytes(E, NBM, DD);
-    if (c->bonds_unique && c->max_nb <= 64 && c->max_nconn <= 64 && E <= 1023 && lds <= 65536) {
+    // slots per row never exceed E, so this bounds the LDS of any plan
+    if (c->bonds_unique && c->max_nb <= 64 && c->max_nconn <= 64 && E <= 1023 &&
+        (size_t)kWaves * sym_slice_bytes(E, NBM, DD) <= 65536) {
       if (c->plan_E != E) {  // the path plan, once per topology
-        if (dalloc(c, &c->plan_e, (size_t)N * E) || dalloc(c, &c->plan_l, (size_t)N * 64) ||
-            dalloc(c, &c->plan_n, (size_t)N))
+        if (dalloc(c, &c->plan_e, (size_t)N * E) || dalloc(c, &c->plan_r, (size_t)N * E) ||
+            dalloc(c, &c->plan_l, (size_t)N * 64) || dalloc(c, &c->plan_n, (size_t)N + 1))
           return -3;
+        LPMG_HIP(hipMemsetAsync(c->plan_n + N, 0, sizeof(int), c->stream));
         tangent_plan_kernel<<<nbw, 64 * kWaves, (size_t)kWaves * plan_slice_bytes(E), c->stream>>>(
-            c->bptr, c->nb0, c->p2, c->cptr, c->conns, N, E, c->plan_e, c->plan_l, c->plan_n);
+            c->bptr, c->nb0, c->p2, c->rev, c->cptr, c->conns, N, E, c->plan_e, c->plan_r, c->plan_l, c->plan_n,
+            c->plan_n + N);
         LPMG_HIP(hipGetLastError());
+        int smax = 0;
+        LPMG_HIP(copy_sync(c, &smax, c->plan_n + N, sizeof(int), hipMemcpyDeviceToHost));
         c->plan_E = E;
+        c->plan_S = std::max(1, smax);
+      }
+      const int S = c->plan_S;
+      const size_t lds = (size_t)kWaves * sym_slice_bytes(S, NBM, DD);
+      if (lds > 65536) {
+        set_error("one-pass assembly: path data exceeds LDS");
+        return -1;
       }
       if (c->dim == 3) {
         assembly_prep_kernel<3><<<nbn, tb, 0, c->stream>>>(c->bptr, c->nb0, c->cs, c->bdamage, c->cs_sum, c->Kn,
@@ -1220,7 +1258,7 @@ int launch_assemble(Context* c, int want32) {
         norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
         tangent_sym_kernel<3><<<nbw, 64 * kWaves, lds, c->stream>>>(
             c->bptr, c->nb0, c->p2, c->rev, c->cs, c->bdamage, c->cs_sum, c->cs_wsum, c->Kn, c->Tv, c->cptr,
-            c->conns, c->fixm, c->sc, c->plan_e, c->plan_l, c->plan_n, N, E, NBM, c->Kval, k32);
+            c->conns, c->fixm, c->sc, c->plan_e, c->plan_r, c->plan_l, c->plan_n, N, E, S, NBM, c->Kval, k32);
       } else {
         assembly_prep_kernel<2><<<nbn, tb, 0, c->stream>>>(c->bptr, c->nb0, c->cs, c->bdamage, c->cs_sum, c->Kn,
                                                           c->Tv, c->dpos, c->cons, c->damage_visual, c->floating,
@@ -1229,7 +1267,7 @@ int launch_assemble(Context* c, int want32) {
         norm_diag_finish_kernel<<<1, tb, 0, c->stream>>>(c->partials, nbn, c->sc);
         tangent_sym_kernel<2><<<nbw, 64 * kWaves, lds, c->stream>>>(
             c->bptr, c->nb0, c->p2, c->rev, c->cs, c->bdamage, c->cs_sum, c->cs_wsum, c->Kn, c->Tv, c->cptr,
-            c->conns, c->fixm, c->sc, c->plan_e, c->plan_l, c->plan_n, N, E, NBM, c->Kval, k32);
+            c->conns, c->fixm, c->sc, c->plan_e, c->plan_r, c->plan_l, c->plan_n, N, E, S, NBM, c->Kval, k32);
       }
       LPMG_HIP(hipGetLastError());
       c->assembled64 = true;

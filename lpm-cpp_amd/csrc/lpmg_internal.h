@@ -144,9 +144,10 @@ struct Context {
   // one-pass assembly's path plan (topology only; built on first use):
   // per row, path entries at stride plan_E, per connection lane words at stride 64
   long long* plan_e = nullptr;
+  int* plan_r = nullptr;
   int* plan_l = nullptr;
-  int* plan_n = nullptr;
-  int plan_E = 0;
+  int* plan_n = nullptr;    // N + 1: the last holds the largest slot count of a row
+  int plan_E = 0, plan_S = 0;
   float* Kval32 = nullptr;  // nnzb*b*b (mixed precision)
   double* diag_sq = nullptr;
   // nonlocal neighbours
