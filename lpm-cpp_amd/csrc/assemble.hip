@@ -981,7 +981,7 @@ tangent_sym_kernel(const int* __restrict__ bptr, const int* __restrict__ nb0, co
   constexpr int DD = D * D;
   extern __shared__ double lds_sym[];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int i = __builtin_amdgcn_readfirstlane(blockIdx.x * kWaves + w);
+  const int i = __builtin_amdgcn_readfirstlane(xcd_contiguous(blockIdx.x, gridDim.x) * kWaves + w);
   const bool row = i < N;  // wave-uniform; every wavefront still reaches the barriers
   char* const slice = reinterpret_cast<char*>(lds_sym) + (size_t)w * sym_slice_bytes(S, NBM, DD);
   double* const Lt = reinterpret_cast<double*>(slice);  // [kSymTable][NBM]

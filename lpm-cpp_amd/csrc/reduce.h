@@ -17,6 +17,18 @@ struct Vec {
   double v[NV];
 };
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup b to XCD b % 8),
+// and each XCD has its own L2.  This maps the launch's workgroup b to a virtual
+// index such that each XCD takes one contiguous range of about n / 8 virtual
+// workgroups, in order: kernels whose neighbouring rows read the same data
+// (particle neighbourhoods) then share it in one L2.  A bijection on [0, n).
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+  constexpr int kXcd = 8;
+  const int q = n / kXcd, rem = n % kXcd;
+  const int x = b % kXcd, l = b / kXcd;
+  return x * q + min(x, rem) + l;
+}
+
 template <int NV>
 __device__ inline Vec<NV> wave_sum(Vec<NV> a) {
 #pragma unroll
