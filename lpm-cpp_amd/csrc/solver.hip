@@ -44,10 +44,12 @@ namespace lpmg {
 
 namespace {
 
-// CG vector step: one element per thread, 16 wavefronts per workgroup (the
-// workgroup also sums the previous kernel's partials, so fewer, fuller
-// workgroups cost less than many small ones)
-constexpr int kVBS = 1024;
+// CG vector step: one element per thread, 4 wavefronts per workgroup.  Every
+// workgroup also sums the previous kernel's partials, but small systems need
+// the workgroups to fill the CUs: per Jacobi-PCG iteration, 1024 / 512 / 256 /
+// 128 threads took 53.1 / 52.7 / 52.1 / 52.3 us on SC 40^3 (FP64) and
+// 12.4 / 10.3 / 9.9 / 10.0 us on a 20k-particle CT plate (FP32).
+constexpr int kVBS = 256;
 
 template <int D, typename T, int U>
 __global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
