@@ -115,3 +115,51 @@ def test_bcc_polycrystal_structure():
     np.testing.assert_allclose(d[deep, 1], 2 * r, rtol=1e-12)
     pc2 = G.bcc_polycrystal(12, 10, radius=r, seed=3)
     assert np.array_equal(pc2["xyz"], xyz)
+
+
+def _elemental(axis, a):
+    """Active rotation by angle a about a fixed axis (right-handed)."""
+    c, s = np.cos(a), np.sin(a)
+    if axis == "x":
+        return np.array([[1, 0, 0], [0, c, -s], [0, s, c]])
+    if axis == "y":
+        return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+    return np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_rotation_matrix_is_the_product_of_elemental_rotations(seed):
+    # createRMatrix (include/utilities.h:15-91) expands a product of elemental
+    # rotations: flag 0 = X(a1) Y(a2) Z(a3), flag 2 (Bunge) = Z(a1) X(a2) Z(a3),
+    # flag 1 (Kocks) = Z(a1) Y(a2) Z(pi - a3).  The restatement's expanded
+    # entries must equal the matrix products; a swapped sign, entry or angle fails.
+    rng = np.random.default_rng(seed)
+    a1, a2, a3 = rng.uniform(-np.pi, np.pi, 3)
+    want = {0: _elemental("x", a1) @ _elemental("y", a2) @ _elemental("z", a3),
+            1: _elemental("z", a1) @ _elemental("y", a2) @ _elemental("z", np.pi - a3),
+            2: _elemental("z", a1) @ _elemental("x", a2) @ _elemental("z", a3)}
+    for flag, M in want.items():
+        R = lat.create_rmatrix(flag, [a1, a2, a3]).reshape(3, 3)
+        np.testing.assert_allclose(R, M, atol=2e-15, rtol=0)
+    # 2D lattices use only the third angle: a rotation in the plane
+    R = lat.create_rmatrix(0, [0.0, 0.0, a3]).reshape(3, 3)
+    np.testing.assert_allclose(R[:2, :2] @ [1.0, 0.0], [np.cos(a3), np.sin(a3)], atol=1e-15)
+    with pytest.raises(ValueError):
+        lat.create_rmatrix(3, [a1, a2, a3])
+
+
+def test_rotated_lattice_is_the_identity_lattice_rotated():
+    # createCuboidSC3D rotates a block of lattice points about the origin and
+    # keeps those inside the box: every kept point, rotated back, lies on the
+    # unrotated lattice x0 + 2r (i, j, k) with x0 the identity generator's origin
+    r = 0.1
+    box = [0.0, 1.0, 0.0, 1.0, 0.0, 1.0]
+    R = lat.create_rmatrix(2, [np.radians(30.0), np.radians(45.0), np.radians(60.0)])
+    pts = lat.create_cuboid_sc3d(box, r, R)
+    back = pts @ R.reshape(3, 3)  # R^T p for each row p
+    ident = lat.create_cuboid_sc3d(box, r)
+    origin = ident.min(axis=0)
+    k = (back - origin) / (2 * r)
+    np.testing.assert_allclose(k, np.round(k), atol=1e-9)
+    lo, hi = np.array(box[0::2]), np.array(box[1::2])
+    assert np.all(pts >= lo - 1e-12) and np.all(pts <= hi + 1e-12)
