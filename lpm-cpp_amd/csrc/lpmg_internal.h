@@ -35,6 +35,7 @@ struct Scalars {
   double rz, rr, rr0, alpha, beta, pq, thresh;  // CG (last iteration)
   double rtol, atol;                            // stopping test of the current solve
   double gam[2], alp[2];                        // (r,u) and alpha of iteration i in slot i & 1
+  double vsum[2];                               // (r,u), (r,r) of the last vector step, summed
   double rnorm_sq, reac_sq;                     // residual / reaction (updateRR)
   double norm_diag;                             // updateStiffnessDispBC
   int iter, done, maxit, any_flag;

@@ -17,8 +17,10 @@
 // iteration one SpMV kernel (w = A u, with the partials of (w,u)) and one fused
 // vector kernel (p = u + beta p; s = w + beta s; x += alpha p; r -= alpha s;
 // u = M r; partials of (r,u) and (r,r)).  No kernel waits on other workgroups of
-// its own launch: each kernel writes one partial per workgroup and the next
-// kernel sums them in a fixed order in every workgroup (deterministic), so no
+// its own launch: each kernel writes one partial per workgroup and a later
+// kernel sums them in a fixed order (deterministic) -- the SpMV's partials in
+// every workgroup of the vector step, the vector step's in the next SpMV's
+// workgroup 0, which leaves the two totals for the next vector step -- so no
 // intra-kernel fence is ever issued (on gfx950 an agent-scope fence writes back
 // and invalidates the XCD's L2).  Scalars that live across iterations are written
 // by workgroup 0 into slots indexed by the iteration parity.  The stopping test
@@ -45,10 +47,15 @@ namespace lpmg {
 namespace {
 
 // CG vector step: one element per thread, 4 wavefronts per workgroup.  Every
-// workgroup also sums the previous kernel's partials, but small systems need
-// the workgroups to fill the CUs: per Jacobi-PCG iteration, 1024 / 512 / 256 /
+// workgroup also sums the SpMV's partials, but small systems need the
+// workgroups to fill the CUs: per Jacobi-PCG iteration, 1024 / 512 / 256 /
 // 128 threads took 53.1 / 52.7 / 52.1 / 52.3 us on SC 40^3 (FP64) and
-// 12.4 / 10.3 / 9.9 / 10.0 us on a 20k-particle CT plate (FP32).
+// 12.4 / 10.3 / 9.9 / 10.0 us on a 20k-particle CT plate (FP32), when each
+// workgroup still summed the vector step's own partials too.  With those
+// summed once by the SpMV's workgroup 0 and u no longer read: 51.1 us (FP64)
+// and 30.0 us (FP32) per iteration on SC 40^3, from 52.2 and 31.2 in the same
+// run (fixed 1,600-iteration solves); the vector step is 7.0 us of it, about
+// 4 us of which an empty launch of the same shape also takes.
 constexpr int kVBS = 256;
 
 template <int D, typename T, int U>
@@ -64,13 +71,25 @@ __global__ void __launch_bounds__(kSpBS) spmv_kernel(const int* __restrict__ cpt
 // other systems share its launches.
 
 // SpMV: w = A u and the per-workgroup partials of (w, u).  `done` is written
-// only by the vector step, never during this launch.
+// only by the vector step, never during this launch.  Workgroup 0 also sums
+// the last vector step's nV partials of (r,u), (r,r) into sc->vsum, so the
+// next vector step reads two totals instead of every partial (one memory
+// round trip less in each of its workgroups); nothing else of this launch
+// reads vsum.
 template <int D, typename T, int U>
 __device__ __forceinline__ void cg_spmv_body(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                              const T* __restrict__ val, const T* __restrict__ u,
-                                             T* __restrict__ w, int N, double* __restrict__ partS, Scalars* sc) {
+                                             T* __restrict__ w, int N, double* __restrict__ partS,
+                                             const double* __restrict__ partV, int nV, Scalars* sc) {
   if (sc->done) return;
-  if (blk == 0 && threadIdx.x == 0) sc->icur = sc->inext;  // iteration index of the next vector step
+  if (blk == 0) {
+    if (threadIdx.x == 0) sc->icur = sc->inext;  // iteration index of the next vector step
+    const Vec<2> v = sum_partials<kSpBS, 2>(partV, nV);
+    if (threadIdx.x == 0) {
+      sc->vsum[0] = v.v[0];
+      sc->vsum[1] = v.v[1];
+    }
+  }
   Vec<1> a;
   a.v[0] = bsr_rows<D, T, kLPR, U, true>(blk, cptr, conns, val, u, w, N);
   write_partial_at<kSpBS, 1>(partS, blk, a);
@@ -122,69 +141,52 @@ __device__ __forceinline__ CGStep cg_step(const Vec<3>& a, const Scalars* sc, in
 // partials and returns without writing.
 //
 // Every load the step needs is issued before its first barrier (whose fence
-// waits for them): this thread's first element, its share of the partials and,
-// in thread 0, the flag -- one memory latency instead of three in a row.  The
-// partial sums keep the per-thread order and the wavefront-then-LDS tree of
-// block_sum, so their bits are unchanged.
+// waits for them): this thread's first element, its share of the SpMV's
+// partials, the two totals of (r,u), (r,r) that the SpMV's workgroup 0 formed
+// and, in thread 0, the flag -- one memory latency instead of several in a
+// row.  u = M r is not read: the last step stored u = dinv * r from the same r
+// and dinv, so the product gives its bits again.
 template <typename T>
 __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x, T* __restrict__ r,
                                             T* __restrict__ u, T* __restrict__ p, T* __restrict__ s,
                                             const T* __restrict__ w, const T* __restrict__ dinv, int64_t n,
-                                            const double* __restrict__ partV_in, int nV,
                                             const double* __restrict__ partS, int nS,
                                             double* __restrict__ partV_out, Scalars* sc) {
   __shared__ int done_s;
-  __shared__ double red[kVBS / 64][3];
+  __shared__ double red[kVBS / 64];
   const int64_t k0 = (int64_t)blk * kVBS + threadIdx.x;
   const bool has0 = k0 < n;
-  T u0 = 0, p0 = 0, w0 = 0, s0 = 0, x0 = 0, r0 = 0, d0 = 0;
+  T p0 = 0, w0 = 0, s0 = 0, x0 = 0, r0 = 0, d0 = 0;
   if (has0) {
-    u0 = u[k0]; p0 = p[k0]; w0 = w[k0]; s0 = s[k0]; x0 = x[k0]; r0 = r[k0]; d0 = dinv[k0];
+    p0 = p[k0]; w0 = w[k0]; s0 = s[k0]; x0 = x[k0]; r0 = r[k0]; d0 = dinv[k0];
   }
-  Vec<3> a;
-  a.v[0] = a.v[1] = a.v[2] = 0;
-  if (nV <= kVBS && nS <= 8 * kVBS) {  // one share of each set: all loads in flight at once
+  const double gam = sc->vsum[0], rrt = sc->vsum[1];
+  Vec<1> a;
+  a.v[0] = 0;
+  if (nS <= 8 * kVBS) {  // one share of the set: all loads in flight at once
     // unconditional loads (indices past the end read the last entry and are
     // not added): a load inside a branch is waited for at the branch's end
     const int b = threadIdx.x;
-    const bool hv = b < nV;
-    const int bv = hv ? b : max(nV - 1, 0);
-    const double v0 = partV_in[2 * (size_t)bv], v1 = partV_in[2 * (size_t)bv + 1];
     double sv[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) sv[j] = partS[min(b + j * kVBS, max(nS - 1, 0))];
-    if (hv) {
-      a.v[0] += v0;
-      a.v[1] += v1;
-    }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      if (b + j * kVBS < nS) a.v[2] += sv[j];
+      if (b + j * kVBS < nS) a.v[0] += sv[j];
   } else {
-    Vec<2> av;
-    av.v[0] = av.v[1] = 0;
-    add_partials<kVBS, 2>(partV_in, nV, av);
-    Vec<1> as;
-    as.v[0] = 0;
-    add_partials<kVBS, 1>(partS, nS, as);
-    a.v[0] = av.v[0];
-    a.v[1] = av.v[1];
-    a.v[2] = as.v[0];
+    add_partials<kVBS, 1>(partS, nS, a);
   }
   const int done0 = threadIdx.x == 0 ? sc->done : 0;
   a = wave_sum(a);
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) red[threadIdx.x >> 6][k] = a.v[k];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a.v[0];
   if (threadIdx.x == 0) done_s = done0;
   __syncthreads();
   if (done_s) return;
   Vec<3> tot;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    tot.v[k] = 0;
-    for (int q = 0; q < kVBS / 64; ++q) tot.v[k] += red[q][k];
-  }
+  tot.v[0] = gam;
+  tot.v[1] = rrt;
+  tot.v[2] = 0;
+  for (int q = 0; q < kVBS / 64; ++q) tot.v[2] += red[q];
   const int i = sc->icur;
   const CGStep st = cg_step(tot, sc, i);
   if (blk == 0 && threadIdx.x == 0) {
@@ -207,6 +209,7 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
   const T alpha = (T)st.alpha, beta = (T)st.beta;
   double ru = 0, rr = 0;
   if (has0) {
+    const T u0 = d0 * r0;
     const T pk = u0 + beta * p0;
     const T sk = w0 + beta * s0;
     p[k0] = pk;
@@ -220,14 +223,15 @@ __device__ __forceinline__ void cg_vec_body(int blk, int nblk, T* __restrict__ x
     rr += (double)rk * (double)rk;
   }
   for (int64_t k = k0 + (int64_t)nblk * kVBS; k < n; k += (int64_t)nblk * kVBS) {
-    const T pk = u[k] + beta * p[k];
+    const T dk = dinv[k], ro = r[k];
+    const T pk = dk * ro + beta * p[k];
     const T sk = w[k] + beta * s[k];
     p[k] = pk;
     s[k] = sk;
     x[k] += alpha * pk;
-    const T rk = r[k] - alpha * sk;
+    const T rk = ro - alpha * sk;
     r[k] = rk;
-    const T uk = dinv[k] * rk;
+    const T uk = dk * rk;
     u[k] = uk;
     ru += (double)rk * (double)uk;
     rr += (double)rk * (double)rk;
@@ -243,8 +247,8 @@ template <int D, typename T, int U>
 __global__ void __launch_bounds__(kSpBS) cg_spmv_kernel(const int* __restrict__ cptr, const int* __restrict__ conns,
                                                         const T* __restrict__ val, const T* __restrict__ u,
                                                         T* __restrict__ w, int N, double* __restrict__ partS,
-                                                        Scalars* sc) {
-  cg_spmv_body<D, T, U>(blockIdx.x, cptr, conns, val, u, w, N, partS, sc);
+                                                        const double* __restrict__ partV, int nV, Scalars* sc) {
+  cg_spmv_body<D, T, U>(blockIdx.x, cptr, conns, val, u, w, N, partS, partV, nV, sc);
 }
 
 __global__ void __launch_bounds__(kBS) reduce_partials_kernel(const double* __restrict__ in, int n,
@@ -257,10 +261,9 @@ template <typename T>
 __global__ void __launch_bounds__(kVBS) cg_vec_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ u,
                                                      T* __restrict__ p, T* __restrict__ s,
                                                      const T* __restrict__ w, const T* __restrict__ dinv,
-                                                     int64_t n, const double* __restrict__ partV_in, int nV,
-                                                     const double* __restrict__ partS, int nS,
+                                                     int64_t n, const double* __restrict__ partS, int nS,
                                                      double* __restrict__ partV_out, Scalars* sc) {
-  cg_vec_body<T>(blockIdx.x, gridDim.x, x, r, u, p, s, w, dinv, n, partV_in, nV, partS, nS, partV_out, sc);
+  cg_vec_body<T>(blockIdx.x, gridDim.x, x, r, u, p, s, w, dinv, n, partS, nS, partV_out, sc);
 }
 
 // ---- fused launches over the systems of a CgSys table.  Workgroup g belongs
@@ -281,11 +284,11 @@ __device__ __forceinline__ int cg_sys_of(const CgSys* __restrict__ tab, int nsys
 // workgroups past them return before any barrier.
 template <int D, typename T, int U>
 __global__ void __launch_bounds__(kSpBS) fused_cg_spmv_kernel(const CgBatchHdr* __restrict__ hdr,
-                                                              const CgSys* __restrict__ tab) {
+                                                              const CgSys* __restrict__ tab, int slot) {
   if ((int)blockIdx.x >= hdr->sp_blocks) return;
   const CgSys& q = tab[cg_sys_of<0>(tab, hdr->nsys, blockIdx.x)];
   cg_spmv_body<D, T, U>(blockIdx.x - q.sp_begin, q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
-                          q.partS_raw, q.sc);
+                          q.partS_raw, slot ? q.partV1 : q.partV0, q.gv, q.sc);
 }
 
 __global__ void __launch_bounds__(kBS) fused_reduce_partials_kernel(const CgBatchHdr* __restrict__ hdr,
@@ -302,8 +305,7 @@ __global__ void __launch_bounds__(kVBS) fused_cg_vec_kernel(const CgBatchHdr* __
   if ((int)blockIdx.x >= hdr->vec_blocks) return;
   const CgSys& q = tab[cg_sys_of<2>(tab, hdr->nsys, blockIdx.x)];
   cg_vec_body<T>(blockIdx.x - q.vec_begin, q.gv, (T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
-                 (const T*)q.dinv, q.n, slot ? q.partV1 : q.partV0, q.gv, q.partS, q.nS,
-                 slot ? q.partV0 : q.partV1, q.sc);
+                 (const T*)q.dinv, q.n, q.partS, q.nS, slot ? q.partV0 : q.partV1, q.sc);
 }
 
 // done flag and iteration count of every system of the table
@@ -397,22 +399,25 @@ void launch_bsr(Context* c, const T* val, const T* x, T* y) {
 // With events, the kernel's own start and end are recorded with its dispatch
 // (hipExtLaunchKernelGGL): the interval is the kernel, without the gaps that
 // separately enqueued event records add.
+// Iteration k of a chunk reads the vector-step partials of slot k & 1.
 template <int D, typename T, int U>
-void launch_cg_spmv_u(hipStream_t st, const CgSys& q, hipEvent_t e0, hipEvent_t e1) {
+void launch_cg_spmv_u(hipStream_t st, const CgSys& q, int k, hipEvent_t e0, hipEvent_t e1) {
+  const double* partV = (k & 1) ? q.partV1 : q.partV0;
   if (e0)  // instrumented (eager) launches only; captured chunks use plain launches
     hipExtLaunchKernelGGL(cg_spmv_kernel<D, T, U>, dim3(q.gs), dim3(kSpBS), 0, st, e0, e1, 0, q.cptr, q.conns,
-                          (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, q.sc);
+                          (const T*)q.val, (const T*)q.u, (T*)q.w, q.N, q.partS_raw, partV, q.gv, q.sc);
   else
     cg_spmv_kernel<D, T, U><<<q.gs, kSpBS, 0, st>>>(q.cptr, q.conns, (const T*)q.val, (const T*)q.u, (T*)q.w, q.N,
-                                                    q.partS_raw, q.sc);
+                                                    q.partS_raw, partV, q.gv, q.sc);
 }
 
 template <int D, typename T>
-void launch_cg_spmv(hipStream_t st, const CgSys& q, int chunks, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
+void launch_cg_spmv(hipStream_t st, const CgSys& q, int chunks, int k, hipEvent_t e0 = nullptr,
+                    hipEvent_t e1 = nullptr) {
   switch (chunks) {
-    case 2: launch_cg_spmv_u<D, T, 2>(st, q, e0, e1); break;
-    case 3: launch_cg_spmv_u<D, T, 3>(st, q, e0, e1); break;
-    default: launch_cg_spmv_u<D, T, 4>(st, q, e0, e1); break;
+    case 2: launch_cg_spmv_u<D, T, 2>(st, q, k, e0, e1); break;
+    case 3: launch_cg_spmv_u<D, T, 3>(st, q, k, e0, e1); break;
+    default: launch_cg_spmv_u<D, T, 4>(st, q, k, e0, e1); break;
   }
 }
 
@@ -470,13 +475,13 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
   auto enqueue_chunk = [&](std::vector<hipEvent_t>* evs) -> int {
     for (int k = 0; k < kCgChunk; ++k) {
       if (evs)
-        launch_cg_spmv<D, T>(c->stream, q, chunks, (*evs)[2 * k], (*evs)[2 * k + 1]);
+        launch_cg_spmv<D, T>(c->stream, q, chunks, k, (*evs)[2 * k], (*evs)[2 * k + 1]);
       else
-        launch_cg_spmv<D, T>(c->stream, q, chunks);
+        launch_cg_spmv<D, T>(c->stream, q, chunks, k);
       if (q.two_level) reduce_partials_kernel<<<kBS, kBS, 0, c->stream>>>(q.partS_raw, q.gs, q.partS, q.sc);
       cg_vec_kernel<T><<<q.gv, kVBS, 0, c->stream>>>((T*)q.x, (T*)q.r, (T*)q.u, (T*)q.p, (T*)q.s, (const T*)q.w,
-                                                    (const T*)q.dinv, q.n, (k & 1) ? q.partV1 : q.partV0, q.gv,
-                                                    q.partS, q.nS, (k & 1) ? q.partV0 : q.partV1, q.sc);
+                                                    (const T*)q.dinv, q.n, q.partS, q.nS,
+                                                    (k & 1) ? q.partV0 : q.partV1, q.sc);
     }
     LPMG_HIP(hipGetLastError());
     return 0;
@@ -536,7 +541,7 @@ int run_cg(Context* c, const CgSys& q, int* iters, double* rr_final) {
 template <int D, typename T, int U>
 void launch_fused_class(hipStream_t st, const CgBatchClass& b, int k) {
   const CgSys* tab = reinterpret_cast<const CgSys*>(reinterpret_cast<const char*>(b.hdr) + kCgHdrBytes);
-  fused_cg_spmv_kernel<D, T, U><<<b.sp_grid, kSpBS, 0, st>>>(b.hdr, tab);
+  fused_cg_spmv_kernel<D, T, U><<<b.sp_grid, kSpBS, 0, st>>>(b.hdr, tab, k & 1);
   if (b.red_grid) fused_reduce_partials_kernel<<<b.red_grid, kBS, 0, st>>>(b.hdr, tab);
   fused_cg_vec_kernel<T><<<b.vec_grid, kVBS, 0, st>>>(b.hdr, tab, k & 1);
 }
