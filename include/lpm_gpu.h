@@ -81,7 +81,9 @@ enum {
 
 /* Solver options, opts[8]:
  *   [0] preconditioner: 0 none (the reference's MKL CG, solver.h:311), 1 Jacobi,
- *       2 aggregation multigrid (one V-cycle; built on first use per handle)
+ *       2 aggregation multigrid (one V-cycle; built on first use per handle);
+ *       3 Jacobi-preconditioned BiCGSTAB instead of CG (no reference
+ *       counterpart; same stopping test, -4 with a message on a breakdown)
  *   [1] rtol, [2] atol: stop when ||r||^2 <= rtol*||r0||^2 + atol
  *       (the reference's dpar[0], dpar[1], solver.h:312-313)
  *   [3] maximum CG iterations (<= 0: the problem size, as ipar[4], solver.h:308)

@@ -821,6 +821,8 @@ class Solver {
   }
   // no reference counterpart: multigrid-preconditioned CG (large 2D systems)
   void setMultigrid(bool on) { opts[0] = on ? 2 : 1; }
+  // no reference counterpart: Jacobi-preconditioned BiCGSTAB inner solves
+  void setBiCGSTAB(bool on) { opts[0] = on ? 3 : 1; }
 
   // Solver::updateDisplacementBC / updateForceBC (solver.h:128-188)
   void updateDisplacementBC(LoadStep<nlayer>& load_step) {

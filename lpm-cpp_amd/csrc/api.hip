@@ -244,6 +244,8 @@ int free_context(lpmg_context* h) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (hipGraphExec_t& g : c->cg_graph)
     if (g) (void)hipGraphExecDestroy(g);
+  for (hipGraphExec_t& g : c->bicg_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   amg_free(c);
   for (void* p : c->allocs) (void)hipFreeAsync(p, c->stream);
   if (c->stream) (void)hipStreamSynchronize(c->stream);

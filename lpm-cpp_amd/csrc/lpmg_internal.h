@@ -36,6 +36,7 @@ struct Scalars {
   double rtol, atol;                            // stopping test of the current solve
   double gam[2], alp[2];                        // (r,u) and alpha of iteration i in slot i & 1
   double vsum[2];                               // (r,u), (r,r) of the last vector step, summed
+  double rho[2], om[2];                         // BiCGSTAB: rho and omega of iteration i in slot i & 1
   double rnorm_sq, reac_sq;                     // residual / reaction (updateRR)
   double norm_diag;                             // updateStiffnessDispBC
   int iter, done, maxit, any_flag;
@@ -178,6 +179,10 @@ struct Context {
   // and launch shapes captured into it (checked at every launch)
   hipGraphExec_t cg_graph[2] = {nullptr, nullptr};
   std::vector<uintptr_t> cg_graph_sig[2];
+  // BiCGSTAB (bicgstab.hip): captured chunk and its own vectors per precision
+  hipGraphExec_t bicg_graph[2] = {nullptr, nullptr};
+  std::vector<uintptr_t> bicg_graph_sig[2];
+  void* bicg_vec[2] = {nullptr, nullptr};  // r~, z, t (3 * dim * N of the precision)
   int sc_host_iter_launched = 0;
   // CG executor (serving runtime); null: the context's own stream
   CgExec cg_exec = nullptr;
@@ -270,7 +275,8 @@ int launch_export_csr(Context* c, int64_t* dKp, int64_t* dIK, int64_t* dJK, doub
 
 // ---- solver.hip
 struct SolveOpts {
-  int precond = 1;      // 0 plain CG (the reference's MKL call), 1 Jacobi PCG, 2 multigrid PCG
+  int precond = 1;      // 0 plain CG (the reference's MKL call), 1 Jacobi PCG, 2 multigrid PCG,
+                        // 3 Jacobi BiCGSTAB
   int precision = 0;    // 0 FP64, 1 FP32 inner solve
   double rtol = 1e-12, atol = 1e-12;  // ||r||^2 <= rtol*||r0||^2 + atol
   int64_t maxit = 0;    // 0 -> problem size
@@ -278,6 +284,9 @@ struct SolveOpts {
   double predict = 0;   // > 0: lpmg_solve_step starts from this multiple of the last step's increment
 };
 int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final);
+// ---- bicgstab.hip
+template <int D, typename T>
+int run_bicgstab(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 // ---- amg.hip
 template <int D, typename T>
 int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final);

@@ -652,11 +652,19 @@ int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
     return -1;
   }
   int rc;
-  if (o.precond == 2) {  // multigrid PCG, on the context's own stream
-    if (o.precision == 0)
+  if (o.precond == 2 || o.precond == 3) {  // multigrid PCG or BiCGSTAB, on the context's own stream
+    if (o.precond == 3) {
+      if (o.precision == 0)
+        rc = c->dim == 3 ? run_bicgstab<3, double>(c, o, iters, rr_final)
+                         : run_bicgstab<2, double>(c, o, iters, rr_final);
+      else
+        rc = c->dim == 3 ? run_bicgstab<3, float>(c, o, iters, rr_final)
+                         : run_bicgstab<2, float>(c, o, iters, rr_final);
+    } else if (o.precision == 0) {
       rc = c->dim == 3 ? run_pcg_amg<3, double>(c, o, iters, rr_final) : run_pcg_amg<2, double>(c, o, iters, rr_final);
-    else
+    } else {
       rc = c->dim == 3 ? run_pcg_amg<3, float>(c, o, iters, rr_final) : run_pcg_amg<2, float>(c, o, iters, rr_final);
+    }
     if (rc) return rc;
     if (o.precision != 0) {
       f32_to_f64_kernel<<<grid_for(n, 1024), 256, 0, c->stream>>>(c->f_x, c->du, n);
