@@ -21,6 +21,10 @@ Beside it, in the same JSON line:
                             CG, the reference's load-step start) and through
                             the served path: speed-up, largest displacement
                             error against the FP64 result, identical damage flags
+  serving.kernels           per-iteration times of the served solvers (FP64/FP32
+                            Jacobi and multigrid CG, the build's own events over
+                            fixed-length solves) and back-to-back SpMVs on the
+                            stream's largest request and one near 20k particles
   polycrystal               the 1.0M-particle BCC polycrystal (80^3 cells, 200
                             grains, cubic C11/C12/C44): FP64 vs mixed, error,
                             and the SpMV on a matrix no cache holds
@@ -167,6 +171,7 @@ def serving_bench(K, W, P, plain=True):
     rs, res_s, served = replay(probs[:K], arrivals[:K], SERVED_OPTS, SERVE_SLOTS, SERVE_FUSE_ROWS,
                                f"served mixed, first {K}")
     out = dict(served_mixed=served, particles=[len(p.xyz) for p in probs[:K]])
+    out["kernels"] = serving_kernels(probs[:K])
     if plain and P > 0:
         r64, _, plain_st = replay(probs[:P], arrivals[:P], PLAIN_OPTS, 1, False, f"plain FP64, first {P}")
         if P == K:
@@ -180,6 +185,43 @@ def serving_bench(K, W, P, plain=True):
             speedup_vs_plain_fp64=servedP["value"] / plain_st["value"], max_disp_rel_err_vs_fp64=max(errs),
             disp_rel_err_vs_fp64=errs, requests_with_identical_damage_flags=same)
     return out, probs
+
+
+def serving_kernels(probs):
+    """Per-iteration times of the served solvers, from the build's own events
+    (lpmg_time_kernel kinds 5-8: fixed-length solves whose captured chunks are
+    replayed as a served solve replays them), and back-to-back SpMVs, on the
+    stream's largest request and on the request nearest 20k particles (a fused
+    size), loaded by their first load step and assembled at it."""
+    import lpmg
+    from lpmg import problems as P
+    sizes = [len(p.xyz) for p in probs]
+    picks = sorted({int(np.argmax(sizes)), int(np.argmin([abs(n - 20000) for n in sizes]))})
+    out = []
+    for k in picks:
+        prob = probs[k]
+        s, gids = P.build(lpmg.System, prob)
+        s.forward()
+        P.apply_step(s, gids, prob.steps[0])
+        s.forward()
+        s.assemble(precision=1)
+        s.update_rr()
+        rec = dict(request=k, particles=s.N, bsr_blocks=s.nnzb)
+        for name, which in (("spmv_fp64", 0), ("spmv_fp32", 1)):
+            s.time_kernel(which, reps=5)
+            ms, b = s.time_kernel(which, reps=40)
+            rec[name] = dict(us=ms * 1e3, GBps=b / (ms * 1e-3) / 1e9)
+        for name, which, reps in (("jacobi_cg_fp64", 5, 320), ("jacobi_cg_fp32", 6, 320),
+                                  ("multigrid_cg_fp64", 7, 64), ("multigrid_cg_fp32", 8, 64)):
+            s.time_kernel(which, reps=16)  # builds the multigrid hierarchy on first use
+            ms, b = s.time_kernel(which, reps=reps)
+            rec[name] = dict(us_per_iteration=ms * 1e3)
+            if b > 0:
+                rec[name]["GBps"] = b / (ms * 1e-3) / 1e9
+        s.close()
+        log(f"served solver kernels, request {k}: {rec}")
+        out.append(rec)
+    return out
 
 
 # ---------------------------------------------------------------- polycrystal (1.0M particles)

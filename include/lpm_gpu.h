@@ -241,7 +241,15 @@ int lpmg_profile_read(lpmg_handle h, double* out);
  * HIP events (no reference counterpart): which 0 FP64 SpMV, 1 FP32 SpMV,
  * 2 assembly (FP64), 3 forward pass, 4 a plain copy of the FP64 stiffness
  * values (byte-counter calibration).  out[0] = average ms per launch,
- * out[1] = algorithmic bytes per launch. */
+ * out[1] = algorithmic bytes per launch.
+ * which 5-8 time one linear-solver iteration: a solve of the current
+ * residual (lpmg_update_rr) stopped after exactly `reps` iterations, its
+ * captured chunks replayed as in any solve: 5 FP64 Jacobi CG, 6 FP32 Jacobi
+ * CG, 7 FP64 multigrid CG, 8 FP32 multigrid CG (the stiffness of that
+ * precision assembled).  out[0] = ms per iteration, out[1] = algorithmic
+ * bytes of a Jacobi-CG iteration (the SpMV's and 11 vectors of the vector
+ * step), 0 for multigrid.  The solver's vectors are overwritten; positions
+ * are not moved. */
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out);
 
 /* Unknowns per level of the handle's multigrid hierarchy (finest first), built

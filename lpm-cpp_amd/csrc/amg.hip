@@ -32,6 +32,13 @@
 // coarse correction would otherwise move them: the preconditioner then acts
 // on the free DOFs only, where it stays symmetric positive definite).
 //
+// Per FP32 iteration on the 205,645-particle CT specimen this takes 153 us
+// (the fine-level SpMV alone 21.5 us): the levels below about 8,000 unknowns
+// are a few microseconds of work each but three launches.  Running them all
+// in one launch of one workgroup (one thread per row, barriers between the
+// phases) was measured at 211 us: one CU cannot hide the row loops'
+// dependent loads that a launch over the whole GPU covers in one round.
+//
 // PCG (standard form, two reductions per iteration): w = A p with partials of
 // (p,w); x += alpha p, r -= alpha w with partials of (r,r); z = M r (V-cycle)
 // with partials of (r,z); stop test, p = z + beta p.  As in the Jacobi CG, no

@@ -920,9 +920,45 @@ int lpmg_profile_read(lpmg_handle h, double* out) {
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (reps <= 0 || which < 0 || which > 4) { set_error("bad kernel id or repetitions"); return -1; }
-  if ((which == 0 || which == 2) && !c->assembled64) { set_error("stiffness not assembled"); return -1; }
-  if (which == 1 && !c->assembled32) { set_error("FP32 stiffness not assembled"); return -1; }
+  if (reps <= 0 || which < 0 || which > 8) { set_error("bad kernel id or repetitions"); return -1; }
+  if ((which == 0 || which == 2 || which == 5 || which == 7) && !c->assembled64) {
+    set_error("stiffness not assembled");
+    return -1;
+  }
+  if ((which == 1 || which == 6 || which == 8) && !c->assembled32) {
+    set_error("FP32 stiffness not assembled");
+    return -1;
+  }
+  if (which >= 5) {  // solver iterations: a fixed-length solve between events
+    SolveOpts so;
+    so.precond = which <= 6 ? 1 : 2;
+    so.precision = (which == 6 || which == 8) ? 1 : 0;
+    so.rtol = 0;
+    so.atol = 0;
+    so.maxit = reps;
+    hipEvent_t s0, s1;
+    API_HIP(hipEventCreate(&s0));
+    API_HIP(hipEventCreate(&s1));
+    int it = 0;
+    double rr = 0;
+    API_HIP(hipEventRecord(s0, c->stream));
+    int rc = solve_linear(c, so, &it, &rr);
+    API_HIP(hipEventRecord(s1, c->stream));
+    API_HIP(hipEventSynchronize(s1));
+    float ms = 0;
+    API_HIP(hipEventElapsedTime(&ms, s0, s1));
+    (void)hipEventDestroy(s0);
+    (void)hipEventDestroy(s1);
+    if (rc) return rc;
+    if (it <= 0) {
+      set_error("the solve stopped at once: the residual is zero (load the system and call lpmg_update_rr)");
+      return -1;
+    }
+    const double w = so.precision ? 4.0 : 8.0;
+    out[0] = ms / it;
+    out[1] = so.precond == 1 ? spmv_bytes(c, so.precision) + 11.0 * w * c->dim * (double)c->N : 0.0;
+    return 0;
+  }
   const int64_t ncopy = (int64_t)c->nnzb * c->dim * c->dim;
   double* copy_dst = nullptr;
   if (which == 4) API_HIP(hipMallocAsync((void**)&copy_dst, sizeof(double) * ncopy, c->stream));

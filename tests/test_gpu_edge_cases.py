@@ -205,3 +205,30 @@ def test_empty_group_and_zero_step():
     (bg, ag, stg), (bo, ao, sto) = res
     np.testing.assert_array_equal(ag, bg)   # nothing moves on the GPU
     assert _relerr(ag - prob.xyz.ravel(), ao - prob.xyz.ravel()) < 1e-8
+
+
+def test_solver_iteration_timing_runs_exactly_the_asked_iterations():
+    # lpmg_time_kernel kinds 5-8 time fixed-length solves of the current
+    # residual; with a zero residual there is nothing to time (an error), and
+    # the solve leaves the positions where they were
+    prob = P.hex_plate(nx=40, ny=30, steps=1)
+    s, gids = P.build(lpmg.System, prob)
+    s.forward()
+    s.assemble(precision=1)
+    s.update_rr()
+    with pytest.raises(lpmg.LpmgError, match="residual is zero"):
+        s.time_kernel(5, reps=8)
+    P.apply_step(s, gids, prob.steps[0])
+    s.forward()
+    s.update_rr()
+    x0 = s.get("XYZ").copy()
+    for which in (5, 6, 7, 8):
+        s.time_kernel(which, reps=16)  # the multigrid hierarchy is built on its first use
+        ms, nbytes = s.time_kernel(which, reps=64)
+        assert 0 < ms < 50, (which, ms)
+        if which <= 6:
+            w = 8 if which == 5 else 4
+            assert nbytes > 11 * w * s.dim * s.N
+        else:
+            assert nbytes == 0
+    np.testing.assert_array_equal(s.get("XYZ"), x0)
