@@ -117,7 +117,11 @@ __device__ __forceinline__ void bsr_row_acc(int lo, int hi, int sl, const int* _
 // workgroups one contiguous range of rows instead (so each L2 gathers only its
 // rows' part of x) measured 4-7% faster on the 250k CT plate and FP32 SC 40^3
 // but 2-8% slower on the 1.0M polycrystal and 1.6 us slower in the SC 40^3
-// FP64 CG SpMV, so it is not used.
+// FP64 CG SpMV, so it is not used.  Each lane reads its block's b*b values as
+// one contiguous run; interleaving the values of every 16 consecutive blocks
+// (value k of the 16 blocks contiguous, so a row's lanes read 128-byte runs)
+// was measured 6-14% slower in FP64 and 4-22% in FP32 on SC 40^3, the 250k CT
+// plate, the BCC bar and the polycrystal (510 -> 580 us FP64 there).
 template <int D, typename T, int LPR, int U, bool DOT>
 __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
