@@ -6,7 +6,7 @@ seeded Poisson arrivals at 50 requests/s of 2D hexagonal compact-tension
 specimens, particles log-uniform in 5k-250k, a/W uniform in 0.2-0.5, critical
 bond stretch 1e-3, 30 load steps each.  One bench step is one request of that
 stream: `--steps K` requests are replayed at their arrival times through the
-serving runtime (16 worker slots, device-resident state, memory-gated
+serving runtime (8 worker slots, device-resident state, memory-gated
 admission, fused Jacobi-CG launches for requests up to 50k particles, multigrid
 CG for larger 2D ones, FP32 stiffness and inner solves inside the FP64 Newton
 iteration, load-step predictor); `--warmup W` requests of a separate small
@@ -77,7 +77,7 @@ SERVE_SEED = 1
 WARM_SEED = 2
 SERVE_TOL = 1e-9
 SERVE_FUSE_ROWS = 50000
-SERVE_SLOTS = 16
+SERVE_SLOTS = 8
 SERVE_OPENING = 7.5e-6  # pin opening per load step, in units of W
 # Served path: FP32 inner solves at ||r||^2 <= 1e-4 ||r0||^2 and the load-step
 # predictor; precond < 0 lets the server pick fused Jacobi (<= 50k particles)

@@ -121,7 +121,11 @@ __device__ __forceinline__ void bsr_row_acc(int lo, int hi, int sl, const int* _
 // one contiguous run; interleaving the values of every 16 consecutive blocks
 // (value k of the 16 blocks contiguous, so a row's lanes read 128-byte runs)
 // was measured 6-14% slower in FP64 and 4-22% in FP32 on SC 40^3, the 250k CT
-// plate, the BCC bar and the polycrystal (510 -> 580 us FP64 there).
+// plate, the BCC bar and the polycrystal (510 -> 580 us FP64 there).  Reading
+// the blocks below the diagonal as transposes of their mirrors (storing the
+// upper triangle only, half the value bytes, the same bits) measured slower
+// too except on the BCC bar: SC 40^3 42.6 -> 53.1 us, polycrystal 512 -> 558
+// us FP64 -- a row's transposed blocks sit in 16 unrelated places.
 template <int D, typename T, int LPR, int U, bool DOT>
 __device__ __forceinline__ double bsr_rows(int blk, const int* __restrict__ cptr, const int* __restrict__ conns,
                                            const T* __restrict__ val, const T* __restrict__ x,
