@@ -346,6 +346,8 @@ PARTS = dict(mixed_no_predictor=dict(precond=1, rtol=MIXED_RTOL, atol=0.0, preci
              fp64_predictor=dict(precond=1, rtol=1e-12, atol=0.0, precision=0, predict=1.0),
              fp64_same_inner_tol=dict(precond=1, rtol=MIXED_RTOL, atol=0.0, precision=0),
              fp64_predictor_same_inner_tol=dict(precond=1, rtol=MIXED_RTOL, atol=0.0, precision=0, predict=1.0))
+# FP64 Jacobi-preconditioned BiCGSTAB at the drop-in's tolerances
+BICG_OPTS = dict(precond=3, rtol=1e-12, atol=0.0, precision=0)
 # the reference's settings: MKL CG unpreconditioned, dpar[0] = dpar[1] = 1e-12
 # (solver.h:311-313), Newton tol_iter = 1e-5 and 30 iterations (examples)
 REF_OPTS = dict(precond=0, rtol=1e-12, atol=1e-12)
@@ -388,6 +390,20 @@ def sc_fp64_profile(load_steps=2):
         rec[name] = dict(us=ms * 1e3, bytes=b, GBps=b / (ms * 1e-3) / 1e9)
     s.close()
     return rec
+
+
+def sc_bicgstab_profile(load_steps=2):
+    """The SC 40^3 FP64 path with BiCGSTAB inner solves, a few load steps, for
+    profiler runs."""
+    import lpmg
+    from lpmg import problems as P
+    prob = P.sc_cube(n=40, steps=20, total=0.05, E=70e3, nu=0.33)
+    s, gids = P.build(lpmg.System, prob)
+    solve_once(s, gids, prob, BICG_OPTS, steps=load_steps)
+    t, it, nw = solve_once(s, gids, prob, BICG_OPTS, steps=load_steps)
+    s.close()
+    return dict(load_steps=load_steps, seconds=t, bicgstab_iterations=it, newton_iterations=nw,
+                us_per_iteration=t / max(it, 1) * 1e6)
 
 
 def sc_bench(n_solves):
@@ -433,6 +449,7 @@ def sc_bench(n_solves):
                     disp_rel_err_vs_fp64=float(np.linalg.norm(u - u64) / np.linalg.norm(u64)))
 
     parts = {k: part(v) for k, v in PARTS.items()}
+    bicg = part(BICG_OPTS)
     # the reference's own settings (unpreconditioned CG, tol_iter 1e-5) on the
     # first three load steps: the GPU side of the CPU comparison
     t_ref, cg_ref, _ = solve_once(s, gids, prob, REF_OPTS, tol=REF_TOL, steps=3)
@@ -465,6 +482,8 @@ def sc_bench(n_solves):
                       force_rel_err_vs_fp64=float(np.linalg.norm(fm - f64) / np.linalg.norm(f64)),
                       inner_rtol_sq=MIXED_RTOL, predictor=1.0, parts=parts,
                       fp32_share=(len(mt) / sum(mt)) / parts["fp64_predictor_same_inner_tol"]["value"]),
+        "bicgstab": dict(bicg, note="FP64 Jacobi-preconditioned BiCGSTAB (solver option 3) at the CG's "
+                                   "tolerances; cg_iterations counts BiCGSTAB iterations (two SpMVs each)"),
         "reference_settings_first_3_steps": dict(seconds=t_ref, cg_iterations=cg_ref, newton_tol=REF_TOL,
                                                   opts=REF_OPTS),
         # whole assembly: tangent, symmetrisation, ||diag||, elimination, FP32 copy;
@@ -620,7 +639,7 @@ def main():
     ap.add_argument("--no-poly", action="store_true")
     ap.add_argument("--no-sc", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["sc", "poly", "serve", "spmv-configs", "sc-fp64", "poly-fp64"],
+    ap.add_argument("--only", choices=["sc", "poly", "serve", "spmv-configs", "sc-fp64", "poly-fp64", "sc-bicgstab"],
                     help="run one section alone and print its record (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -635,6 +654,9 @@ def main():
         return
     if args.only == "poly-fp64":
         print(json.dumps(polycrystal_fp64_profile()))
+        return
+    if args.only == "sc-bicgstab":
+        print(json.dumps(sc_bicgstab_profile()))
         return
     if args.only == "sc":
         print(json.dumps(sc_bench(args.sc_solves)[0]))
