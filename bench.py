@@ -299,6 +299,8 @@ def polycrystal_bench():
     sp64_ms, sp64_b = s.time_kernel(0, reps=20)
     sp32_ms, sp32_b = s.time_kernel(1, reps=20)
     asm_ms, asm_b = s.time_kernel(2, reps=3)
+    s.time_kernel(9, reps=2)
+    rd_ms, rd_b = s.time_kernel(9, reps=10)
     spmv_ms = prof["spmv_ms"] / max(prof["spmv_launches"], 1)
     rec = dict(
         workload=f"BCC polycrystal 80^3 cells, 200 Voronoi grains (seed 7), {len(pc.xyz)} particles, cubic "
@@ -315,7 +317,11 @@ def polycrystal_bench():
                                fp64_frac_of_hbm_peak=sp64_b / (sp64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                fp32_us=sp32_ms * 1e3, fp32_GBps=sp32_b / (sp32_ms * 1e-3) / 1e9,
                                fp32_frac_of_hbm_peak=sp32_b / (sp32_ms * 1e-3) / 1e9 / HBM_PEAK_GBS),
-        assembly=dict(ms=asm_ms, algorithmic_bytes=asm_b, GBps=asm_b / (asm_ms * 1e-3) / 1e9))
+        assembly=dict(ms=asm_ms, algorithmic_bytes=asm_b, GBps=asm_b / (asm_ms * 1e-3) / 1e9),
+        # a plain streaming read of the FP64 values: what a read-dominated
+        # kernel can attain on this part
+        stream_read=dict(us=rd_ms * 1e3, bytes=rd_b, GBps=rd_b / (rd_ms * 1e-3) / 1e9,
+                         spmv_back_to_back_over_read=(sp64_b / (sp64_ms * 1e-3)) / (rd_b / (rd_ms * 1e-3))))
     chunks = min(4, max(2, -(-int(s.max_nconn) // 16)))
     achieved = prof["spmv_bytes_fp64"] / (spmv_ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

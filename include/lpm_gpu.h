@@ -240,8 +240,9 @@ int lpmg_profile_read(lpmg_handle h, double* out);
 /* Time `reps` back-to-back launches of one kernel on the handle's stream with
  * HIP events (no reference counterpart): which 0 FP64 SpMV, 1 FP32 SpMV,
  * 2 assembly (FP64), 3 forward pass, 4 a plain copy of the FP64 stiffness
- * values (byte-counter calibration).  out[0] = average ms per launch,
- * out[1] = algorithmic bytes per launch.
+ * values (byte-counter calibration), 9 a plain streaming read of the FP64
+ * stiffness values (the attainable read rate).  out[0] = average ms per
+ * launch, out[1] = algorithmic bytes per launch.
  * which 5-8 time one linear-solver iteration: a solve of the current
  * residual (lpmg_update_rr) stopped after exactly `reps` iterations, its
  * captured chunks replayed as in any solve: 5 FP64 Jacobi CG, 6 FP32 Jacobi

@@ -130,6 +130,37 @@ __global__ void __launch_bounds__(256) stream_copy_kernel(const double* __restri
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
     b[t] = a[t];
 }
+// Plain streaming read of known size (n2 16-byte words summed, one partial
+// per workgroup): the attainable read rate that a read-dominated kernel such as
+// the SpMV is measured against.  Each workgroup reads one contiguous chunk of
+// kReadChunk words, four loads in flight per thread -- the fastest of a sweep
+// of read kernels on a 3 GiB buffer (grid-stride loops with 2,048-32,768
+// workgroups and 4 or 8 loads in flight: 5.5-6.2 TB/s; chunks of 4,096-65,536
+// words: 6.26-6.37 TB/s, 79.6% of the 8 TB/s peak).
+constexpr int kReadChunk = 4096;
+__global__ void __launch_bounds__(256) stream_read_kernel(const double2* __restrict__ a, int64_t n2,
+                                                          double* __restrict__ part) {
+  const int64_t lo = (int64_t)blockIdx.x * kReadChunk;
+  const int64_t hi = lo + kReadChunk < n2 ? lo + kReadChunk : n2;
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int64_t t = lo + threadIdx.x;
+  for (; t + 3 * 256 < hi; t += 4 * 256) {
+    const double2 v0 = a[t], v1 = a[t + 256], v2 = a[t + 512], v3 = a[t + 768];
+    s0 += v0.x + v0.y;
+    s1 += v1.x + v1.y;
+    s2 += v2.x + v2.y;
+    s3 += v3.x + v3.y;
+  }
+  for (; t < hi; t += 256) s0 += a[t].x + a[t].y;
+  __shared__ double red[256];
+  red[threadIdx.x] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
 __global__ void i2d_kernel(const int* __restrict__ a, double* __restrict__ b, int64_t n) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) b[t] = (double)a[t];
@@ -920,8 +951,8 @@ int lpmg_profile_read(lpmg_handle h, double* out) {
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (reps <= 0 || which < 0 || which > 8) { set_error("bad kernel id or repetitions"); return -1; }
-  if ((which == 0 || which == 2 || which == 5 || which == 7) && !c->assembled64) {
+  if (reps <= 0 || which < 0 || which > 9) { set_error("bad kernel id or repetitions"); return -1; }
+  if ((which == 0 || which == 2 || which == 5 || which == 7 || which == 9) && !c->assembled64) {
     set_error("stiffness not assembled");
     return -1;
   }
@@ -929,7 +960,7 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
     set_error("FP32 stiffness not assembled");
     return -1;
   }
-  if (which >= 5) {  // solver iterations: a fixed-length solve between events
+  if (which >= 5 && which <= 8) {  // solver iterations: a fixed-length solve between events
     SolveOpts so;
     so.precond = which <= 6 ? 1 : 2;
     so.precision = (which == 6 || which == 8) ? 1 : 0;
@@ -962,6 +993,10 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   const int64_t ncopy = (int64_t)c->nnzb * c->dim * c->dim;
   double* copy_dst = nullptr;
   if (which == 4) API_HIP(hipMallocAsync((void**)&copy_dst, sizeof(double) * ncopy, c->stream));
+  double* read_part = nullptr;  // one partial per workgroup of the streaming read
+  if (which == 9)
+    API_HIP(hipMallocAsync((void**)&read_part, sizeof(double) * (size_t)((ncopy / 2 + kReadChunk - 1) / kReadChunk + 1),
+                           c->stream));
   hipEvent_t e0, e1;
   API_HIP(hipEventCreate(&e0));
   API_HIP(hipEventCreate(&e1));
@@ -972,6 +1007,9 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
     else if (which == 1) rc = launch_spmv(c, 1, c->f_p, c->f_q);
     else if (which == 2) rc = launch_assemble(c, c->assembled32 ? 1 : 0);
     else if (which == 3) rc = launch_forward(c);
+    else if (which == 9)
+      stream_read_kernel<<<(unsigned)((ncopy / 2 + kReadChunk - 1) / kReadChunk), 256, 0, c->stream>>>(
+          (const double2*)c->Kval, ncopy / 2, read_part);
     else stream_copy_kernel<<<2048, 256, 0, c->stream>>>(c->Kval, copy_dst, ncopy);
   }
   API_HIP(hipEventRecord(e1, c->stream));
@@ -981,9 +1019,11 @@ int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (copy_dst) (void)hipFreeAsync(copy_dst, c->stream);
+  if (read_part) (void)hipFreeAsync(read_part, c->stream);
   out[0] = ms / reps;
   out[1] = which == 0 ? spmv_bytes(c, 0) : which == 1 ? spmv_bytes(c, 1)
-           : which == 2 ? assembly_bytes(c) : which == 3 ? forward_bytes(c) : 16.0 * (double)ncopy;
+           : which == 2 ? assembly_bytes(c) : which == 3 ? forward_bytes(c)
+           : which == 9 ? 8.0 * (double)(ncopy / 2 * 2) : 16.0 * (double)ncopy;
   return rc;
 }
 
