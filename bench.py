@@ -606,7 +606,8 @@ def cpu_baseline(stream_probs, sc_prob, sc_rec):
 
 def spmv_configs():
     """Back-to-back SpMV (50 launches each, FP64 then FP32) on the largest
-    system of each configuration, assembled at rest."""
+    system of each configuration, assembled at rest, and a plain streaming read
+    of its FP64 values (the attainable read rate for the same bytes)."""
     import lpmg
     from lpmg import problems as P
     out = []
@@ -619,11 +620,12 @@ def spmv_configs():
         s.forward()
         s.assemble(precision=1)
         rec = dict(workload=name, particles=s.N, bsr_blocks=s.nnzb)
-        for prec, which in (("fp64", 0), ("fp32", 1)):
+        for prec, which in (("fp64", 0), ("fp32", 1), ("read_fp64_values", 9)):
             s.time_kernel(which, reps=5)
             ms, nbytes = s.time_kernel(which, reps=50)
             rec[prec] = dict(us=ms * 1e3, algorithmic_bytes=nbytes, GBps=nbytes / (ms * 1e-3) / 1e9,
                              frac_of_hbm_peak=nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        rec["fp64_spmv_over_read"] = rec["fp64"]["GBps"] / rec["read_fp64_values"]["GBps"]
         log(f"{name}: {rec}")
         out.append(rec)
         s.close()
