@@ -7,7 +7,7 @@ specimens, particles log-uniform in 5k-250k, a/W uniform in 0.2-0.5, critical
 bond stretch 1e-3, 30 load steps each.  One bench step is one request of that
 stream: `--steps K` requests are replayed at their arrival times through the
 serving runtime (8 worker slots, device-resident state, memory-gated
-admission, fused Jacobi-CG launches for requests up to 50k particles, multigrid
+admission, fused Jacobi-CG launches for requests up to 10k particles, multigrid
 CG for larger 2D ones, FP32 stiffness and inner solves inside the FP64 Newton
 iteration, load-step predictor); `--warmup W` requests of a separate small
 warm-up stream run first.  value = K / makespan; p50/p99 are per-request
@@ -76,11 +76,11 @@ SERVE_RATE = 50.0
 SERVE_SEED = 1
 WARM_SEED = 2
 SERVE_TOL = 1e-9
-SERVE_FUSE_ROWS = 50000
+SERVE_FUSE_ROWS = 10000
 SERVE_SLOTS = 8
 SERVE_OPENING = 7.5e-6  # pin opening per load step, in units of W
 # Served path: FP32 inner solves at ||r||^2 <= 1e-4 ||r0||^2 and the load-step
-# predictor; precond < 0 lets the server pick fused Jacobi (<= 50k particles)
+# predictor; precond < 0 lets the server pick fused Jacobi (<= 10k particles)
 # or multigrid (larger 2D requests).  At 3e-4 the served displacements moved
 # 2.0e-8 from the plain FP64 ones (the Newton tolerance's floor), at 1e-4 1.6e-9.
 SERVED_OPTS = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
