@@ -189,7 +189,7 @@ def serving_bench(K, W, P, plain=True):
 
 def serving_kernels(probs):
     """Per-iteration times of the served solvers, from the build's own events
-    (lpmg_time_kernel kinds 5-8: fixed-length solves whose captured chunks are
+    (lpmg_time_kernel kinds 5-8, 10, 11: fixed-length solves whose captured chunks are
     replayed as a served solve replays them), and back-to-back SpMVs, on the
     stream's largest request and on the request nearest 20k particles (a fused
     size), loaded by their first load step and assembled at it."""
@@ -212,7 +212,8 @@ def serving_kernels(probs):
             ms, b = s.time_kernel(which, reps=40)
             rec[name] = dict(us=ms * 1e3, GBps=b / (ms * 1e-3) / 1e9)
         for name, which, reps in (("jacobi_cg_fp64", 5, 320), ("jacobi_cg_fp32", 6, 320),
-                                  ("multigrid_cg_fp64", 7, 64), ("multigrid_cg_fp32", 8, 64)):
+                                  ("multigrid_cg_fp64", 7, 64), ("multigrid_cg_fp32", 8, 64),
+                                  ("sa_multigrid_cg_fp64", 10, 64), ("sa_multigrid_cg_fp32", 11, 64)):
             s.time_kernel(which, reps=16)  # builds the multigrid hierarchy on first use
             ms, b = s.time_kernel(which, reps=reps)
             rec[name] = dict(us_per_iteration=ms * 1e3)

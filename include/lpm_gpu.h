@@ -83,7 +83,10 @@ enum {
  *   [0] preconditioner: 0 none (the reference's MKL CG, solver.h:311), 1 Jacobi,
  *       2 aggregation multigrid (one V-cycle; built on first use per handle);
  *       3 Jacobi-preconditioned BiCGSTAB instead of CG (no reference
- *       counterpart; same stopping test, -4 with a message on a breakdown)
+ *       counterpart; same stopping test, -4 with a message on a breakdown);
+ *       4 smoothed-aggregation multigrid (as 2, with the prolongation
+ *       smoothed by one damped-Jacobi step and Galerkin coarse operators;
+ *       fewer iterations, more setup and memory; no reference counterpart)
  *   [1] rtol, [2] atol: stop when ||r||^2 <= rtol*||r0||^2 + atol
  *       (the reference's dpar[0], dpar[1], solver.h:312-313)
  *   [3] maximum CG iterations (<= 0: the problem size, as ipar[4], solver.h:308)
@@ -243,18 +246,19 @@ int lpmg_profile_read(lpmg_handle h, double* out);
  * values (byte-counter calibration), 9 a plain streaming read of the FP64
  * stiffness values (the attainable read rate).  out[0] = average ms per
  * launch, out[1] = algorithmic bytes per launch.
- * which 5-8 time one linear-solver iteration: a solve of the current
- * residual (lpmg_update_rr) stopped after exactly `reps` iterations, its
- * captured chunks replayed as in any solve: 5 FP64 Jacobi CG, 6 FP32 Jacobi
- * CG, 7 FP64 multigrid CG, 8 FP32 multigrid CG (the stiffness of that
- * precision assembled).  out[0] = ms per iteration, out[1] = algorithmic
+ * which 5-8, 10 and 11 time one linear-solver iteration: a solve of the
+ * current residual (lpmg_update_rr) stopped after exactly `reps` iterations,
+ * its captured chunks replayed as in any solve: 5 FP64 Jacobi CG, 6 FP32
+ * Jacobi CG, 7 FP64 multigrid CG, 8 FP32 multigrid CG, 10 FP64 and 11 FP32
+ * smoothed-aggregation multigrid CG (the stiffness of that precision
+ * assembled).  out[0] = ms per iteration, out[1] = algorithmic
  * bytes of a Jacobi-CG iteration (the SpMV's and 11 vectors of the vector
  * step), 0 for multigrid.  The solver's vectors are overwritten; positions
  * are not moved. */
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out);
 
 /* Unknowns per level of the handle's multigrid hierarchy (finest first), built
- * on its first multigrid solve (opts[0] = 2); returns the number of levels,
+ * on its first multigrid solve (opts[0] = 2 or 4); returns the number of levels,
  * 0 before.  No reference counterpart (the reference solves with MKL CG or
  * PARDISO). */
 int lpmg_multigrid_levels(lpmg_handle h, int32_t* out, int cap);
@@ -331,8 +335,9 @@ typedef struct lpmg_request {
   double tol;                  /* Newton tolerance */
   double opts[8];              /* solver options as lpmg_solve_linear; opts[0] < 0:
                                   the server picks fused Jacobi for requests it
-                                  fuses, multigrid for larger 2D requests and
-                                  Jacobi for larger 3D ones */
+                                  fuses, smoothed-aggregation multigrid (4) for
+                                  larger 2D requests and Jacobi for larger 3D
+                                  ones */
   int max_halvings;            /* step halvings allowed before the request fails */
   int n_out;
   const lpmg_out_field* out;

@@ -819,8 +819,9 @@ class Solver {
     stiffness.precision = on ? 1 : 0;
     if (on) opts[1] = inner_rtol_sq;
   }
-  // no reference counterpart: multigrid-preconditioned CG (large 2D systems)
-  void setMultigrid(bool on) { opts[0] = on ? 2 : 1; }
+  // no reference counterpart: multigrid-preconditioned CG (large 2D systems);
+  // smoothed: the smoothed-aggregation hierarchy (fewer iterations)
+  void setMultigrid(bool on, bool smoothed = false) { opts[0] = on ? (smoothed ? 4 : 2) : 1; }
   // no reference counterpart: Jacobi-preconditioned BiCGSTAB inner solves
   void setBiCGSTAB(bool on) { opts[0] = on ? 3 : 1; }
 

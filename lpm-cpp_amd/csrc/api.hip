@@ -951,19 +951,19 @@ int lpmg_profile_read(lpmg_handle h, double* out) {
 int lpmg_time_kernel(lpmg_handle h, int which, int reps, double* out) {
   CHECK_HANDLE(h);
   Context* c = &h->c;
-  if (reps <= 0 || which < 0 || which > 9) { set_error("bad kernel id or repetitions"); return -1; }
-  if ((which == 0 || which == 2 || which == 5 || which == 7 || which == 9) && !c->assembled64) {
+  if (reps <= 0 || which < 0 || which > 11) { set_error("bad kernel id or repetitions"); return -1; }
+  if ((which == 0 || which == 2 || which == 5 || which == 7 || which == 9 || which == 10) && !c->assembled64) {
     set_error("stiffness not assembled");
     return -1;
   }
-  if ((which == 1 || which == 6 || which == 8) && !c->assembled32) {
+  if ((which == 1 || which == 6 || which == 8 || which == 11) && !c->assembled32) {
     set_error("FP32 stiffness not assembled");
     return -1;
   }
-  if (which >= 5 && which <= 8) {  // solver iterations: a fixed-length solve between events
+  if ((which >= 5 && which <= 8) || which >= 10) {  // solver iterations: a fixed-length solve between events
     SolveOpts so;
-    so.precond = which <= 6 ? 1 : 2;
-    so.precision = (which == 6 || which == 8) ? 1 : 0;
+    so.precond = which <= 6 ? 1 : which <= 8 ? 2 : 4;
+    so.precision = (which == 6 || which == 8 || which == 11) ? 1 : 0;
     so.rtol = 0;
     so.atol = 0;
     so.maxit = reps;

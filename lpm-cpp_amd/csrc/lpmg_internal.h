@@ -276,7 +276,7 @@ int launch_export_csr(Context* c, int64_t* dKp, int64_t* dIK, int64_t* dJK, doub
 // ---- solver.hip
 struct SolveOpts {
   int precond = 1;      // 0 plain CG (the reference's MKL call), 1 Jacobi PCG, 2 multigrid PCG,
-                        // 3 Jacobi BiCGSTAB
+                        // 3 Jacobi BiCGSTAB, 4 smoothed-aggregation multigrid PCG
   int precision = 0;    // 0 FP64, 1 FP32 inner solve
   double rtol = 1e-12, atol = 1e-12;  // ||r||^2 <= rtol*||r0||^2 + atol
   int64_t maxit = 0;    // 0 -> problem size

@@ -446,6 +446,17 @@ int64_t estimate_bytes(const lpmg_request* r) {
   b += nnzb * (12.0 + DD * 16.0 + (mixed ? DD * 4.0 : 0.0));   // conns, BSR values (tangent + symmetrised)
   b += N * 48.0;                                                // setup scratch
   for (int g = 0; g < r->n_groups; ++g) b += 4.0 * std::max(1, (int)r->group_size[g]) + 256;
+  if (r->opts[0] == 4 || (r->opts[0] < 0 && dim == 2)) {
+    // smoothed-aggregation multigrid: per fine row about Pr prolongator
+    // blocks (values, FP64 and FP32, in row and column order, and five index
+    // arrays), 3 Pr blocks of A P (FP64 values), the pair lists of A P (A's
+    // blocks of the row times the P blocks of their columns) and of P^T (A P)
+    // (P's blocks of the row times the row's A P blocks), 8 bytes a pair; the
+    // coarser levels add up to 20%.  Counted on the CT plates: Pr 8.0, A P
+    // 21.3, 244 and 172 pairs per row.
+    const double pr = dim == 2 ? 9.0 : 20.0, yr = 3.0 * pr, k = nnzb / N;
+    b += 1.2 * N * (8.0 * (k * pr + pr * yr) + yr * DD * 8.0 + pr * (DD * 24.0 + 24.0));
+  }
   if (r->nonlocal_ratio > 0 && r->n_params > 0) {
     // nonlocal neighbours within ratio * L (params[0] is nonlocal_L)
     const double R = std::max(r->nonlocal_ratio * r->params[0], 1.01 * cell.cut[1]);
@@ -506,14 +517,16 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
   if (r.nonlocal_ratio > 0 && (rc = lpmg_search_nonlocal(h, r.nonlocal_ratio))) return rc;
   // the fused launches run the Jacobi CG; multigrid solves use the handle's
   // stream.  opts[0] < 0 lets the server choose: fused Jacobi for the systems
-  // it fuses; for the larger ones multigrid in 2D (Jacobi needs thousands of
-  // iterations per solve there; measured 3.5-4x faster at 1e5-2.5e5
-  // particles) and Jacobi in 3D (a V-cycle costs about two SpMVs, more than
-  // the iterations it saves on the 3D lattices measured).
+  // it fuses; for the larger ones smoothed-aggregation multigrid in 2D
+  // (Jacobi needs thousands of iterations per solve there; the tentative
+  // aggregation measured 3.5-4x faster than Jacobi at 1e5-2.5e5 particles, the
+  // smoothed one 27-29% faster again on the CT stream's 205k and 109k requests)
+  // and Jacobi in 3D (a V-cycle costs about two SpMVs, more than the
+  // iterations it saves on the 3D lattices measured).
   const bool fusable = S->batching && (S->fuse_max_rows < 0 || (int64_t)c->N <= S->fuse_max_rows);
   double opts[8];
   std::memcpy(opts, r.opts, sizeof(opts));
-  if (opts[0] < 0) opts[0] = (fusable || c->dim == 3) ? 1 : 2;
+  if (opts[0] < 0) opts[0] = (fusable || c->dim == 3) ? 1 : 4;
   if (fusable && opts[0] <= 1) {
     c->cg_exec = batcher_exec;
     c->cg_exec_arg = &S->batcher;

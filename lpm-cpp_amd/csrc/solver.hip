@@ -652,7 +652,7 @@ int solve_linear(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
     return -1;
   }
   int rc;
-  if (o.precond == 2 || o.precond == 3) {  // multigrid PCG or BiCGSTAB, on the context's own stream
+  if (o.precond >= 2 && o.precond <= 4) {  // multigrid PCG or BiCGSTAB, on the context's own stream
     if (o.precond == 3) {
       if (o.precision == 0)
         rc = c->dim == 3 ? run_bicgstab<3, double>(c, o, iters, rr_final)

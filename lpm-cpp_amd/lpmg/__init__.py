@@ -114,7 +114,8 @@ def _err():
 def solver_opts(precond=1, rtol=1e-12, atol=1e-12, maxit=0, precision=0, chunk=16, predict=0.0):
     """opts[8] of lpmg_solve_linear / lpmg_newton / lpmg_solve_step.  precond:
     0 plain CG (the reference's MKL call), 1 Jacobi CG, 2 multigrid CG, 3 Jacobi
-    BiCGSTAB; precision 1: FP32 stiffness and inner solve in the FP64 Newton."""
+    BiCGSTAB, 4 smoothed-aggregation multigrid CG; precision 1: FP32 stiffness
+    and inner solve in the FP64 Newton."""
     return np.array([precond, rtol, atol, maxit, precision, chunk, predict, 0], dtype=np.float64)
 
 

@@ -208,7 +208,7 @@ def test_empty_group_and_zero_step():
 
 
 def test_solver_iteration_timing_runs_exactly_the_asked_iterations():
-    # lpmg_time_kernel kinds 5-8 time fixed-length solves of the current
+    # lpmg_time_kernel kinds 5-8, 10, 11 time fixed-length solves of the current
     # residual; with a zero residual there is nothing to time (an error), and
     # the solve leaves the positions where they were
     prob = P.hex_plate(nx=40, ny=30, steps=1)
@@ -222,7 +222,7 @@ def test_solver_iteration_timing_runs_exactly_the_asked_iterations():
     s.forward()
     s.update_rr()
     x0 = s.get("XYZ").copy()
-    for which in (5, 6, 7, 8):
+    for which in (5, 6, 7, 8, 10, 11):
         s.time_kernel(which, reps=16)  # the multigrid hierarchy is built on its first use
         ms, nbytes = s.time_kernel(which, reps=64)
         assert 0 < ms < 50, (which, ms)

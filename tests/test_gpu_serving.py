@@ -165,13 +165,14 @@ def test_two_level_reduction_in_fused_launch():
 
 def test_server_picks_solver_by_size():
     """precond < 0: requests up to the fusion threshold run the fused Jacobi CG,
-    larger ones the multigrid CG on their own stream -- each bit-identical to
-    the same solve on its own handle with that preconditioner."""
+    larger 2D ones the smoothed-aggregation multigrid CG on their own stream --
+    each bit-identical to the same solve on its own handle with that
+    preconditioner."""
     small = P.ct_plate(2000, 0.30, steps=12, opening=3e-5)
     large = P.hex_plate(60, 60, steps=3, total=0.003)
     auto = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1)
     ref_small = _direct(small, dict(auto, precond=1))
-    ref_large = _direct(large, dict(auto, precond=2))
+    ref_large = _direct(large, dict(auto, precond=4))
     reqs = [_request(small, auto), _request(large, auto)]
     with Server(max_active=2, batching=3000) as srv:
         res = [srv.wait(t) for t in [srv.submit(q) for q in reqs]]

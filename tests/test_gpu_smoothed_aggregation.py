@@ -1,0 +1,278 @@
+"""Smoothed-aggregation multigrid PCG (solver option precond=4) on the GPU.
+
+No reference counterpart: LPM-CPP solves with MKL's CG or PARDISO
+(solver.h:190-354).  The preconditioner changes the CG's path, not the system
+it solves, so quasi-static steps must meet the Jacobi CG's bar against the CPU
+oracle: displacements and forces within 1e-8 relative L2 at a tight Newton
+tolerance, damage flags exactly equal.  The hierarchy itself is pinned by
+`sa_pcg_np`, a plain numpy/scipy restatement of the method on the oracle's
+matrix: grid-cell aggregates (cell edge 3 lattice spacings, x3.5 per level,
+doubled while a level keeps more than 80% of its nodes), the prolongation
+P = (I - 2/3 D^-1 A) T of the piecewise-constant T, Galerkin coarse operators
+P^T A P, a dense coarsest inverse, one V-cycle of damped Jacobi (2/3) before
+and after the coarse correction, z = 0 on the eliminated DOFs: the level sizes
+must agree exactly and the PCG iteration counts to rounding.  A dropped
+smoothing term, a transposed P or a wrong sign in it changes the count by far
+more than that (the tentative hierarchy takes 2-3x as many iterations on
+CT plates).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import lpmg
+from lpmg import lattice as lat
+from lpmg import problems as P
+from lpmg import solver_opts
+from lpmg.serve import Request, Server
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+R = 0.1
+SA64 = dict(precond=4, rtol=1e-24, atol=0.0, precision=0)
+ORC = dict(precond=0, rtol=1e-24, atol=0.0)
+OMEGA = 2.0 / 3.0
+
+
+def _relerr(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def _aggregate(pos, h):
+    """Nonempty cells of the grid of edge h over pos (n x 3), numbered in
+    (z, y, x) cell order; returns (aggregate of each node, count)."""
+    cc = np.floor((pos - pos.min(axis=0)) / h).astype(np.int64)
+    dims = cc.max(axis=0) + 1
+    key = (cc[:, 2] * dims[1] + cc[:, 1]) * dims[0] + cc[:, 0]
+    _, agg = np.unique(key, return_inverse=True)
+    return agg.ravel(), int(agg.max()) + 1
+
+
+def sa_hierarchy(A, pos, radius, dim, coarsest=128):
+    """Levels [(A_l, P_l, dinv_l)] and the coarsest matrix."""
+    levels = []
+    h = 3.0 * 2.0 * radius
+    while A.shape[0] > coarsest:
+        n = len(pos)
+        agg, nc = _aggregate(pos, h)
+        if nc > 0.8 * n:
+            h *= 2.0
+            continue
+        rows = np.arange(n * dim)
+        T = sp.csr_matrix((np.ones(n * dim), (rows, np.repeat(agg, dim) * dim + np.tile(np.arange(dim), n))),
+                          shape=(n * dim, nc * dim))
+        dinv = 1.0 / A.diagonal()
+        Pm = (T - OMEGA * (sp.diags(dinv) @ (A @ T))).tocsr()
+        levels.append((A, Pm, dinv))
+        A = (Pm.T @ (A @ Pm)).tocsr()
+        cen = np.zeros((nc, 3))
+        np.add.at(cen, agg, pos)
+        pos = cen / np.bincount(agg)[:, None]
+        h *= 3.5
+    return levels, A.toarray()
+
+
+def sa_pcg_np(A, b, mask, pos, radius, dim, rtol, maxit):
+    """PCG with one smoothed-aggregation V-cycle as M; stops when
+    ||r||^2 <= rtol ||r0||^2.  Returns (x, iterations, level sizes)."""
+    levels, Ac = sa_hierarchy(A, pos, radius, dim)
+    inv = np.linalg.inv(Ac)
+
+    def vcycle(l, rhs):
+        if l == len(levels):
+            return inv @ rhs
+        Al, Pm, dinv = levels[l]
+        x1 = OMEGA * dinv * rhs
+        xc = vcycle(l + 1, Pm.T @ (rhs - Al @ x1))
+        x = x1 + Pm @ xc
+        return x + OMEGA * dinv * (rhs - Al @ x)
+
+    def M(r):
+        return vcycle(0, r) * mask
+
+    x = np.zeros_like(b)
+    r = b.copy()
+    thresh = rtol * (r @ r)
+    z = M(r)
+    p = z.copy()
+    rz = r @ z
+    sizes = [lv[0].shape[0] for lv in levels] + [Ac.shape[0]]
+    for it in range(1, maxit + 1):
+        w = A @ p
+        alpha = rz / (p @ w)
+        x += alpha * p
+        r -= alpha * w
+        if r @ r <= thresh:
+            return x, it, sizes
+        z = M(r)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, maxit, sizes
+
+
+def _full_matrix(s):
+    _, IK, JK, K = s.csr()
+    n = s.dim * s.N
+    U = sp.csr_matrix((K, (np.repeat(np.arange(n), np.diff(IK)), JK - 1)), shape=(n, n))
+    return (U + sp.triu(U, 1).T).tocsr()
+
+
+def _loaded_pair(L, xyz):
+    g = lpmg.System(L, R, lat.ELASTIC, xyz)
+    o = Oracle(L, R, lat.ELASTIC, xyz)
+    dim = o.dim
+    z = xyz[:, dim - 1]
+    cons = np.zeros((o.N, 3))
+    cons[z < z.min() + 0.5 * R] = 1
+    for s in (g, o):
+        s.set_material(0, [0, 1, 200e3, 0.3, 1.0])
+        s.set("CONSTRAINT", cons)
+        s.set("PEX", np.tile([0.3, -0.2, 0.5], o.N) * (1 - cons.ravel()))
+        s.forward()
+        s.assemble()
+        s.update_rr()
+    return g, o, cons[:, :dim].ravel()
+
+
+LATTICES = [("hex", lat.HEXAGON2D, lambda: lat.hex_plate(60, 52, R)),
+            ("square", lat.SQUARE2D, lambda: lat.sc_block(40, 36, 1, R)),
+            ("sc", lat.SC3D, lambda: lat.sc_block(14, 12, 11, R)),
+            ("bcc", lat.BCC3D, lambda: lat.bcc_block(12, 12, 13, R))]
+
+
+@pytest.mark.parametrize("name,L,gen", LATTICES, ids=[x[0] for x in LATTICES])
+def test_sa_levels_and_iterations_match_numpy_restatement(name, L, gen):
+    xyz = gen()
+    g, o, fixed = _loaded_pair(L, xyz)
+    A = _full_matrix(o)
+    b = o.get("RESIDUAL")
+    x0 = g.get("XYZ")
+    st = g.solve_linear(precond=4, rtol=1e-16, atol=0.0)
+    x_np, it_np, sizes = sa_pcg_np(A, b, 1.0 - fixed, xyz, R, o.dim, 1e-16, 2000)
+    assert g.multigrid_levels() == sizes
+    assert len(sizes) >= 3
+    assert abs(int(st[0]) - it_np) <= max(2, it_np // 20), (int(st[0]), it_np)
+    du_g = (g.get("XYZ") - x0).reshape(-1, 3)[:, : o.dim].ravel()
+    assert _relerr(du_g, x_np) < 1e-6
+    # the tentative hierarchy on the same system takes more iterations
+    g.set("XYZ", x0)
+    st2 = g.solve_linear(precond=2, rtol=1e-16, atol=0.0)
+    assert st2[0] > st[0], (st2[0], st[0])
+
+
+def _run(prob, backend, opts, tol=1e-11):
+    s, gids = P.build(backend, prob)
+    s.forward()
+    stats = P.solve_problem(s, gids, P.Problem(**{**prob.__dict__, "tol": tol}), opts)
+    return s, stats
+
+
+@pytest.mark.parametrize("prob", [P.sc_cube(n=10, steps=3), P.hex_plate(nx=40, ny=36, steps=3),
+                                  P.bcc_j2_bar(nx=3, ny=3, nz=6, steps=10, strain_step=5e-4)],
+                         ids=["sc_cube", "hex_plate", "bcc_j2_bar"])
+def test_sa_quasi_static_matches_oracle(prob):
+    g, _ = _run(prob, lpmg.System, SA64)
+    o, _ = _run(prob, Oracle, ORC)
+    assert len(g.multigrid_levels()) >= 2
+    x0 = o.get("XYZ_INITIAL")
+    assert _relerr(g.get("XYZ") - x0, o.get("XYZ") - x0) < 1e-8
+    assert _relerr(g.get("PIN"), o.get("PIN")) < 1e-8
+    if prob.ptype == lat.J2PLASTICITY:
+        ag, ao = g.get("STATE_VAR")[0::7], o.get("STATE_VAR")[0::7]
+        np.testing.assert_array_equal(ag > 0, ao > 0)
+        assert (ao > 0).any()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sa_ct_with_bond_breaking_matches_oracle(precision):
+    prob = P.ct_plate(3000, 0.30, steps=12, opening=3e-5)
+    opts = dict(SA64) if precision == 0 else dict(precond=4, rtol=1e-4, atol=0.0, precision=1)
+    g, _ = _run(prob, lpmg.System, opts, tol=1e-10)
+    o, _ = _run(prob, Oracle, ORC, tol=1e-10)
+    x0 = o.get("XYZ_INITIAL")
+    bg, bo = g.get("BDAMAGE") > 0.5, o.get("BDAMAGE") > 0.5
+    assert bo.sum() > 0
+    np.testing.assert_array_equal(bg, bo)
+    assert _relerr(g.get("XYZ") - x0, o.get("XYZ") - x0) < 1e-8
+
+
+def test_sa_fp32_inner_solve_matches_fp64_oracle():
+    prob = P.hex_plate(nx=40, ny=36, steps=3)
+    g, _ = _run(prob, lpmg.System, dict(precond=4, rtol=1e-4, atol=0.0, precision=1))
+    o, _ = _run(prob, Oracle, ORC)
+    x0 = o.get("XYZ_INITIAL")
+    assert _relerr(g.get("XYZ") - x0, o.get("XYZ") - x0) < 1e-8
+
+
+def test_sa_single_level_dense_coarsest():
+    prob = P.hex_plate(nx=8, ny=8, steps=2)  # 128 unknowns: the coarsest level is the system
+    s, gids = P.build(lpmg.System, prob)
+    s.forward()
+    P.apply_step(s, gids, prob.steps[0])
+    s.assemble()
+    s.forward()
+    s.update_rr()
+    x0 = s.get("XYZ")
+    st = s.solve_linear(precond=4, rtol=1e-20, atol=0.0)
+    assert s.multigrid_levels() == [128]
+    assert st[0] <= 2
+    du = s.get("XYZ") - x0
+    s.set("XYZ", x0)
+    s.solve_linear(precond=1, rtol=1e-24, atol=0.0)
+    assert _relerr(du, s.get("XYZ") - x0) < 1e-10
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sa_reruns_are_bit_identical(precision):
+    prob = P.hex_plate(nx=60, ny=50, steps=3)
+    opts = SA64 if precision == 0 else dict(precond=4, rtol=1e-4, atol=0.0, precision=1)
+    a, _ = _run(prob, lpmg.System, opts)
+    b, _ = _run(prob, lpmg.System, opts)
+    np.testing.assert_array_equal(a.get("XYZ"), b.get("XYZ"))
+
+
+def test_switching_between_hierarchies_on_one_handle():
+    # the handle rebuilds its hierarchy when the kind asked for changes
+    prob = P.hex_plate(nx=50, ny=44, steps=1)
+    s, gids = P.build(lpmg.System, prob)
+    s.forward()
+    P.apply_step(s, gids, prob.steps[0])
+    s.assemble()
+    s.forward()
+    s.update_rr()
+    x0 = s.get("XYZ")
+    out = {}
+    for pc in (4, 2, 4):
+        s.set("XYZ", x0)
+        st = s.solve_linear(precond=pc, rtol=1e-20, atol=0.0)
+        out.setdefault(pc, []).append((int(st[0]), s.get("XYZ") - x0))
+    assert out[4][0][0] == out[4][1][0]
+    np.testing.assert_array_equal(out[4][0][1], out[4][1][1])
+    assert out[2][0][0] > out[4][0][0]
+    assert _relerr(out[2][0][1], out[4][0][1]) < 1e-7
+
+
+def test_served_sa_request_within_its_memory_estimate():
+    # explicit option 4 in 3D and the server's own choice for a 2D request
+    # above the fusion limit: each stays within the admission estimate and
+    # gives the same bits as the same request solved on its own handle
+    reqs = [(P.sc_cube(n=16, steps=2), dict(precond=4, rtol=1e-4, atol=0.0, precision=1)),
+            (P.ct_plate(12000, 0.35, steps=4, opening=7.5e-6, crit=1e-3, tol=1e-9),
+             dict(precond=-1, rtol=1e-4, atol=0.0, precision=1, predict=1.0))]
+    for prob, o in reqs:
+        out = []
+        for slots in (1, 3):
+            rq = Request(prob, solver_opts(**o), out_fields=("XYZ",))
+            with Server(max_active=slots, batching=10000) as srv:
+                extra = [srv.submit(Request(P.hex_plate(nx=30, ny=30, steps=2),
+                                            solver_opts(**dict(o, precond=-1)), out_fields=("XYZ",)))
+                         for _ in range(slots - 1)]
+                status, st = srv.wait(srv.submit(rq))
+                for t in extra:
+                    assert srv.wait(t)[0] == 0
+            assert status == 0, st.get("error")
+            assert st["bytes"] <= st["bytes_est"], (st["bytes"], st["bytes_est"])
+            out.append(rq.out["XYZ"].copy())
+        np.testing.assert_array_equal(out[0], out[1])
