@@ -393,93 +393,193 @@ __global__ void __launch_bounds__(1024) amg_invert_kernel(const int* __restrict_
 // the CT stream's 205,645- and 109,174-particle requests (30 load steps, FP32
 // inner solves) take 15.0 and 6.1 s against 20.6 and 8.6 s.
 
+// P's blocks: the sum of the level's blocks each one covers (pair list atidx,
+// contiguous per P block), one wavefront per G consecutive P blocks.  The
+// wavefront walks their lists 64 entries at a time -- lane t loads entry
+// base+t's block into LDS (coalesced index loads, 64 gathers in flight) --
+// and each P block's lane adds its own entries in list order; the block then
+// becomes [aggregate of the row] I - om D_i^-1 (sum), written in row order (Pv)
+// and column order (PvT).
 template <int D>
-__global__ void sa_p_kernel(const int* __restrict__ prow, const int* __restrict__ pcol,
-                            const int* __restrict__ atptr, const int* __restrict__ atidx,
-                            const int* __restrict__ agg, const int* __restrict__ dpos,
-                            const int* __restrict__ qpos, const double* __restrict__ A, int nP, double om,
-                            double* __restrict__ Pv, double* __restrict__ PvT) {
+__global__ void __launch_bounds__(64) sa_p_kernel(const int* __restrict__ atptr, const int* __restrict__ atidx,
+                                                  const double* __restrict__ A, const int* __restrict__ prow,
+                                                  const int* __restrict__ pcol, const int* __restrict__ agg,
+                                                  const int* __restrict__ dpos, const int* __restrict__ qpos,
+                                                  int nP, int G, double om, double* __restrict__ Pv,
+                                                  double* __restrict__ PvT) {
   constexpr int DD = D * D;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= nP) return;
-  const int i = prow[p];
+  __shared__ double term[DD * 64];
+  const int lane = threadIdx.x;
+  const int o0 = blockIdx.x * G;
+  const int o = o0 + lane;
+  const bool mine = lane < G && o < nP;
+  const int k0 = atptr[o0], k1 = atptr[min(o0 + G, nP)];
+  const int lo = mine ? atptr[o] : 0, hi = mine ? atptr[o + 1] : 0;
   double s[DD];
 #pragma unroll
   for (int t = 0; t < DD; ++t) s[t] = 0;
-  for (int k = atptr[p]; k < atptr[p + 1]; ++k) {
-    const size_t e = (size_t)atidx[k] * DD;
+  for (int base = k0; base < k1; base += 64) {
+    const int k = base + lane;
+    if (k < k1) {
+      const double* a = A + (size_t)atidx[k] * DD;
 #pragma unroll
-    for (int t = 0; t < DD; ++t) s[t] += A[e + t];
+      for (int t = 0; t < DD; ++t) term[t * 64 + lane] = a[t];
+    }
+    __syncthreads();
+    const int a0 = max(lo, base), a1 = min(hi, base + 64);
+    for (int j = a0; j < a1; ++j)
+#pragma unroll
+      for (int t = 0; t < DD; ++t) s[t] += term[t * 64 + (j - base)];
+    __syncthreads();
   }
+  if (!mine) return;
+  const int i = prow[o];
   const size_t dg = (size_t)dpos[i] * DD;
-  const bool own = pcol[p] == agg[i];
-  const size_t pt = (size_t)qpos[p] * DD;
+  const bool own = pcol[o] == agg[i];
+  const size_t pt = (size_t)qpos[o] * DD;
 #pragma unroll
   for (int r = 0; r < D; ++r) {
     const double w = om / A[dg + r * D + r];
 #pragma unroll
     for (int q = 0; q < D; ++q) {
       const double v = ((own && r == q) ? 1.0 : 0.0) - w * s[r * D + q];
-      Pv[(size_t)p * DD + r * D + q] = v;
+      Pv[(size_t)o * DD + r * D + q] = v;
       PvT[pt + r * D + q] = v;
     }
   }
 }
 
-// Y(i,b) = sum over the pairs (A block, P block) of A(i,j) P(j,b), in list order
-template <int D>
-__global__ void sa_y_kernel(const int64_t* __restrict__ ypp, const int* __restrict__ ype,
-                            const int* __restrict__ ypq, const double* __restrict__ A,
-                            const double* __restrict__ Pv, int64_t nY, double* __restrict__ Yv) {
-  constexpr int DD = D * D;
-  const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= nY) return;
-  double s[DD];
-#pragma unroll
-  for (int t = 0; t < DD; ++t) s[t] = 0;
-  for (int64_t k = ypp[y]; k < ypp[y + 1]; ++k) {
-    const double* a = A + (size_t)ype[k] * DD;
-    const double* pq = Pv + (size_t)ypq[k] * DD;
-#pragma unroll
-    for (int r = 0; r < D; ++r)
-#pragma unroll
-      for (int q = 0; q < D; ++q) {
-        double t = a[r * D] * pq[q];
-#pragma unroll
-        for (int m = 1; m < D; ++m) t += a[r * D + m] * pq[m * D + q];
-        s[r * D + q] += t;
-      }
-  }
-#pragma unroll
-  for (int t = 0; t < DD; ++t) Yv[(size_t)y * DD + t] = s[t];
+// P blocks per wavefront of sa_p_kernel: about 10 passes of 64 list entries
+// on average, at most 64 blocks.
+inline int sa_outputs_per_wave(int64_t pairs, int64_t outputs) {
+  const double avg = outputs > 0 ? (double)pairs / (double)outputs : 1.0;
+  return (int)std::max(1.0, std::min(64.0, std::floor(640.0 / std::max(avg, 1.0))));
 }
 
-// Coarse block (a,b) = sum over the pairs (P block, Y block) of P(i,a)^T Y(i,b)
-template <int D>
-__global__ void sa_c_kernel(const int64_t* __restrict__ cpp, const int* __restrict__ cpq,
-                            const int* __restrict__ cpy, const double* __restrict__ Pv,
-                            const double* __restrict__ Yv, int64_t nb, double* __restrict__ Ac) {
+// Row products by scattering into LDS: LPR lanes per output row, whose
+// blocks accumulate in LDS (`width` blocks per row).  The row's sources are
+// taken in order; each source contributes to a contiguous run of blocks (a P
+// row for Y = A P, a Y row for P^T Y), one per lane, each lane adding its term
+// to the slot the host listed for it (a source's terms go to distinct slots,
+// and a wavefront's LDS operations complete in order).  Every output block is
+// therefore the sum of its terms in source order -- the sequence a thread
+// summing its own pair list adds -- and each source block is read once per
+// row, its run contiguously.  Loads are batched so that no source waits for
+// the previous one's: the lanes load LPR sources' indices, slot offsets and
+// run bounds at once (broadcast by shuffles), and a batch of SB sources issues
+// all its block and slot loads before its LDS updates.  (Taking the sources
+// one at a time, each behind its three dependent loads, measured 462 + 340 us
+// for Y and P^T Y on the finest level of the 205,645-particle CT specimen.)
+//   MODE 1  Y(i,:) = sum over A's blocks e of row i of A(e) P(conns[e], :)
+//   MODE 2  C(a,:) = sum over P's blocks k of column a of P(k)^T Y(qrow[k], :)
+// ONE: every run fits the row's lanes (the longest run <= LPR).
+template <int D, int MODE, int LPR, bool ONE>
+__global__ void sa_rows_kernel(int nrows, const int* __restrict__ sptr, const int* __restrict__ sidx,
+                               const double* __restrict__ S, const int* __restrict__ rptr,
+                               const double* __restrict__ R, const int64_t* __restrict__ slot0,
+                               const int* __restrict__ soff, const uint16_t* __restrict__ slot,
+                               const int* __restrict__ optr, int width, double* __restrict__ out) {
   constexpr int DD = D * D;
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nb) return;
-  double s[DD];
+  constexpr int SB = D == 2 ? 4 : 2;  // sources per batch
+  extern __shared__ double acc[];
+  const int g = threadIdx.x / LPR, t = threadIdx.x - g * LPR;
+  const int row = blockIdx.x * (blockDim.x / LPR) + g;
+  if (row >= nrows) return;  // whole groups leave together; no barriers below
+  double* my = acc + (size_t)g * width * DD;
+  const int o0 = optr[row], no = optr[row + 1] - o0;
+  for (int u = t; u < no * DD; u += LPR) my[u] = 0;
+  const uint16_t* rs = slot + slot0[row];
+  const int s0 = sptr[row], s1 = sptr[row + 1];
+  for (int kb = s0; kb < s1; kb += LPR) {
+    // lane t: source kb + t's run start, length and first slot
+    const int nk = min(LPR, s1 - kb);
+    int r0l = 0, ml = 0, ol = 0;
+    if (t < nk) {
+      const int j = sidx[kb + t];
+      ol = soff[kb + t];
+      r0l = rptr[j];
+      ml = rptr[j + 1] - r0l;
+    }
+    for (int b0 = 0; b0 < nk; b0 += SB) {
+      double sv[SB][DD], rv[SB][DD];
+      int sl[SB], m[SB];
 #pragma unroll
-  for (int t = 0; t < DD; ++t) s[t] = 0;
-  for (int64_t k = cpp[c]; k < cpp[c + 1]; ++k) {
-    const double* pq = Pv + (size_t)cpq[k] * DD;
-    const double* yv = Yv + (size_t)cpy[k] * DD;
+      for (int b = 0; b < SB; ++b) {
+        // unconditional loads from addresses that are always valid (the
+        // batch's first source, the run's first block): a lane without a term
+        // never uses what it loaded, and no load waits behind a branch
+        const int src = b0 + b;
+        const bool ok = src < nk;
+        const int r0 = __shfl(r0l, src & (LPR - 1), LPR);
+        const int off = __shfl(ol, src & (LPR - 1), LPR);
+        m[b] = ok ? __shfl(ml, src & (LPR - 1), LPR) : 0;
+        const bool act = t < m[b];
+        const double* sp = S + (size_t)(kb + (ok ? src : b0)) * DD;
+        const double* rp = R + (size_t)(act ? r0 + t : 0) * DD;
 #pragma unroll
-    for (int r = 0; r < D; ++r)
+        for (int q = 0; q < DD; ++q) sv[b][q] = sp[q];
+        sl[b] = rs[act ? off + t : 0];
 #pragma unroll
-      for (int q = 0; q < D; ++q) {
-        double t = pq[r] * yv[q];
-#pragma unroll
-        for (int m = 1; m < D; ++m) t += pq[m * D + r] * yv[m * D + q];
-        s[r * D + q] += t;
+        for (int q = 0; q < DD; ++q) rv[b][q] = rp[q];
       }
-  }
 #pragma unroll
-  for (int t = 0; t < DD; ++t) Ac[(size_t)c * DD + t] = s[t];
+      for (int b = 0; b < SB; ++b) {
+        // the head of source b's run (preloaded), then, when it is longer
+        // than the row's lanes, its tail
+        const int r0 = ONE ? 0 : __shfl(r0l, (b0 + b) & (LPR - 1), LPR);
+        const int off = ONE ? 0 : __shfl(ol, (b0 + b) & (LPR - 1), LPR);
+        for (int c = t; c < m[b]; c += LPR) {
+          int sc;
+          double rc[DD];
+          if (ONE || c == t) {
+            sc = sl[b];
+#pragma unroll
+            for (int q = 0; q < DD; ++q) rc[q] = rv[b][q];
+          } else {
+            sc = rs[off + c];
+#pragma unroll
+            for (int q = 0; q < DD; ++q) rc[q] = R[(size_t)(r0 + c) * DD + q];
+          }
+#pragma unroll
+          for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+              // MODE 1: sum_m s(r,m) R(m,q); MODE 2: sum_m s(m,r) R(m,q)
+              double v = (MODE == 1 ? sv[b][r * D] : sv[b][r]) * rc[q];
+#pragma unroll
+              for (int mm = 1; mm < D; ++mm)
+                v += (MODE == 1 ? sv[b][r * D + mm] : sv[b][mm * D + r]) * rc[mm * D + q];
+              my[sc * DD + r * D + q] += v;
+            }
+          if (ONE) break;
+        }
+      }
+    }
+  }
+  for (int u = t; u < no * DD; u += LPR) out[(size_t)o0 * DD + u] = my[u];
+}
+
+// Launch sa_rows_kernel: LPR from the longest run (16, 32 or 64 lanes), rows
+// per workgroup so that their accumulators take at most 32 KB of LDS.
+template <int D, int MODE>
+void launch_sa_rows(hipStream_t st, int nrows, int longest_run, int width, const int* sptr, const int* sidx,
+                    const double* S, const int* rptr, const double* R, const int64_t* slot0, const int* soff,
+                    const uint16_t* slot, const int* optr, double* out) {
+  const size_t row_bytes = (size_t)std::max(width, 1) * D * D * sizeof(double);
+  auto go = [&](auto LP) {
+    constexpr int LPR = decltype(LP)::value;
+    const int rpb = (int)std::max<size_t>(1, std::min<size_t>(256 / LPR, 32768 / row_bytes));
+    const unsigned grid = (unsigned)((nrows + rpb - 1) / rpb);
+    if (longest_run <= LPR)
+      sa_rows_kernel<D, MODE, LPR, true><<<grid, rpb * LPR, rpb * row_bytes, st>>>(nrows, sptr, sidx, S, rptr, R,
+                                                                                  slot0, soff, slot, optr, width, out);
+    else
+      sa_rows_kernel<D, MODE, LPR, false><<<grid, rpb * LPR, rpb * row_bytes, st>>>(nrows, sptr, sidx, S, rptr, R,
+                                                                                   slot0, soff, slot, optr, width, out);
+  };
+  if (longest_run <= 16) go(std::integral_constant<int, 16>());
+  else if (longest_run <= 32) go(std::integral_constant<int, 32>());
+  else go(std::integral_constant<int, 64>());
 }
 
 // bc(a) = sum over the aggregate's P blocks of P(i,a)^T r(i): one wavefront per
@@ -804,12 +904,16 @@ struct AmgLevel {  // level l >= 1; its nodes are the aggregates of level l-1
   // this level's blocks P^T Y
   int nP = 0;
   int64_t nY = 0;
+  int sa_gp = 1;                 // P blocks per wavefront of sa_p_kernel
+  int pmax = 0, ymax = 0, cmax = 0;  // longest P, Y and coarse row
   int *pptr = nullptr, *pcol = nullptr, *prow = nullptr, *atptr = nullptr, *atidx = nullptr;
-  int *qptr = nullptr, *qidx = nullptr, *qrow = nullptr, *qpos = nullptr;
-  int64_t *ypp = nullptr, *cpp = nullptr;
-  int *ype = nullptr, *ypq = nullptr, *cpq = nullptr, *cpy = nullptr;
+  int *qptr = nullptr, *qrow = nullptr, *qpos = nullptr;
+  int* yptr = nullptr;                       // Y's rows
+  int64_t *ysp = nullptr, *csp = nullptr;     // each row's start in the slot lists
+  int *yso = nullptr, *cso = nullptr;         // each source's first slot within its row's list
+  uint16_t *ys = nullptr, *cs = nullptr;      // slot lists of the Y and coarse-block sums
   void* pval[2] = {nullptr, nullptr};   // P per precision, by row
-  void* pvalT[2] = {nullptr, nullptr};  // the same blocks by column (qidx order)
+  void* pvalT[2] = {nullptr, nullptr};  // the same blocks by column (qpos gives each one's place)
   double* yval = nullptr;
 };
 
@@ -837,9 +941,13 @@ int amg_alloc(Context* c, T** p, size_t n) { return dalloc(c, p, n); }
 // each of their blocks sums (in a fixed order: by column, then by the blocks'
 // positions), and the coarse level's pattern.
 struct SaSym {
-  std::vector<int> pptr, pcol, prow, atptr, atidx, qptr, qidx, qrow, qpos, ype, ypq, cpq, cpy;
-  std::vector<int64_t> ypp, cpp;
+  std::vector<int> pptr, pcol, prow, atptr, atidx, qptr, qidx, qrow, qpos;
+  std::vector<int> yptr, ycol;      // pattern of Y = A P
+  std::vector<uint16_t> ys, cs;     // slot lists of the Y and coarse-block sums
+  std::vector<int64_t> ysp, csp;    // each row's start in them
+  std::vector<int> yso, cso;        // each source's first slot within its row's list
   std::vector<int> ccptr, cconns, dpos;
+  int pmax = 0, ymax = 0, cmax = 0;  // longest P, Y and coarse row
 };
 
 // Rows [0, nrows) each emit (col, a, b) triples through gen(row, emit); each
@@ -912,6 +1020,65 @@ void group_rows(int nrows, int ncols, Gen gen, std::vector<int>* rcnt, std::vect
   }
 }
 
+// Rows [0, nrows) each emit columns through gen(row, emit).  Out, in row
+// order: each row's distinct columns ascending (rptr, cols), and for every
+// emission, in emission order, the position of its column among them (slot,
+// rows starting at sptr).  Split over host threads; the result does not
+// depend on the split.  Returns the longest row, or -1 past 65,535 columns.
+template <class Gen>
+int slot_rows(int nrows, int ncols, Gen gen, std::vector<int>* rptr, std::vector<int>* cols,
+              std::vector<uint16_t>* slot, std::vector<int64_t>* sptr) {
+  struct Part {
+    std::vector<int> rcnt, cols;
+    std::vector<uint16_t> slot;
+    std::vector<int64_t> scnt;
+    int longest = 0;
+  };
+  const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+  const int per = (nrows + nt - 1) / nt;
+  std::vector<Part> parts(nt);
+  auto work = [&](int t) {
+    Part& P = parts[t];
+    const int r0 = t * per, r1 = std::min(nrows, r0 + per);
+    std::vector<int> stamp(ncols, -1), pos(ncols, 0), touched, tmp;
+    for (int i = r0; i < r1; ++i) {
+      touched.clear();
+      tmp.clear();
+      gen(i, [&](int col) {
+        if (stamp[col] != i) {
+          stamp[col] = i;
+          touched.push_back(col);
+        }
+        tmp.push_back(col);
+      });
+      std::sort(touched.begin(), touched.end());
+      for (size_t k = 0; k < touched.size(); ++k) pos[touched[k]] = (int)k;
+      P.longest = std::max(P.longest, (int)touched.size());
+      P.rcnt.push_back((int)touched.size());
+      P.cols.insert(P.cols.end(), touched.begin(), touched.end());
+      for (int col : tmp) P.slot.push_back((uint16_t)std::min(pos[col], 65535));
+      P.scnt.push_back((int64_t)tmp.size());
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+  rptr->assign(1, 0);
+  sptr->assign(1, 0);
+  cols->clear();
+  slot->clear();
+  int longest = 0;
+  for (const Part& P : parts) {
+    for (int n : P.rcnt) rptr->push_back(rptr->back() + n);
+    for (int64_t n : P.scnt) sptr->push_back(sptr->back() + n);
+    cols->insert(cols->end(), P.cols.begin(), P.cols.end());
+    slot->insert(slot->end(), P.slot.begin(), P.slot.end());
+    longest = std::max(longest, P.longest);
+  }
+  return longest > 65535 ? -1 : longest;
+}
+
 int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<int>& conns,
                 const std::vector<int>& agg, SaSym* s) {
   std::vector<int> rcnt, unused;
@@ -944,41 +1111,47 @@ int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<i
     s->qrow[k] = s->prow[s->qidx[k]];
     s->qpos[s->qidx[k]] = k;
   }
-  // Y = A P: row i's blocks by column b; pairs (A block e, P block q)
-  std::vector<int> ycol;
-  group_rows(N, Nc, [&](int i, auto emit) {
+  for (int i = 0; i < N; ++i) s->pmax = std::max(s->pmax, s->pptr[i + 1] - s->pptr[i]);
+  // Y = A P: row i's blocks by column b; a slot per (A block e, P block q) in
+  // (e, q) order
+  s->ymax = slot_rows(N, Nc, [&](int i, auto emit) {
     for (int e = cptr[i]; e < cptr[i + 1]; ++e) {
       const int j = conns[e];
-      for (int q = s->pptr[j]; q < s->pptr[j + 1]; ++q) emit(s->pcol[q], e, q);
+      for (int q = s->pptr[j]; q < s->pptr[j + 1]; ++q) emit(s->pcol[q]);
     }
-  }, &rcnt, &ycol, &eend, &s->ype, &s->ypq);
-  std::vector<int> yptr(N + 1, 0);
-  for (int i = 0; i < N; ++i) yptr[i + 1] = yptr[i] + rcnt[i];
-  s->ypp.assign(1, 0);
-  s->ypp.insert(s->ypp.end(), eend.begin(), eend.end());
-  // coarse blocks (a,b) = P^T Y: pairs (P block p of column a, Y block y of row i(p), column b)
-  group_rows(Nc, Nc, [&](int a, auto emit) {
+  }, &s->yptr, &s->ycol, &s->ys, &s->ysp);
+  // coarse blocks (a,b) = P^T Y: a slot per (P block k of column a, Y block y
+  // of row i(k)) in (k, y) order
+  s->cmax = slot_rows(Nc, Nc, [&](int a, auto emit) {
     for (int k = s->qptr[a]; k < s->qptr[a + 1]; ++k) {
-      const int p = s->qidx[k], i = s->prow[p];
-      for (int y = yptr[i]; y < yptr[i + 1]; ++y) emit(ycol[y], p, y);
+      const int i = s->qrow[k];
+      for (int y = s->yptr[i]; y < s->yptr[i + 1]; ++y) emit(s->ycol[y]);
     }
-  }, &rcnt, &s->cconns, &eend, &s->cpq, &s->cpy);
-  s->ccptr.assign(Nc + 1, 0);
+  }, &s->ccptr, &s->cconns, &s->cs, &s->csp);
+  if (s->ymax < 0 || s->cmax < 0) {
+    set_error("multigrid: a smoothed-aggregation row exceeds 65,535 blocks");
+    return -1;
+  }
+  s->yso.assign(cptr[N], 0);
+  for (int i = 0; i < N; ++i)
+    for (int e = cptr[i], off = 0; e < cptr[i + 1]; ++e) {
+      s->yso[e] = off;
+      off += s->pptr[conns[e] + 1] - s->pptr[conns[e]];
+    }
+  s->cso.assign(nP, 0);
+  for (int a = 0; a < Nc; ++a)
+    for (int k = s->qptr[a], off = 0; k < s->qptr[a + 1]; ++k) {
+      s->cso[k] = off;
+      off += s->yptr[s->qrow[k] + 1] - s->yptr[s->qrow[k]];
+    }
   s->dpos.assign(Nc, -1);
   for (int a = 0; a < Nc; ++a) {
-    s->ccptr[a + 1] = s->ccptr[a] + rcnt[a];
     for (int k = s->ccptr[a]; k < s->ccptr[a + 1]; ++k)
       if (s->cconns[k] == a) s->dpos[a] = k;
     if (s->dpos[a] < 0) {
       set_error("multigrid: aggregate without a diagonal block");
       return -1;
     }
-  }
-  s->cpp.assign(1, 0);
-  s->cpp.insert(s->cpp.end(), eend.begin(), eend.end());
-  if ((int64_t)s->ype.size() >= ((int64_t)1 << 31) || (int64_t)s->cpq.size() >= ((int64_t)1 << 31)) {
-    set_error("multigrid: smoothed-aggregation pair lists exceed 2^31 entries");
-    return -1;
   }
   return 0;
 }
@@ -1051,6 +1224,12 @@ int amg_build(Context* c) {
     SaSym sa;
     if (A->smoothed) {
       if (int rc = sa_symbolic(N, Nc, cptr, conns, agg, &sa)) return rc;
+      // a row's accumulators live in LDS (sa_rows_kernel)
+      if ((int64_t)std::max(sa.ymax, sa.cmax) * D * D * (int64_t)sizeof(double) > 65536 ||
+          sa.ycol.size() >= ((size_t)1 << 31)) {
+        set_error("multigrid: smoothed-aggregation rows too long for the device sums");
+        return -1;
+      }
       ccptr = sa.ccptr;
       cconns = sa.cconns;
       dpos = sa.dpos;
@@ -1106,15 +1285,21 @@ int amg_build(Context* c) {
       LPMG_HIP(copy_sync(c, L.gidx, gidx.data(), sizeof(int) * gidx.size(), hipMemcpyHostToDevice));
     } else {
       L.nP = (int)sa.pcol.size();
-      L.nY = (int64_t)sa.ypp.size() - 1;
+      L.nY = (int64_t)sa.ycol.size();
+      L.sa_gp = sa_outputs_per_wave((int64_t)sa.atidx.size(), L.nP);
+      L.pmax = sa.pmax;
+      L.ymax = sa.ymax;
+      L.cmax = sa.cmax;
       const std::pair<int**, const std::vector<int>*> ups[] = {
           {&L.pptr, &sa.pptr}, {&L.pcol, &sa.pcol}, {&L.prow, &sa.prow}, {&L.atptr, &sa.atptr},
-          {&L.atidx, &sa.atidx}, {&L.qptr, &sa.qptr}, {&L.qidx, &sa.qidx}, {&L.qrow, &sa.qrow},
-          {&L.qpos, &sa.qpos}, {&L.ype, &sa.ype}, {&L.ypq, &sa.ypq}, {&L.cpq, &sa.cpq}, {&L.cpy, &sa.cpy}};
+          {&L.atidx, &sa.atidx}, {&L.qptr, &sa.qptr}, {&L.qrow, &sa.qrow},
+          {&L.qpos, &sa.qpos}, {&L.yptr, &sa.yptr}, {&L.yso, &sa.yso}, {&L.cso, &sa.cso}};
       for (const auto& u : ups)
         if (int r2 = sa_upload(c, u.first, *u.second)) return r2;
-      if (int r2 = sa_upload(c, &L.ypp, sa.ypp)) return r2;
-      if (int r2 = sa_upload(c, &L.cpp, sa.cpp)) return r2;
+      if (int r2 = sa_upload(c, &L.ysp, sa.ysp)) return r2;
+      if (int r2 = sa_upload(c, &L.csp, sa.csp)) return r2;
+      if (int r2 = sa_upload(c, &L.ys, sa.ys)) return r2;
+      if (int r2 = sa_upload(c, &L.cs, sa.cs)) return r2;
     }
     A->lv.push_back(L);
     // next level: the aggregates' centroids and pattern
@@ -1194,17 +1379,21 @@ int amg_values(Context* c) {
   const int P = prec_of<T>();
   const double* fine = c->Kval;
   const int* fdpos = c->dpos;
+  const int *fcptr = c->cptr, *fconns = c->conns;
+  int fN = c->N;
   for (auto& L : A->lv) {
     const int64_t nt = L.nnzb * D * D;
     const unsigned gb = (unsigned)((nt + 255) / 256);
     if (A->smoothed) {
-      sa_p_kernel<D><<<(L.nP + 255) / 256, 256, 0, c->stream>>>(L.prow, L.pcol, L.atptr, L.atidx, L.agg, fdpos,
-                                                               L.qpos, fine, L.nP, kOmega, (double*)L.pval[0],
-                                                               (double*)L.pvalT[0]);
-      sa_y_kernel<D><<<(unsigned)((L.nY + 255) / 256), 256, 0, c->stream>>>(L.ypp, L.ype, L.ypq, fine,
-                                                                            (const double*)L.pval[0], L.nY, L.yval);
-      sa_c_kernel<D><<<(unsigned)((L.nnzb + 255) / 256), 256, 0, c->stream>>>(
-          L.cpp, L.cpq, L.cpy, (const double*)L.pval[0], L.yval, L.nnzb, (double*)L.val[0]);
+      sa_p_kernel<D><<<(unsigned)((L.nP + L.sa_gp - 1) / L.sa_gp), 64, 0, c->stream>>>(
+          L.atptr, L.atidx, fine, L.prow, L.pcol, L.agg, fdpos, L.qpos, L.nP, L.sa_gp, kOmega, (double*)L.pval[0],
+          (double*)L.pvalT[0]);
+      // Y = A P over the level's rows (sources: A's blocks, runs: P's rows)
+      launch_sa_rows<D, 1>(c->stream, fN, L.pmax, L.ymax, fcptr, fconns, fine, L.pptr, (const double*)L.pval[0],
+                           L.ysp, L.yso, L.ys, L.yptr, L.yval);
+      // coarse blocks P^T Y over the aggregates (sources: P's columns, runs: Y's rows)
+      launch_sa_rows<D, 2>(c->stream, L.N, L.ymax, L.cmax, L.qptr, L.qrow, (const double*)L.pvalT[0], L.yptr,
+                           L.yval, L.csp, L.cso, L.cs, L.cptr, (double*)L.val[0]);
       if (P) {
         const int64_t np = (int64_t)L.nP * D * D;
         amg_round_kernel<T><<<(unsigned)((np + 255) / 256), 256, 0, c->stream>>>((const double*)L.pval[0],
@@ -1217,6 +1406,9 @@ int amg_values(Context* c) {
                                                                          (double*)L.val[0]);
     }
     fdpos = L.dpos;
+    fcptr = L.cptr;
+    fconns = L.conns;
+    fN = L.N;
     if (P) amg_round_kernel<T><<<gb, 256, 0, c->stream>>>((const double*)L.val[0], (T*)L.val[P], nt);
     amg_dinv_kernel<T, D><<<(L.N + 255) / 256, 256, 0, c->stream>>>(L.dpos, (const T*)L.val[P], L.N, (T*)L.dinv[P]);
     fine = (const double*)L.val[0];
