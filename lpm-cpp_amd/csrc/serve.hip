@@ -449,13 +449,13 @@ int64_t estimate_bytes(const lpmg_request* r) {
   if (r->opts[0] == 4 || (r->opts[0] < 0 && dim == 2)) {
     // smoothed-aggregation multigrid: per fine row about Pr prolongator
     // blocks (values, FP64 and FP32, in row and column order, and five index
-    // arrays), 3 Pr blocks of A P (FP64 values), the pair lists of A P (A's
-    // blocks of the row times the P blocks of their columns) and of P^T (A P)
-    // (P's blocks of the row times the row's A P blocks), 8 bytes a pair; the
-    // coarser levels add up to 20%.  Counted on the CT plates: Pr 8.0, A P
-    // 21.3, 244 and 172 pairs per row.
+    // arrays), 3 Pr blocks of A P (FP64 values), a 2-byte slot per term of A P
+    // (A's blocks of the row times the P blocks of their columns) and of
+    // P^T (A P) (P's blocks of the row times the row's A P blocks), and a
+    // 4-byte offset per source of either; the coarser levels add up to 20%.
+    // Counted on the CT plates: Pr 8.0, A P 21.3, 244 and 172 terms per row.
     const double pr = dim == 2 ? 9.0 : 20.0, yr = 3.0 * pr, k = nnzb / N;
-    b += 1.2 * N * (8.0 * (k * pr + pr * yr) + yr * DD * 8.0 + pr * (DD * 24.0 + 24.0));
+    b += 1.2 * N * (2.0 * (k * pr + pr * yr) + 4.0 * (k + pr) + yr * DD * 8.0 + pr * (DD * 24.0 + 24.0));
   }
   if (r->nonlocal_ratio > 0 && r->n_params > 0) {
     // nonlocal neighbours within ratio * L (params[0] is nonlocal_L)
