@@ -65,6 +65,7 @@ namespace lpmg {
 namespace {
 
 constexpr int kCoarsestMax = 128;  // unknowns of the dense coarsest level (fits LDS)
+constexpr int kTailChunk = 4;      // PCG iterations of the short chunk that ends a solve
 constexpr double kOmega = 2.0 / 3.0;
 // Scale of the coarse-grid correction.  Unsmoothed aggregation's piecewise-
 // constant prolongation under-corrects the smooth error; scaling the correction
@@ -414,6 +415,13 @@ __global__ void __launch_bounds__(1024) amg_invert_kernel(const int* __restrict_
 // at 150k, against the fixed 2/3.
 constexpr int kLzSteps = 12;
 constexpr double kOmegaMax = 0.9, kOmegaTarget = 1.65;
+// The prolongator's smoothing P = (I - om_p D^-1 A) T takes om_p = kPTarget /
+// lambda_est (at most 1): 1.5 / 1.86 = 0.81 on the intact CT lattice instead of
+// 2/3 (CPU experiment, PCG iterations to ||r||^2 <= 1e-4 ||r0||^2 with the
+// adaptive smoother: 9 -> 7 at 20k particles, 15 -> 12 at 80k, 10 -> 10 on a
+// cracked 6k specimen; 4/3 took 8, 13 and 10).  Any om_p gives a valid
+// prolongator: the V-cycle stays a positive definite preconditioner.
+constexpr double kPTarget = 1.5;
 
 __device__ __forceinline__ double lz_start(int64_t k) {  // start vector entries in [-0.5, 0.5)
   const uint32_t h = (uint32_t)((uint64_t)k * 2654435761ull) >> 16;
@@ -494,7 +502,8 @@ __global__ void lz_update_kernel(const int* __restrict__ dpos, const double* __r
 // off-diagonals beta_1..beta_{m-1}; m stops at the first vanishing beta) by
 // bisection on its Sturm count, and the damping it gives (2/3 if the estimate
 // is not usable).  One thread.
-__global__ void lz_omega_kernel(const double* __restrict__ coef, double* __restrict__ omega) {
+__global__ void lz_omega_kernel(const double* __restrict__ coef, double* __restrict__ omega,
+                                double* __restrict__ omega_p) {
   if (threadIdx.x != 0) return;
   const double* al = coef;
   const double* be = coef + kLzSteps;
@@ -523,7 +532,9 @@ __global__ void lz_omega_kernel(const double* __restrict__ coef, double* __restr
     else lo = x;
   }
   const double lam = hi;
-  *omega = (lam > 0 && isfinite(lam)) ? fmin(kOmegaMax, kOmegaTarget / lam) : kOmega;
+  const bool ok = lam > 0 && isfinite(lam);
+  *omega = ok ? fmin(kOmegaMax, kOmegaTarget / lam) : kOmega;
+  *omega_p = ok ? fmin(1.0, kPTarget / lam) : kOmega;
 }
 
 // P's blocks: the sum of the level's blocks each one covers (pair list atidx,
@@ -538,7 +549,8 @@ __global__ void __launch_bounds__(64) sa_p_kernel(const int* __restrict__ atptr,
                                                   const double* __restrict__ A, const int* __restrict__ prow,
                                                   const int* __restrict__ pcol, const int* __restrict__ agg,
                                                   const int* __restrict__ dpos, const int* __restrict__ qpos,
-                                                  int nP, int G, double om, double* __restrict__ Pv,
+                                                  int nP, int G, const double* __restrict__ omp,
+                                                  double* __restrict__ Pv,
                                                   double* __restrict__ PvT) {
   constexpr int DD = D * D;
   __shared__ double term[DD * 64];
@@ -566,6 +578,7 @@ __global__ void __launch_bounds__(64) sa_p_kernel(const int* __restrict__ atptr,
     __syncthreads();
   }
   if (!mine) return;
+  const double om = omp[0];
   const int i = prow[o];
   const size_t dg = (size_t)dpos[i] * DD;
   const bool own = pcol[o] == agg[i];
@@ -1065,18 +1078,20 @@ struct Amg {
   uint16_t* fine16 = nullptr;  // 2D FP32: the finest level's values as bfloat16, for the smoothers
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   std::vector<uintptr_t> graph_sig[2];  // pointers and shapes captured into graph[P]
+  hipGraphExec_t graph_tail[2] = {nullptr, nullptr};  // the short chunk that ends a solve
+  std::vector<uintptr_t> graph_tail_sig[2];
+  int last_iters[2] = {0, 0};  // the previous solve's iterations per precision
   // smoother damping per level with a smoother (device memory, read by the
   // V-cycle's kernels): 2/3 for the tentative hierarchy; for the smoothed one
   // set after every value rebuild from a Lanczos estimate of the level's
   // largest eigenvalue (lz_coef: its coefficients, lz_part: its partial sums)
   double* omega = nullptr;
+  double* omega_p = nullptr;  // the prolongator's smoothing factor per level (smoothed hierarchy)
   double* lz_coef = nullptr;
   double* lz_part = nullptr;
   int lz_nparts = 0;
-  hipGraphExec_t lz_graph = nullptr;  // the estimate's launches, captured once
-  std::vector<uintptr_t> lz_sig;
-  std::vector<hipStream_t> lz_streams;  // the coarser levels' estimates run beside the finest's
-  std::vector<hipEvent_t> lz_events;    // fork, then one join per coarser level
+  std::vector<hipGraphExec_t> lz_graphs;  // each level's estimate, captured once
+  std::vector<std::vector<uintptr_t>> lz_sigs;
   // the hierarchy's device arrays (freed with it) and their bytes
   std::vector<void*> allocs;
   size_t bytes = 0;
@@ -1489,15 +1504,17 @@ int amg_build(Context* c) {
   // replaces it) and the estimate's buffers
   const int L = (int)A->lv.size();
   if (L > 0) {
-    if (amg_alloc(c, &A->omega, L)) return -3;
+    if (amg_alloc(c, &A->omega, L) || amg_alloc(c, &A->omega_p, L)) return -3;
     const std::vector<double> om(L, kOmega);
     LPMG_HIP(copy_sync(c, A->omega, om.data(), sizeof(double) * L, hipMemcpyHostToDevice));
+    LPMG_HIP(copy_sync(c, A->omega_p, om.data(), sizeof(double) * L, hipMemcpyHostToDevice));
     if (A->smoothed) {
-      // partials: each level's own (its estimate runs concurrently with the others)
-      int lpr = 64, u = 1;
+      // partials of the largest level's estimate (the levels' estimates run in turn)
+      int lpr = 64, u = 1, gmax = 0;
       fine_shape(c, &lpr, &u);
-      A->lz_nparts = rows_grid(c->N, lpr) + 2 * kMaxGrid;
-      for (int l = 1; l < L; ++l) A->lz_nparts += rows_grid(A->lv[l - 1].N, A->lv[l - 1].lpr) + 2 * kMaxGrid;
+      gmax = rows_grid(c->N, lpr);
+      for (int l = 1; l < L; ++l) gmax = std::max(gmax, rows_grid(A->lv[l - 1].N, A->lv[l - 1].lpr));
+      A->lz_nparts = gmax + 2 * kMaxGrid;
       if (amg_alloc(c, &A->lz_part, A->lz_nparts) || amg_alloc(c, &A->lz_coef, 2 * kLzSteps * L)) return -3;
     }
   }
@@ -1553,113 +1570,73 @@ __global__ void amg_round_kernel(const double* __restrict__ a, T* __restrict__ b
   if (t < n) b[t] = (T)a[t];
 }
 
-// The smoothed hierarchy's smoother damping, level by level (levels 0..L-1,
-// those with a smoother), from kLzSteps Lanczos steps on the level's FP64
-// values; the vectors used are free while the values are rebuilt (level 0:
-// the FP64 CG's, coarser levels: their own FP64 work vectors).  The launches
-// (two per step and level) are captured into a graph on first use and
-// replayed after every value rebuild: 1.3 ms of eager launches per rebuild on
-// the 205,645-particle CT specimen.
+// Level l's smoother damping and prolongator smoothing factor (smoothed
+// hierarchy, levels 0..L-1) from kLzSteps Lanczos steps on its FP64 values,
+// enqueued before its prolongator is formed; the vectors used are free while
+// the values are rebuilt (level 0: the FP64 CG's, coarser levels: their own
+// FP64 work vectors).  A level's launches (two per step) are captured into a
+// graph on first use and replayed after every later value rebuild (eager, the
+// launches of all levels took 1.3 ms per rebuild on the 205,645-particle CT
+// specimen).
 template <int D>
-int amg_damping(Context* c) {
+int amg_damping_level(Context* c, int l) {
   Amg* A = c->amg;
-  const int L = (int)A->lv.size();
-  if (L == 0) return 0;  // a single level: the coarsest solve, no smoother
-  struct Lv {
-    int N, lpr, uc, g, gv;
-    int64_t n, poff;
-    const int *cptr, *conns, *dpos;
-    const double* val;
-    double *t, *y, *u0, *u1;
-  };
-  std::vector<Lv> lv(L);
+  const AmgLevel* B = l == 0 ? nullptr : &A->lv[l - 1];
+  const int N = l == 0 ? c->N : B->N;
+  const int* cptr = l == 0 ? c->cptr : B->cptr;
+  const int* conns = l == 0 ? c->conns : B->conns;
+  const int* dpos = l == 0 ? c->dpos : B->dpos;
+  const double* val = l == 0 ? c->Kval : (const double*)B->val[0];
+  double* t = l == 0 ? c->cg_p : (double*)B->vec[0][0];
+  double* y = l == 0 ? c->cg_w : (double*)B->vec[0][1];
+  double* u[2] = {l == 0 ? c->cg_z : (double*)B->vec[0][2], l == 0 ? c->cg_r : (double*)B->vec[0][3]};
+  int lpr = 64, uc = 1;
+  if (l == 0) fine_shape(c, &lpr, &uc);
+  else lpr = B->lpr;
+  const int g = rows_grid(N, lpr);
+  const int64_t n = (int64_t)D * N;
+  const int gv = grid_n(n, kBS * 4);
+  if (g + 2 * kMaxGrid > A->lz_nparts) {
+    set_error("multigrid: damping estimate partials buffer too small");
+    return -1;
+  }
+  double* pa = A->lz_part;
+  double* pb[2] = {pa + g, pa + g + kMaxGrid};
+  double* coef = A->lz_coef + 2 * kLzSteps * l;
   std::vector<uintptr_t> sig;
-  for (int l = 0; l < L; ++l) {
-    const AmgLevel* B = l == 0 ? nullptr : &A->lv[l - 1];
-    Lv& v = lv[l];
-    v.N = l == 0 ? c->N : B->N;
-    v.cptr = l == 0 ? c->cptr : B->cptr;
-    v.conns = l == 0 ? c->conns : B->conns;
-    v.dpos = l == 0 ? c->dpos : B->dpos;
-    v.val = l == 0 ? c->Kval : (const double*)B->val[0];
-    v.t = l == 0 ? c->cg_p : (double*)B->vec[0][0];
-    v.y = l == 0 ? c->cg_w : (double*)B->vec[0][1];
-    v.u0 = l == 0 ? c->cg_z : (double*)B->vec[0][2];
-    v.u1 = l == 0 ? c->cg_r : (double*)B->vec[0][3];
-    v.lpr = 64;
-    v.uc = 1;
-    if (l == 0) fine_shape(c, &v.lpr, &v.uc);
-    else v.lpr = B->lpr;
-    v.g = rows_grid(v.N, v.lpr);
-    v.n = (int64_t)D * v.N;
-    v.gv = grid_n(v.n, kBS * 4);
-    v.poff = l == 0 ? 0 : lv[l - 1].poff + lv[l - 1].g + 2 * kMaxGrid;
-    if (v.poff + v.g + 2 * kMaxGrid > A->lz_nparts) {
-      set_error("multigrid: damping estimate partials buffer too small");
-      return -1;
-    }
-    for (const void* q : {(const void*)v.cptr, (const void*)v.conns, (const void*)v.dpos, (const void*)v.val,
-                          (const void*)v.t, (const void*)v.y, (const void*)v.u0, (const void*)v.u1})
-      graph_sig_push(&sig, q);
-    for (int64_t x : {(int64_t)v.N, (int64_t)v.lpr, (int64_t)v.uc}) graph_sig_push(&sig, x);
-  }
-  for (const void* q : {(const void*)A->lz_part, (const void*)A->lz_coef, (const void*)A->omega})
+  for (const void* q : {(const void*)cptr, (const void*)conns, (const void*)dpos, (const void*)val, (const void*)t,
+                        (const void*)y, (const void*)u[0], (const void*)u[1], (const void*)A->lz_part,
+                        (const void*)A->lz_coef, (const void*)A->omega, (const void*)A->omega_p})
     graph_sig_push(&sig, q);
-  // side streams and events for the coarser levels (default priority)
-  while ((int)A->lz_streams.size() < L - 1) {
-    hipStream_t q = nullptr;
-    LPMG_HIP(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
-    A->lz_streams.push_back(q);
+  for (int64_t x : {(int64_t)N, (int64_t)lpr, (int64_t)uc}) graph_sig_push(&sig, x);
+  if ((int)A->lz_graphs.size() <= l) {
+    A->lz_graphs.resize(l + 1, nullptr);
+    A->lz_sigs.resize(l + 1);
   }
-  while ((int)A->lz_events.size() < L) {
-    hipEvent_t e = nullptr;
-    LPMG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    A->lz_events.push_back(e);
-  }
-  auto level = [&](int l, hipStream_t st) {
-    const Lv& v = lv[l];
-    double* pa = A->lz_part + v.poff;
-    double* pb[2] = {pa + v.g, pa + v.g + kMaxGrid};
-    double* u[2] = {v.u0, v.u1};
-    double* coef = A->lz_coef + 2 * kLzSteps * l;
-    lz_init_kernel<D><<<v.gv, kBS, 0, st>>>(v.dpos, v.val, v.n, v.t, pb[0]);
-    for (int k = 0; k < kLzSteps; ++k) {
-      with_shape(v.lpr, v.uc, [&](auto LP, auto UC) {
-        lz_spmv_kernel<D, decltype(LP)::value, decltype(UC)::value>
-            <<<v.g, kSpBS, 0, st>>>(v.cptr, v.conns, v.val, v.t, v.y, v.N, pa);
-      });
-      lz_update_kernel<D><<<v.gv, kBS, 0, st>>>(v.dpos, v.val, v.n, k, pb[k & 1], v.gv, pa, v.g, v.y, v.t,
-                                                u[(k + 1) & 1], u[k & 1], pb[(k + 1) & 1], coef);
-    }
-    lz_omega_kernel<<<1, 64, 0, st>>>(coef, A->omega + l);
-  };
-  // fork: the coarser levels on the side streams, the finest on the
-  // handle's stream; join before anything after the estimate
-  auto enqueue = [&]() {
-    (void)hipEventRecord(A->lz_events[0], c->stream);
-    for (int l = 1; l < L; ++l) {
-      (void)hipStreamWaitEvent(A->lz_streams[l - 1], A->lz_events[0], 0);
-      level(l, A->lz_streams[l - 1]);
-      (void)hipEventRecord(A->lz_events[l], A->lz_streams[l - 1]);
-    }
-    level(0, c->stream);
-    for (int l = 1; l < L; ++l) (void)hipStreamWaitEvent(c->stream, A->lz_events[l], 0);
-  };
-  if (A->lz_graph) {
-    if (int rc = check_graph_signature(A->lz_sig, sig, "multigrid damping estimate")) return rc;
+  if (A->lz_graphs[l]) {
+    if (int rc = check_graph_signature(A->lz_sigs[l], sig, "multigrid damping estimate")) return rc;
   } else {
     hipGraph_t graph = nullptr;
     LPMG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    enqueue();
+    lz_init_kernel<D><<<gv, kBS, 0, c->stream>>>(dpos, val, n, t, pb[0]);
+    for (int k = 0; k < kLzSteps; ++k) {
+      with_shape(lpr, uc, [&](auto LP, auto UC) {
+        lz_spmv_kernel<D, decltype(LP)::value, decltype(UC)::value>
+            <<<g, kSpBS, 0, c->stream>>>(cptr, conns, val, t, y, N, pa);
+      });
+      lz_update_kernel<D><<<gv, kBS, 0, c->stream>>>(dpos, val, n, k, pb[k & 1], gv, pa, g, y, t, u[(k + 1) & 1],
+                                                     u[k & 1], pb[(k + 1) & 1], coef);
+    }
+    lz_omega_kernel<<<1, 64, 0, c->stream>>>(coef, A->omega + l, A->omega_p + l);
     const hipError_t e1 = hipGetLastError();
     const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
     LPMG_HIP(e1);
     LPMG_HIP(e2);
-    LPMG_HIP(hipGraphInstantiate(&A->lz_graph, graph, nullptr, nullptr, 0));
+    LPMG_HIP(hipGraphInstantiate(&A->lz_graphs[l], graph, nullptr, nullptr, 0));
     LPMG_HIP(hipGraphDestroy(graph));
-    A->lz_sig = sig;
+    A->lz_sigs[l] = sig;
   }
-  LPMG_HIP(hipGraphLaunch(A->lz_graph, c->stream));
+  LPMG_HIP(hipGraphLaunch(A->lz_graphs[l], c->stream));
   return 0;
 }
 
@@ -1671,12 +1648,15 @@ int amg_values(Context* c) {
   const int* fdpos = c->dpos;
   const int *fcptr = c->cptr, *fconns = c->conns;
   int fN = c->N;
-  for (auto& L : A->lv) {
+  for (int l = 0; l < (int)A->lv.size(); ++l) {
+    AmgLevel& L = A->lv[l];
     const int64_t nt = L.nnzb * D * D;
     const unsigned gb = (unsigned)((nt + 255) / 256);
     if (A->smoothed) {
+      if (int rc = amg_damping_level<D>(c, l)) return rc;
       sa_p_kernel<D><<<(unsigned)((L.nP + L.sa_gp - 1) / L.sa_gp), 64, 0, c->stream>>>(
-          L.atptr, L.atidx, fine, L.prow, L.pcol, L.agg, fdpos, L.qpos, L.nP, L.sa_gp, kOmega, (double*)L.pval[0],
+          L.atptr, L.atidx, fine, L.prow, L.pcol, L.agg, fdpos, L.qpos, L.nP, L.sa_gp, A->omega_p + l,
+          (double*)L.pval[0],
           (double*)L.pvalT[0]);
       // Y = A P over the level's rows (sources: A's blocks, runs: P's rows)
       launch_sa_rows<D, 1>(c->stream, fN, L.pmax, L.ymax, fcptr, fconns, fine, L.pptr, (const double*)L.pval[0],
@@ -1711,8 +1691,6 @@ int amg_values(Context* c) {
                                                               (T*)A->inv[P]);
   }
   LPMG_HIP(hipGetLastError());
-  if (A->smoothed)
-    if (int rc = amg_damping<D>(c)) return rc;
   return 0;
 }
 
@@ -1802,10 +1780,10 @@ void amg_free(Context* c) {
   if (!c->amg) return;
   for (auto& g : c->amg->graph)
     if (g) (void)hipGraphExecDestroy(g);
-  if (c->amg->lz_graph) (void)hipGraphExecDestroy(c->amg->lz_graph);
-  if (c->stream) (void)hipStreamSynchronize(c->stream);  // nothing queued may still use the side streams
-  for (hipEvent_t e : c->amg->lz_events) (void)hipEventDestroy(e);
-  for (hipStream_t q : c->amg->lz_streams) (void)hipStreamDestroy(q);
+  for (auto& g : c->amg->graph_tail)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& g : c->amg->lz_graphs)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* q : c->amg->allocs) (void)hipFreeAsync(q, c->stream);  // after the work queued on them
   c->bytes -= c->amg->bytes;
   delete c->amg;
@@ -1875,8 +1853,8 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
   pcg_init2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv, c->sc);
   LPMG_HIP(hipGetLastError());
-  auto enqueue_chunk = [&]() -> int {
-    for (int k = 0; k < kCgChunk; ++k) {
+  auto enqueue_chunk = [&](int iters) -> int {
+    for (int k = 0; k < iters; ++k) {
       with_shape(lpr, u, [&](auto LP, auto UC) {
         pcg_spmv_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
             <<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
@@ -1891,7 +1869,6 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
     LPMG_HIP(hipGetLastError());
     return 0;
   };
-  hipGraphExec_t* gexec = &A->graph[P];
   std::vector<uintptr_t> sig;
   for (const void* q : {(const void*)c->cptr, (const void*)c->conns, (const void*)val0, (const void*)dinv0,
                         (const void*)x, (const void*)r, (const void*)z, (const void*)p, (const void*)w,
@@ -1908,25 +1885,48 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   graph_sig_push(&sig, (int64_t)A->smoothed);
   for (int64_t v : {(int64_t)N, (int64_t)gs, (int64_t)gv, (int64_t)lpr, (int64_t)u, (int64_t)two, (int64_t)A->nc})
     graph_sig_push(&sig, v);
-  if (*gexec) {
-    if (int rc = check_graph_signature(A->graph_sig[P], sig, "multigrid PCG chunk")) return rc;
-  } else {
+  // two captured chunks: kCgChunk iterations, and kTailChunk for the end of
+  // a solve -- once the iterations launched pass the previous solve's count
+  // on this handle, short chunks follow, so that fewer iterations past
+  // convergence are launched (each costs its launches though every kernel
+  // returns at once); chunks of either length run the same iterations
+  auto graph_for = [&](int which) -> hipGraphExec_t* {
+    hipGraphExec_t* g = which ? &A->graph_tail[P] : &A->graph[P];
+    std::vector<uintptr_t>& gs_sig = which ? A->graph_tail_sig[P] : A->graph_sig[P];
+    if (*g) {
+      if (check_graph_signature(gs_sig, sig, "multigrid PCG chunk")) return nullptr;
+      return g;
+    }
     hipGraph_t graph = nullptr;
-    LPMG_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = enqueue_chunk();
-    hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
-    if (rc) return rc;
-    LPMG_HIP(e2);
-    LPMG_HIP(hipGraphInstantiate(gexec, graph, nullptr, nullptr, 0));
-    LPMG_HIP(hipGraphDestroy(graph));
-    A->graph_sig[P] = sig;
-  }
-  for (;;) {
-    LPMG_HIP(hipGraphLaunch(*gexec, c->stream));
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+    const int rc = enqueue_chunk(which ? kTailChunk : kCgChunk);
+    const hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+    if (rc || e2 != hipSuccess) {
+      if (!rc) hip_fail(e2, "hipStreamEndCapture");
+      return nullptr;
+    }
+    const hipError_t e3 = hipGraphInstantiate(g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e3 != hipSuccess) {
+      hip_fail(e3, "hipGraphInstantiate");
+      *g = nullptr;
+      return nullptr;
+    }
+    gs_sig = sig;
+    return g;
+  };
+  const int pred = A->last_iters[P];
+  for (int launched = 0;;) {
+    const bool tail = pred > 0 && launched + kCgChunk > pred + kTailChunk && launched < pred + 2 * kCgChunk;
+    hipGraphExec_t* g = graph_for(tail ? 1 : 0);
+    if (!g) return -1;
+    LPMG_HIP(hipGraphLaunch(*g, c->stream));
+    launched += tail ? kTailChunk : kCgChunk;
     LPMG_HIP(hipMemcpyAsync(c->sc_host, c->sc, sizeof(Scalars), hipMemcpyDeviceToHost, c->stream));
     LPMG_HIP(hipStreamSynchronize(c->stream));
     if (c->sc_host->done) break;
   }
+  A->last_iters[P] = c->sc_host->iter;
   *iters = c->sc_host->iter;
   *rr_final = c->sc_host->rr;
   return 0;

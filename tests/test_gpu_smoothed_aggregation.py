@@ -8,12 +8,13 @@ tolerance, damage flags exactly equal.  The hierarchy itself is pinned by
 `sa_pcg_np`, a plain numpy/scipy restatement of the method on the oracle's
 matrix: grid-cell aggregates (cell edge 3 lattice spacings, x3.5 per level,
 doubled while a level keeps more than 80% of its nodes), the prolongation
-P = (I - 2/3 D^-1 A) T of the piecewise-constant T, Galerkin coarse operators
-P^T A P, a dense coarsest inverse, one V-cycle of damped Jacobi before and
-after the coarse correction -- its damping per level min(0.9, 1.65 / lambda)
-with lambda the largest eigenvalue of the 12-step Lanczos tridiagonal of
-D^-1 A from a fixed hashed start vector -- z = 0 on the eliminated DOFs: the
-level sizes must agree exactly and the PCG iteration counts to rounding.  A dropped
+P = (I - om_p D^-1 A) T of the piecewise-constant T, Galerkin coarse
+operators P^T A P, a dense coarsest inverse, one V-cycle of damped Jacobi
+before and after the coarse correction, z = 0 on the eliminated DOFs.  Per
+level, lambda is the largest eigenvalue of the 12-step Lanczos tridiagonal of
+D^-1 A from a fixed hashed start vector; the smoother's damping is
+min(0.9, 1.65 / lambda) and om_p = min(1, 1.5 / lambda).  The level sizes
+must agree exactly and the PCG iteration counts to rounding.  A dropped
 smoothing term, a transposed P or a wrong sign in it changes the count by far
 more than that (the tentative hierarchy takes 2-3x as many iterations on
 CT plates).
@@ -34,8 +35,8 @@ pytestmark = pytest.mark.gpu
 R = 0.1
 SA64 = dict(precond=4, rtol=1e-24, atol=0.0, precision=0)
 ORC = dict(precond=0, rtol=1e-24, atol=0.0)
-OMEGA = 2.0 / 3.0        # the prolongator's smoothing
-LZ_STEPS, OMEGA_MAX, OMEGA_TARGET = 12, 0.9, 1.65
+OMEGA = 2.0 / 3.0        # the damping when no estimate is usable
+LZ_STEPS, OMEGA_MAX, OMEGA_TARGET, P_TARGET = 12, 0.9, 1.65, 1.5
 
 
 def _relerr(a, b):
@@ -58,10 +59,11 @@ def lanczos_start(n):
     return (h & np.uint64(1023)).astype(np.float64) / 1024.0 - 0.5
 
 
-def smoother_damping(A):
-    """min(OMEGA_MAX, OMEGA_TARGET / lambda): lambda the largest eigenvalue of
-    the LZ_STEPS-step Lanczos tridiagonal of D^-1/2 A D^-1/2 from the hashed
-    start vector (a lower estimate of rho(D^-1 A))."""
+def dampings(A):
+    """(smoother damping min(OMEGA_MAX, OMEGA_TARGET / lambda), prolongator
+    smoothing min(1, P_TARGET / lambda)): lambda the largest eigenvalue of the
+    LZ_STEPS-step Lanczos tridiagonal of D^-1/2 A D^-1/2 from the hashed start
+    vector (a lower estimate of rho(D^-1 A))."""
     s = 1.0 / np.sqrt(A.diagonal())
     v = lanczos_start(A.shape[0])
     v /= np.linalg.norm(v)
@@ -79,7 +81,9 @@ def smoother_damping(A):
     m = len(al)
     T = np.diag(al) + np.diag(be[1:m], 1) + np.diag(be[1:m], -1)
     lam = np.linalg.eigvalsh(T)[-1]
-    return min(OMEGA_MAX, OMEGA_TARGET / lam) if lam > 0 else OMEGA
+    if not lam > 0:
+        return OMEGA, OMEGA
+    return min(OMEGA_MAX, OMEGA_TARGET / lam), min(1.0, P_TARGET / lam)
 
 
 def sa_hierarchy(A, pos, radius, dim, coarsest=128):
@@ -96,8 +100,9 @@ def sa_hierarchy(A, pos, radius, dim, coarsest=128):
         T = sp.csr_matrix((np.ones(n * dim), (rows, np.repeat(agg, dim) * dim + np.tile(np.arange(dim), n))),
                           shape=(n * dim, nc * dim))
         dinv = 1.0 / A.diagonal()
-        Pm = (T - OMEGA * (sp.diags(dinv) @ (A @ T))).tocsr()
-        levels.append((A, Pm, dinv, smoother_damping(A)))
+        om, om_p = dampings(A)
+        Pm = (T - om_p * (sp.diags(dinv) @ (A @ T))).tocsr()
+        levels.append((A, Pm, dinv, om))
         A = (Pm.T @ (A @ Pm)).tocsr()
         cen = np.zeros((nc, 3))
         np.add.at(cen, agg, pos)
