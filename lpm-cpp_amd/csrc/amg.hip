@@ -210,6 +210,27 @@ __global__ void __launch_bounds__(kSpBS) amg_presmooth_resid_kernel(const int* _
   }
 }
 
+// r = b - A x1 with x1 = om D^-1 b written by the kernel that produced b (the
+// smoothed hierarchy's pre-smoothing: one gathered vector per block, as an
+// SpMV, where forming x1 at every gathered column reads two)
+template <int D, typename T, int LPR, int U, typename TV = T>
+__global__ void __launch_bounds__(kSpBS) amg_presmooth_x1_kernel(const int* __restrict__ cptr,
+                                                                 const int* __restrict__ conns,
+                                                                 const TV* __restrict__ val, int N,
+                                                                 const T* __restrict__ b, const T* __restrict__ x1,
+                                                                 T* __restrict__ r, const Scalars* sc) {
+  if (sc->done) return;
+  int row, sl;
+  bool live;
+  T acc[D];
+  row_gather<D, T, LPR, U, 0, TV>(blockIdx.x * (kSpBS / 64) + (threadIdx.x >> 6), cptr, conns, val, x1, nullptr,
+                                  nullptr, T(0), N, &row, &live, &sl, acc);
+  if (live && sl < D) {
+    const size_t q = (size_t)row * D + sl;
+    r[q] = b[q] - pick<D, T>(acc, sl);
+  }
+}
+
 // z = x + om D^-1 (b - A x) with x = om D^-1 b + P xc, the pre-smoothed
 // iterate plus the prolonged coarse correction, formed where it is read (for
 // the row itself and for every gathered column: no prolongation launch and
@@ -736,7 +757,9 @@ void launch_sa_rows(hipStream_t st, int nrows, int longest_run, int width, const
 template <int D, typename T>
 __global__ void __launch_bounds__(256) sa_restrict_kernel(const int* __restrict__ qptr, const int* __restrict__ qrow,
                                                           const T* __restrict__ PvT, int Nc, const T* __restrict__ r,
-                                                          T* __restrict__ bc, const Scalars* sc) {
+                                                          T* __restrict__ bc, const T* __restrict__ dinvc,
+                                                          const double* __restrict__ omc, T* __restrict__ x1c,
+                                                          const Scalars* sc) {
   if (sc->done) return;
   constexpr int DD = D * D;
   const int a = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -760,19 +783,22 @@ __global__ void __launch_bounds__(256) sa_restrict_kernel(const int* __restrict_
 #pragma unroll
     for (int k = 1; k < D; ++k)
       if (lane == k) v = s[k];
-    bc[(size_t)a * D + lane] = (T)v;
+    const size_t q = (size_t)a * D + lane;
+    const T bq = (T)v;
+    bc[q] = bq;
+    // the coarse level's pre-smoothed iterate om D^-1 bc (when it has a smoother)
+    if (x1c) x1c[q] = (T)omc[0] * dinvc[q] * bq;
   }
 }
 
-// x = om D^-1 b + P xc (thread per row; the row's P blocks in order)
+// x = x1 + P xc, x1 = om D^-1 b the pre-smoothed iterate (thread per row; the
+// row's P blocks in order)
 template <int D, typename T>
 __global__ void __launch_bounds__(256) sa_prolong_kernel(const int* __restrict__ pptr, const int* __restrict__ pcol,
-                                                         const T* __restrict__ Pv, int N, const T* __restrict__ b,
-                                                         const T* __restrict__ dinv, const double* __restrict__ omp,
+                                                         const T* __restrict__ Pv, int N, const T* __restrict__ x1,
                                                          const T* __restrict__ xc, T* __restrict__ x,
                                                          const Scalars* sc) {
   if (sc->done) return;
-  const T om = (T)omp[0];
   constexpr int DD = D * D;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= N) return;
@@ -811,7 +837,7 @@ __global__ void __launch_bounds__(256) sa_prolong_kernel(const int* __restrict__
 #pragma unroll
   for (int q = 0; q < D; ++q) {
     const size_t iq = (size_t)i * D + q;
-    x[iq] = om * dinv[iq] * b[iq] + acc[q];
+    x[iq] = x1[iq] + acc[q];
   }
 }
 
@@ -875,11 +901,15 @@ __global__ void __launch_bounds__(kBS) pcg_reduce_kernel(const double* __restric
 }
 
 // x += alpha p; r -= alpha w; partials of (r,r).  alpha = gamma_i / (p,w).
+// With x1 (the smoothed hierarchy): x1 = om D^-1 r, the finest level's
+// pre-smoothed iterate.
 template <typename T>
 __global__ void __launch_bounds__(kBS) pcg_vec1_kernel(T* __restrict__ x, T* __restrict__ r,
                                                        const T* __restrict__ p, const T* __restrict__ w, int64_t n,
                                                        const double* __restrict__ pw_part, int npw,
-                                                       double* __restrict__ rr_part, Scalars* sc) {
+                                                       double* __restrict__ rr_part, const T* __restrict__ dinv,
+                                                       const double* __restrict__ omp, T* __restrict__ x1,
+                                                       Scalars* sc) {
   if (sc->done) return;
   const int i = sc->icur;
   const double pw = sum_partials<kBS, 1>(pw_part, npw).v[0];
@@ -892,11 +922,13 @@ __global__ void __launch_bounds__(kBS) pcg_vec1_kernel(T* __restrict__ x, T* __r
     sc->alpha = alpha;
   }
   const T a = (T)alpha;
+  const T om = x1 ? (T)omp[0] : T(0);
   double rr = 0;
   for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
     x[k] += a * p[k];
     const T rk = r[k] - a * w[k];
     r[k] = rk;
+    if (x1) x1[k] = om * dinv[k] * rk;
     rr += (double)rk * (double)rk;
   }
   Vec<1> v;
@@ -933,15 +965,19 @@ __global__ void __launch_bounds__(kBS) pcg_vec2_kernel(T* __restrict__ p, const 
   for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) p[k] = z[k] + b * p[k];
 }
 
-// start: x = 0, r = b (rhs); partials of (r,r)
+// start: x = 0, r = b (rhs); partials of (r,r); with x1, x1 = om D^-1 r
 template <typename T>
 __global__ void __launch_bounds__(kBS) pcg_init_kernel(const double* __restrict__ rhs, T* __restrict__ x,
-                                                       T* __restrict__ r, int64_t n, double* __restrict__ rr_part) {
+                                                       T* __restrict__ r, int64_t n, double* __restrict__ rr_part,
+                                                       const T* __restrict__ dinv, const double* __restrict__ omp,
+                                                       T* __restrict__ x1) {
+  const T om = x1 ? (T)omp[0] : T(0);
   double rr = 0;
   for (int64_t k = (int64_t)blockIdx.x * kBS + threadIdx.x; k < n; k += (int64_t)gridDim.x * kBS) {
     const T rk = (T)rhs[k];
     x[k] = 0;
     r[k] = rk;
+    if (x1) x1[k] = om * dinv[k] * rk;
     rr += (double)rk * (double)rk;
   }
   Vec<1> v;
@@ -1047,7 +1083,7 @@ struct AmgLevel {  // level l >= 1; its nodes are the aggregates of level l-1
   int* agg = nullptr;        // aggregate of each level l-1 node
   void* val[2] = {nullptr, nullptr};   // FP64 / FP32 values
   void* dinv[2] = {nullptr, nullptr};
-  void* vec[2][4] = {{nullptr}};       // b, r, x, y per precision
+  void* vec[2][5] = {{nullptr}};       // b, r, x, y, x1 (smoothed hierarchy) per precision
   // smoothed aggregation (level l-1 -> l): P's blocks by row (pptr over level
   // l-1 nodes; column, row, and the level l-1 blocks each one sums), by
   // column (qptr over the aggregates), and the pair lists of Y = A P and of
@@ -1541,13 +1577,13 @@ int amg_alloc_prec(Context* c) {
       if (P == 0 && amg_alloc(c, &L.yval, (size_t)L.nY * D * D)) return -3;
     }
     if (L.val[P]) continue;
-    T *v = nullptr, *di = nullptr, *w[4] = {nullptr, nullptr, nullptr, nullptr};
+    T *v = nullptr, *di = nullptr, *w[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     int rc = amg_alloc(c, &v, (size_t)L.nnzb * D * D) | amg_alloc(c, &di, (size_t)L.N * D);
     for (auto& q : w) rc |= amg_alloc(c, &q, (size_t)L.N * D);
     if (rc) return -3;
     L.val[P] = v;
     L.dinv[P] = di;
-    for (int k = 0; k < 4; ++k) L.vec[P][k] = w[k];
+    for (int k = 0; k < 5; ++k) L.vec[P][k] = w[k];
   }
   if (P == 1 && D == 2 && !A->fine16)
     if (amg_alloc(c, &A->fine16, (size_t)c->nnzb * 4)) return -3;
@@ -1697,7 +1733,9 @@ int amg_values(Context* c) {
 // Enqueue z = M b at level l (l = 0: the handle's operator, b = the CG
 // residual, z written with partials of (b, z) into rz_part).
 template <int D, typename T>
-void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r, double* rz_part) {
+// x1: the smoothed hierarchy's om D^-1 b at level 0 (written with b by the
+// caller; coarser levels take theirs from the restriction), else unused.
+void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r, double* rz_part, const T* x1) {
   Amg* A = c->amg;
   const int P = prec_of<T>();
   const int L = (int)A->lv.size();
@@ -1723,26 +1761,32 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
   // 2D FP32: the finest level's smoothers read the bfloat16 copy of the values
   const uint16_t* v16 = nullptr;
   if constexpr (D == 2 && std::is_same<T, float>::value) v16 = l == 0 ? A->fine16 : nullptr;
-  with_shape(lpr, u, [&](auto LP, auto UC) {
-    if constexpr (D == 2 && std::is_same<T, float>::value) {
-      if (v16) {
-        amg_presmooth_resid_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
-            <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, dinv, om, r, c->sc);
-        return;
-      }
-    }
-    amg_presmooth_resid_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
-        <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
-  });
   const T* mask = l == 0 ? (const T*)A->mask[P] : nullptr;
   if (A->smoothed) {
-    // restriction P^T r, the coarse cycle, prolongation x = om D^-1 b + P xc
-    // (into r, which is dead by then), post-smoothing from x
-    sa_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(C.qptr, C.qrow, (const T*)C.pvalT[P], C.N, r, bc,
-                                                            c->sc);
-    vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr);
-    sa_prolong_kernel<D, T><<<(N + 255) / 256, 256, 0, st>>>(C.pptr, C.pcol, (const T*)C.pval[P], N, b, dinv, om,
-                                                             xc, r, c->sc);
+    // the smoothed hierarchy: x1 = om D^-1 b came with b (from the CG's
+    // vector kernel at the finest level, from the restriction below it)
+    if (l > 0) x1 = (const T*)A->lv[l - 1].vec[P][4];
+    with_shape(lpr, u, [&](auto LP, auto UC) {
+      if constexpr (D == 2 && std::is_same<T, float>::value) {
+        if (v16) {
+          amg_presmooth_x1_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
+              <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, x1, r, c->sc);
+          return;
+        }
+      }
+      amg_presmooth_x1_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
+          <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, x1, r, c->sc);
+    });
+    // restriction P^T r (with the coarse level's x1 when it has a smoother),
+    // the coarse cycle, prolongation x = x1 + P xc (into r, which is dead by
+    // then), post-smoothing from x
+    const bool coarse_smoothed = l + 1 < L;
+    sa_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(
+        C.qptr, C.qrow, (const T*)C.pvalT[P], C.N, r, bc, (const T*)C.dinv[P], A->omega + l + 1,
+        coarse_smoothed ? (T*)C.vec[P][4] : nullptr, c->sc);
+    vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr, nullptr);
+    sa_prolong_kernel<D, T><<<(N + 255) / 256, 256, 0, st>>>(C.pptr, C.pcol, (const T*)C.pval[P], N, x1, xc, r,
+                                                             c->sc);
     with_shape(lpr, u, [&](auto LP, auto UC) {
       if constexpr (D == 2 && std::is_same<T, float>::value) {
         if (v16) {
@@ -1757,8 +1801,20 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
     });
     return;
   }
+  // the tentative hierarchy: pre-smoothing forms om D^-1 b where it reads it
+  with_shape(lpr, u, [&](auto LP, auto UC) {
+    if constexpr (D == 2 && std::is_same<T, float>::value) {
+      if (v16) {
+        amg_presmooth_resid_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
+            <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, dinv, om, r, c->sc);
+        return;
+      }
+    }
+    amg_presmooth_resid_kernel<D, T, decltype(LP)::value, decltype(UC)::value>
+        <<<g, kSpBS, 0, st>>>(cptr, conns, val, N, b, dinv, om, r, c->sc);
+  });
   amg_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(C.mptr, C.midx, C.N, r, bc, c->sc);
-  vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr);
+  vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr, nullptr);
   with_shape(lpr, u, [&](auto LP, auto UC) {
     if constexpr (D == 2 && std::is_same<T, float>::value) {
       if (v16) {
@@ -1847,9 +1903,12 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   const int nrz = coarse0 ? 1 : (two ? kBS : gs);
   const int npw = two ? kBS : gs;
   const int64_t maxit = o.maxit > 0 ? o.maxit : n;
+  // the smoothed hierarchy's finest pre-smoothed iterate x1 = om D^-1 r,
+  // written with r (the multigrid PCG does not use the q vector)
+  T* x1 = (A->smoothed && !coarse0) ? (T*)(P ? (void*)c->f_q : (void*)c->cg_q) : nullptr;
   amg_setup_scalars_kernel<<<1, 64, 0, c->stream>>>(c->sc, o.rtol, o.atol, maxit);
-  pcg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, n, rr_part);
-  vcycle<D, T>(c, c->stream, 0, r, z, w, rz_raw);
+  pcg_init_kernel<T><<<gv, kBS, 0, c->stream>>>(c->residual, x, r, n, rr_part, dinv0, A->omega, x1);
+  vcycle<D, T>(c, c->stream, 0, r, z, w, rz_raw, x1);
   if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
   pcg_init2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv, c->sc);
   LPMG_HIP(hipGetLastError());
@@ -1860,8 +1919,9 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
             <<<gs, kSpBS, 0, c->stream>>>(c->cptr, c->conns, val0, p, w, N, pw_raw, c->sc);
       });
       if (two) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(pw_raw, gs, pw2, c->sc);
-      pcg_vec1_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, p, w, n, two ? pw2 : pw_raw, npw, rr_part, c->sc);
-      vcycle<D, T>(c, c->stream, 0, r, z, w, rz_raw);
+      pcg_vec1_kernel<T><<<gv, kBS, 0, c->stream>>>(x, r, p, w, n, two ? pw2 : pw_raw, npw, rr_part, dinv0, A->omega,
+                                                    x1, c->sc);
+      vcycle<D, T>(c, c->stream, 0, r, z, w, rz_raw, x1);
       if (two && !coarse0) pcg_reduce_kernel<<<kBS, kBS, 0, c->stream>>>(rz_raw, gs, rz2, c->sc);
       pcg_vec2_kernel<T><<<gv, kBS, 0, c->stream>>>(p, z, n, (two && !coarse0) ? rz2 : rz_raw, nrz, rr_part, gv,
                                                    c->sc);
@@ -1873,13 +1933,15 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   for (const void* q : {(const void*)c->cptr, (const void*)c->conns, (const void*)val0, (const void*)dinv0,
                         (const void*)x, (const void*)r, (const void*)z, (const void*)p, (const void*)w,
                         (const void*)c->partials, (const void*)c->sc, (const void*)A->inv[P],
-                        (const void*)A->mask[P], (const void*)A->fine16, (const void*)A->omega})
+                        (const void*)A->mask[P], (const void*)A->fine16, (const void*)A->omega,
+                        (const void*)x1})
     graph_sig_push(&sig, q);
   for (const auto& L : A->lv)
     for (const void* q : {(const void*)L.cptr, (const void*)L.conns, (const void*)L.mptr, (const void*)L.midx,
                           (const void*)L.agg, (const void*)L.val[P], (const void*)L.dinv[P],
                           (const void*)L.vec[P][0], (const void*)L.vec[P][1], (const void*)L.vec[P][2],
-                          (const void*)L.vec[P][3], (const void*)L.pptr, (const void*)L.pcol, (const void*)L.qrow,
+                          (const void*)L.vec[P][3], (const void*)L.vec[P][4], (const void*)L.pptr,
+                          (const void*)L.pcol, (const void*)L.qrow,
                           (const void*)L.qptr, (const void*)L.pvalT[P], (const void*)L.pval[P]})
       graph_sig_push(&sig, q);
   graph_sig_push(&sig, (int64_t)A->smoothed);
