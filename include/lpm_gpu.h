@@ -336,8 +336,8 @@ typedef struct lpmg_request {
   double opts[8];              /* solver options as lpmg_solve_linear; opts[0] < 0:
                                   the server picks fused Jacobi for requests it
                                   fuses, smoothed-aggregation multigrid (4) for
-                                  larger 2D requests and Jacobi for larger 3D
-                                  ones */
+                                  larger 2D and BCC requests and Jacobi for the
+                                  larger ones on other 3D lattices */
   int max_halvings;            /* step halvings allowed before the request fails */
   int n_out;
   const lpmg_out_field* out;

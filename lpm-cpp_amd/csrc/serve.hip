@@ -446,15 +446,17 @@ int64_t estimate_bytes(const lpmg_request* r) {
   b += nnzb * (12.0 + DD * 16.0 + (mixed ? DD * 4.0 : 0.0));   // conns, BSR values (tangent + symmetrised)
   b += N * 48.0;                                                // setup scratch
   for (int g = 0; g < r->n_groups; ++g) b += 4.0 * std::max(1, (int)r->group_size[g]) + 256;
-  if (r->opts[0] == 4 || (r->opts[0] < 0 && dim == 2)) {
+  if (r->opts[0] == 4 || (r->opts[0] < 0 && (dim == 2 || r->lattice == LPMG_BCC3D))) {
     // smoothed-aggregation multigrid: per fine row about Pr prolongator
     // blocks (values, FP64 and FP32, in row and column order, and five index
     // arrays), 3 Pr blocks of A P (FP64 values), a 2-byte slot per term of A P
     // (A's blocks of the row times the P blocks of their columns) and of
     // P^T (A P) (P's blocks of the row times the row's A P blocks), and a
     // 4-byte offset per source of either; the coarser levels add up to 20%.
-    // Counted on the CT plates: Pr 8.0, A P 21.3, 244 and 172 terms per row.
-    const double pr = dim == 2 ? 9.0 : 20.0, yr = 3.0 * pr, k = nnzb / N;
+    // Counted on the CT plates: Pr 8.0, A P 21.3, 244 and 172 terms per row;
+    // the 3D figure holds the 1.0M-particle BCC polycrystal's hierarchy (5.5 GB
+    // measured) with a 1.8x margin.
+    const double pr = dim == 2 ? 9.0 : 12.0, yr = 3.0 * pr, k = nnzb / N;
     b += 1.2 * N * (2.0 * (k * pr + pr * yr) + 4.0 * (k + pr) + yr * DD * 8.0 + pr * (DD * 24.0 + 24.0));
   }
   if (r->nonlocal_ratio > 0 && r->n_params > 0) {
@@ -520,13 +522,15 @@ int run_request(lpmg_server_s* S, Req* q, int64_t* bytes) {
   // it fuses; for the larger ones smoothed-aggregation multigrid in 2D
   // (Jacobi needs thousands of iterations per solve there; the tentative
   // aggregation measured 3.5-4x faster than Jacobi at 1e5-2.5e5 particles, the
-  // smoothed one 27-29% faster again on the CT stream's 205k and 109k requests)
-  // and Jacobi in 3D (a V-cycle costs about two SpMVs, more than the
-  // iterations it saves on the 3D lattices measured).
+  // smoothed one about 2x faster again on the CT stream's large requests) and
+  // on the BCC lattice (mixed precision: 1.0M-particle polycrystal 2.5 s
+  // against 5.5 s, 108k-particle J2 bar 0.92 against 1.83 s), Jacobi on the
+  // other 3D lattices (SC 40^3 and 60^3 need so few Jacobi iterations that
+  // the V-cycle does not pay: 0.34 against 0.15 s, 0.70 against 0.40 s).
   const bool fusable = S->batching && (S->fuse_max_rows < 0 || (int64_t)c->N <= S->fuse_max_rows);
   double opts[8];
   std::memcpy(opts, r.opts, sizeof(opts));
-  if (opts[0] < 0) opts[0] = (fusable || c->dim == 3) ? 1 : 4;
+  if (opts[0] < 0) opts[0] = (!fusable && (c->dim == 2 || r.lattice == LPMG_BCC3D)) ? 4 : 1;
   if (fusable && opts[0] <= 1) {
     c->cg_exec = batcher_exec;
     c->cg_exec_arg = &S->batcher;

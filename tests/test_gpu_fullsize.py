@@ -9,6 +9,9 @@ next to others giving the same bits as alone.
   polycrystal: BCC 80^3 cells, 200 Voronoi grains (about 1.0M particles),
               cubic C11/C12/C44 = 230/135/117 GPa, 10 tension steps of 1e-4,
               served together with CT requests
+
+Both run the mixed path with Jacobi and with smoothed-aggregation multigrid
+(the server's choice for BCC requests above its fusion limit).
 """
 import numpy as np
 import pytest
@@ -22,6 +25,8 @@ from lpmg.serve import Request, Server
 
 FP64 = dict(precond=1, rtol=1e-12, atol=0.0, precision=0)
 MIXED = dict(precond=1, rtol=1e-4, atol=0.0, precision=1)
+# the server's choice for these BCC requests: smoothed-aggregation multigrid
+MIXED_SA = dict(precond=4, rtol=1e-4, atol=0.0, precision=1)
 NEWTON_TOL = 1e-11
 
 
@@ -35,7 +40,7 @@ def test_bcc_j2_bar_mixed_matches_fp64_every_step():
     s, gids = P.build(lpmg.System, prob)
     s.forward()
     runs = {}
-    for name, o in (("fp64", FP64), ("mixed", MIXED)):
+    for name, o in (("fp64", FP64), ("mixed", MIXED), ("mixed_sa", MIXED_SA)):
         s.restart()
         us, flags = [], []
         for step in prob.steps:
@@ -45,10 +50,11 @@ def test_bcc_j2_bar_mixed_matches_fp64_every_step():
             us.append(s.get("XYZ") - s.get("XYZ_INITIAL"))
             flags.append(s.get("STATE_VAR")[0::7] > 0)
         runs[name] = (us, flags)
-    for k, (a, b) in enumerate(zip(runs["mixed"][0], runs["fp64"][0])):
-        assert _rel(a, b) <= 1e-8, k
-    for fa, fb in zip(runs["mixed"][1], runs["fp64"][1]):
-        np.testing.assert_array_equal(fa, fb)
+    for name in ("mixed", "mixed_sa"):
+        for k, (a, b) in enumerate(zip(runs[name][0], runs["fp64"][0])):
+            assert _rel(a, b) <= 1e-8, (name, k)
+        for fa, fb in zip(runs[name][1], runs["fp64"][1]):
+            np.testing.assert_array_equal(fa, fb)
     assert runs["fp64"][1][-1].mean() > 0.9  # the bar is yielding throughout
 
 
@@ -68,14 +74,20 @@ def poly():
 
 def test_polycrystal_mixed_matches_fp64(poly):
     p, alone = poly
-    for name, o in (("fp64", FP64), ("mixed", MIXED)):
+    # with a fusion limit (the bench's 10k particles) the server picks
+    # smoothed aggregation for this BCC request
+    auto = dict(MIXED, precond=-1)
+    for name, o in (("fp64", FP64), ("mixed", MIXED), ("mixed_auto", auto)):
         r = _req(p, o)
-        with Server(max_active=1) as srv:
+        with Server(max_active=1, batching=10000 if name == "mixed_auto" else True) as srv:
             status, st = srv.wait(srv.submit(r))
         assert status == 0, st.get("error")
         assert st["bytes"] <= st["bytes_est"]
         alone[name] = r.out["XYZ"].reshape(-1, 3) - p.xyz
+        if name == "mixed_auto":
+            assert st["cg"] < 2000, st["cg"]  # Jacobi takes 16,856 iterations
     assert _rel(alone["mixed"], alone["fp64"]) <= 1e-8
+    assert _rel(alone["mixed_auto"], alone["fp64"]) <= 1e-8
 
 
 def test_polycrystal_served_with_ct_requests(poly):
