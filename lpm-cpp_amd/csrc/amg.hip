@@ -1165,167 +1165,168 @@ struct SaSym {
   int pmax = 0, ymax = 0, cmax = 0;  // longest P, Y and coarse row
 };
 
-// Rows [0, nrows) each emit (col, a, b) triples through gen(row, emit); each
-// row's triples are grouped by col (ascending), keeping their emission order
-// within a col.  Out: per-row entry counts (rcnt), entry columns (ecol), entry
-// ends in the pair lists (eend) and the pairs (pa, pb) -- all in row order, so
-// the result does not depend on how the rows are split over host threads.
-template <class Gen>
-void group_rows(int nrows, int ncols, Gen gen, std::vector<int>* rcnt, std::vector<int>* ecol,
-                std::vector<int64_t>* eend, std::vector<int>* pa, std::vector<int>* pb) {
-  struct Part {
-    std::vector<int> rcnt, ecol, pa, pb;
-    std::vector<int64_t> eend;
-  };
+// Runs f(t, r0, r1) over the row ranges of nt host threads (thread 0 is the
+// caller).
+template <class F>
+void host_rows(int nrows, F&& f) {
   const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
   const int per = (nrows + nt - 1) / nt;
-  std::vector<Part> parts(nt);
-  auto work = [&](int t) {
-    Part& P = parts[t];
-    const int r0 = t * per, r1 = std::min(nrows, r0 + per);
-    std::vector<int> stamp(ncols, -1), cnt(ncols, 0), touched;
-    std::vector<int64_t> off(ncols, 0);
-    std::vector<std::array<int, 3>> tmp;
-    for (int i = r0; i < r1; ++i) {
-      touched.clear();
-      tmp.clear();
-      gen(i, [&](int col, int x, int y) {
-        if (stamp[col] != i) {
-          stamp[col] = i;
-          cnt[col] = 0;
-          touched.push_back(col);
-        }
-        ++cnt[col];
-        tmp.push_back({col, x, y});
-      });
-      std::sort(touched.begin(), touched.end());
-      int64_t base = (int64_t)P.pa.size();
-      for (int col : touched) {
-        off[col] = base;
-        base += cnt[col];
-        P.ecol.push_back(col);
-        P.eend.push_back(base);
-      }
-      P.pa.resize(base);
-      P.pb.resize(base);
-      for (const auto& q : tmp) {
-        const int64_t pos = off[q[0]]++;
-        P.pa[pos] = q[1];
-        P.pb[pos] = q[2];
-      }
-      P.rcnt.push_back((int)touched.size());
-    }
-  };
   std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
-  work(0);
-  for (auto& x : th) x.join();
-  rcnt->clear();
-  ecol->clear();
-  eend->clear();
-  pa->clear();
-  pb->clear();
-  for (const Part& P : parts) {
-    const int64_t shift = (int64_t)pa->size();
-    rcnt->insert(rcnt->end(), P.rcnt.begin(), P.rcnt.end());
-    ecol->insert(ecol->end(), P.ecol.begin(), P.ecol.end());
-    for (int64_t e : P.eend) eend->push_back(e + shift);
-    pa->insert(pa->end(), P.pa.begin(), P.pa.end());
-    pb->insert(pb->end(), P.pb.begin(), P.pb.end());
+  for (int t = 1; t < nt; ++t) {
+    const int r0 = std::min(nrows, t * per), r1 = std::min(nrows, r0 + per);
+    if (r0 < r1) th.emplace_back([&f, t, r0, r1] { f(t, r0, r1); });
   }
+  f(0, 0, std::min(nrows, per));
+  for (auto& x : th) x.join();
 }
 
 // Rows [0, nrows) each emit columns through gen(row, emit).  Out, in row
 // order: each row's distinct columns ascending (rptr, cols), and for every
 // emission, in emission order, the position of its column among them (slot,
-// rows starting at sptr).  Split over host threads; the result does not
-// depend on the split.  Returns the longest row, or -1 past 65,535 columns.
+// rows starting at sptr).  Two passes over host threads -- counts, then the
+// rows written in place at their offsets -- so nothing is copied or grown;
+// the result does not depend on the split.  Returns the longest row, or -1
+// past 65,535 columns.
 template <class Gen>
 int slot_rows(int nrows, int ncols, Gen gen, std::vector<int>* rptr, std::vector<int>* cols,
               std::vector<uint16_t>* slot, std::vector<int64_t>* sptr) {
-  struct Part {
-    std::vector<int> rcnt, cols;
-    std::vector<uint16_t> slot;
-    std::vector<int64_t> scnt;
-    int longest = 0;
-  };
-  const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
-  const int per = (nrows + nt - 1) / nt;
-  std::vector<Part> parts(nt);
-  auto work = [&](int t) {
-    Part& P = parts[t];
-    const int r0 = t * per, r1 = std::min(nrows, r0 + per);
-    std::vector<int> stamp(ncols, -1), pos(ncols, 0), touched, tmp;
+  std::vector<int> ucnt(nrows);
+  std::vector<int64_t> ecnt(nrows);
+  std::vector<int> longest(16, 0);
+  host_rows(nrows, [&](int t, int r0, int r1) {
+    std::vector<int> stamp(ncols, -1);
     for (int i = r0; i < r1; ++i) {
-      touched.clear();
-      tmp.clear();
+      int u = 0;
+      int64_t e = 0;
       gen(i, [&](int col) {
         if (stamp[col] != i) {
           stamp[col] = i;
-          touched.push_back(col);
+          ++u;
         }
-        tmp.push_back(col);
+        ++e;
       });
-      std::sort(touched.begin(), touched.end());
-      for (size_t k = 0; k < touched.size(); ++k) pos[touched[k]] = (int)k;
-      P.longest = std::max(P.longest, (int)touched.size());
-      P.rcnt.push_back((int)touched.size());
-      P.cols.insert(P.cols.end(), touched.begin(), touched.end());
-      for (int col : tmp) P.slot.push_back((uint16_t)std::min(pos[col], 65535));
-      P.scnt.push_back((int64_t)tmp.size());
+      ucnt[i] = u;
+      ecnt[i] = e;
+      longest[t] = std::max(longest[t], u);
     }
-  };
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
-  work(0);
-  for (auto& x : th) x.join();
-  rptr->assign(1, 0);
-  sptr->assign(1, 0);
-  cols->clear();
-  slot->clear();
-  int longest = 0;
-  for (const Part& P : parts) {
-    for (int n : P.rcnt) rptr->push_back(rptr->back() + n);
-    for (int64_t n : P.scnt) sptr->push_back(sptr->back() + n);
-    cols->insert(cols->end(), P.cols.begin(), P.cols.end());
-    slot->insert(slot->end(), P.slot.begin(), P.slot.end());
-    longest = std::max(longest, P.longest);
+  });
+  rptr->assign(nrows + 1, 0);
+  sptr->assign(nrows + 1, 0);
+  for (int i = 0; i < nrows; ++i) {
+    (*rptr)[i + 1] = (*rptr)[i] + ucnt[i];
+    (*sptr)[i + 1] = (*sptr)[i] + ecnt[i];
   }
-  return longest > 65535 ? -1 : longest;
+  const int lmax = *std::max_element(longest.begin(), longest.end());
+  if (lmax > 65535) return -1;
+  cols->resize((*rptr)[nrows]);
+  slot->resize((*sptr)[nrows]);
+  host_rows(nrows, [&](int, int r0, int r1) {
+    std::vector<int> stamp(ncols, -1), pos(ncols, 0), em;
+    for (int i = r0; i < r1; ++i) {
+      int* rc = cols->data() + (*rptr)[i];
+      int u = 0;
+      em.clear();
+      gen(i, [&](int col) {
+        em.push_back(col);
+        if (stamp[col] != i) {
+          stamp[col] = i;
+          rc[u++] = col;
+        }
+      });
+      std::sort(rc, rc + u);
+      for (int k = 0; k < u; ++k) pos[rc[k]] = k;
+      uint16_t* rs = slot->data() + (*sptr)[i];
+      for (size_t e = 0; e < em.size(); ++e) rs[e] = (uint16_t)pos[em[e]];
+    }
+  });
+  return lmax;
 }
 
 int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<int>& conns,
                 const std::vector<int>& agg, SaSym* s) {
-  std::vector<int> rcnt, unused;
-  std::vector<int64_t> eend;
-  // P: a row's blocks grouped by the aggregate of their column
-  group_rows(N, Nc, [&](int i, auto emit) {
-    for (int e = cptr[i]; e < cptr[i + 1]; ++e) emit(agg[conns[e]], e, 0);
-  }, &rcnt, &s->pcol, &eend, &s->atidx, &unused);
-  const int nP = (int)s->pcol.size();
+  // P: a row's blocks are the distinct aggregates of its columns (ascending);
+  // each lists the row's level blocks of that aggregate in order (atidx, the
+  // row's entries at the row's own offset: every level block belongs to one
+  // P block).  Two passes over host threads: counts, then the rows in place.
+  std::vector<int> ucnt(N);
+  host_rows(N, [&](int, int r0, int r1) {
+    std::vector<int> stamp(Nc, -1);
+    for (int i = r0; i < r1; ++i) {
+      int u = 0;
+      for (int e = cptr[i]; e < cptr[i + 1]; ++e) {
+        const int a = agg[conns[e]];
+        if (stamp[a] != i) {
+          stamp[a] = i;
+          ++u;
+        }
+      }
+      ucnt[i] = u;
+    }
+  });
   s->pptr.assign(N + 1, 0);
+  for (int i = 0; i < N; ++i) s->pptr[i + 1] = s->pptr[i] + ucnt[i];
+  const int nP = s->pptr[N];
+  s->pcol.resize(nP);
   s->prow.resize(nP);
-  for (int i = 0, p = 0; i < N; ++i) {
-    s->pptr[i + 1] = s->pptr[i] + rcnt[i];
-    for (; p < s->pptr[i + 1]; ++p) s->prow[p] = i;
-  }
   s->atptr.assign(nP + 1, 0);
-  for (int p = 0; p < nP; ++p) s->atptr[p + 1] = (int)eend[p];
-  // P's blocks by aggregate (in row order)
+  s->atidx.resize(cptr[N]);
+  host_rows(N, [&](int, int r0, int r1) {
+    std::vector<int> stamp(Nc, -1), pos(Nc, 0), fill;
+    for (int i = r0; i < r1; ++i) {
+      int* pc = s->pcol.data() + s->pptr[i];
+      int u = 0;
+      for (int e = cptr[i]; e < cptr[i + 1]; ++e) {
+        const int a = agg[conns[e]];
+        if (stamp[a] != i) {
+          stamp[a] = i;
+          pc[u++] = a;
+        }
+      }
+      std::sort(pc, pc + u);
+      fill.assign(u + 1, 0);
+      for (int k = 0; k < u; ++k) pos[pc[k]] = k;
+      for (int e = cptr[i]; e < cptr[i + 1]; ++e) ++fill[pos[agg[conns[e]]] + 1];
+      for (int k = 0; k < u; ++k) fill[k + 1] += fill[k];
+      for (int k = 0; k < u; ++k) {
+        s->prow[s->pptr[i] + k] = i;
+        s->atptr[s->pptr[i] + k + 1] = cptr[i] + fill[k + 1];
+      }
+      for (int e = cptr[i]; e < cptr[i + 1]; ++e) s->atidx[cptr[i] + fill[pos[agg[conns[e]]]]++] = e;
+    }
+  });
+  // P's blocks by aggregate, in row order (a stable counting sort: per-thread
+  // counts over row ranges, then each thread's blocks after the earlier
+  // threads' in every column)
+  const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+  std::vector<std::vector<int>> tc(nt, std::vector<int>(Nc, 0));
+  host_rows(nP, [&](int t, int k0, int k1) {
+    for (int p = k0; p < k1; ++p) ++tc[t][s->pcol[p]];
+  });
   s->qptr.assign(Nc + 1, 0);
-  for (int p = 0; p < nP; ++p) s->qptr[s->pcol[p] + 1] += 1;
-  for (int a = 0; a < Nc; ++a) s->qptr[a + 1] += s->qptr[a];
-  s->qidx.assign(nP, 0);
   {
-    std::vector<int> fill(s->qptr.begin(), s->qptr.end() - 1);
-    for (int p = 0; p < nP; ++p) s->qidx[fill[s->pcol[p]]++] = p;
+    int run = 0;
+    for (int a = 0; a < Nc; ++a) {
+      s->qptr[a] = run;
+      for (int t = 0; t < nt; ++t) {
+        const int n = tc[t][a];
+        tc[t][a] = run;
+        run += n;
+      }
+    }
+    s->qptr[Nc] = run;
   }
+  s->qidx.assign(nP, 0);
+  host_rows(nP, [&](int t, int k0, int k1) {
+    for (int p = k0; p < k1; ++p) s->qidx[tc[t][s->pcol[p]]++] = p;
+  });
   s->qrow.assign(nP, 0);
   s->qpos.assign(nP, 0);
-  for (int k = 0; k < nP; ++k) {
-    s->qrow[k] = s->prow[s->qidx[k]];
-    s->qpos[s->qidx[k]] = k;
-  }
+  host_rows(nP, [&](int, int k0, int k1) {
+    for (int k = k0; k < k1; ++k) {
+      s->qrow[k] = s->prow[s->qidx[k]];
+      s->qpos[s->qidx[k]] = k;
+    }
+  });
   for (int i = 0; i < N; ++i) s->pmax = std::max(s->pmax, s->pptr[i + 1] - s->pptr[i]);
   // Y = A P: row i's blocks by column b; a slot per (A block e, P block q) in
   // (e, q) order
@@ -1348,17 +1349,21 @@ int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<i
     return -1;
   }
   s->yso.assign(cptr[N], 0);
-  for (int i = 0; i < N; ++i)
-    for (int e = cptr[i], off = 0; e < cptr[i + 1]; ++e) {
-      s->yso[e] = off;
-      off += s->pptr[conns[e] + 1] - s->pptr[conns[e]];
-    }
+  host_rows(N, [&](int, int r0, int r1) {
+    for (int i = r0; i < r1; ++i)
+      for (int e = cptr[i], off = 0; e < cptr[i + 1]; ++e) {
+        s->yso[e] = off;
+        off += s->pptr[conns[e] + 1] - s->pptr[conns[e]];
+      }
+  });
   s->cso.assign(nP, 0);
-  for (int a = 0; a < Nc; ++a)
-    for (int k = s->qptr[a], off = 0; k < s->qptr[a + 1]; ++k) {
-      s->cso[k] = off;
-      off += s->yptr[s->qrow[k] + 1] - s->yptr[s->qrow[k]];
-    }
+  host_rows(Nc, [&](int, int r0, int r1) {
+    for (int a = r0; a < r1; ++a)
+      for (int k = s->qptr[a], off = 0; k < s->qptr[a + 1]; ++k) {
+        s->cso[k] = off;
+        off += s->yptr[s->qrow[k] + 1] - s->yptr[s->qrow[k]];
+      }
+  });
   s->dpos.assign(Nc, -1);
   for (int a = 0; a < Nc; ++a) {
     for (int k = s->ccptr[a]; k < s->ccptr[a + 1]; ++k)
