@@ -85,8 +85,9 @@ constexpr double kCoarseScale = 1.5;
 // `wave` is the wavefront's index over the level's rows (RPW = 64 / LPR rows
 // each); a row's sums do not depend on which wavefront forms them.
 // Block e's D*D values as T, from values stored as T or -- the smoothers of
-// the finest level of a 2D FP32 V-cycle -- as bfloat16 bit patterns (round to
-// nearest even of the FP32 stiffness), a 2x2 block in one 8-byte word.  The
+// the finest level of an FP32 V-cycle -- as bfloat16 bit patterns (round to
+// nearest even of the FP32 stiffness): a 2x2 block in one 8-byte word, a 3x3
+// block in 20 bytes (nine values and a zero, read as five 4-byte words).  The
 // preconditioner only has to be symmetric and close to K: the PCG's own
 // operator stays FP32, and the bf16 smoothers take the same CG iterations on
 // CT specimens (CPU experiment: 23 / 56 / 80 PCG iterations to 1e-4 / 1e-8 /
@@ -94,25 +95,46 @@ constexpr double kCoarseScale = 1.5;
 // 20 bytes per block: 153 -> 148 us per FP32 multigrid iteration on the
 // 205,645-particle specimen (the smoothers' dependent gathers, not their
 // bytes, bound them).
+constexpr int bf16_stride(int D) { return D == 2 ? 4 : 10; }  // bfloat16 values per stored block
+
 template <int D, typename T, typename TV>
 __device__ __forceinline__ void load_block(const TV* __restrict__ val, int e, T v[D * D]) {
   if constexpr (std::is_same<TV, uint16_t>::value) {
-    static_assert(D == 2 && std::is_same<T, float>::value, "bfloat16 blocks are 2x2 FP32 only");
-    const uint2 w = reinterpret_cast<const uint2*>(val)[e];
-    v[0] = __uint_as_float(w.x << 16);
-    v[1] = __uint_as_float(w.x & 0xffff0000u);
-    v[2] = __uint_as_float(w.y << 16);
-    v[3] = __uint_as_float(w.y & 0xffff0000u);
+    static_assert(std::is_same<T, float>::value, "bfloat16 blocks expand to FP32");
+    if constexpr (D == 2) {
+      const uint2 w = reinterpret_cast<const uint2*>(val)[e];
+      v[0] = __uint_as_float(w.x << 16);
+      v[1] = __uint_as_float(w.x & 0xffff0000u);
+      v[2] = __uint_as_float(w.y << 16);
+      v[3] = __uint_as_float(w.y & 0xffff0000u);
+    } else {
+      const uint32_t* w = reinterpret_cast<const uint32_t*>(val) + (size_t)e * 5;
+      uint32_t x[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) x[k] = w[k];
+#pragma unroll
+      for (int k = 0; k < D * D; ++k) v[k] = __uint_as_float((k & 1) ? (x[k >> 1] & 0xffff0000u) : (x[k >> 1] << 16));
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < D * D; ++k) v[k] = val[(size_t)e * D * D + k];
   }
 }
 
-__global__ void bf16_round_kernel(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t n) {
+// b = bfloat16 of the FP32 blocks a (nb blocks of D*D values), bf16_stride(D)
+// values per stored block (the padding zero)
+template <int D>
+__global__ void bf16_round_kernel(const float* __restrict__ a, uint16_t* __restrict__ b, int64_t nb) {
+  constexpr int S = bf16_stride(D);
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  uint32_t u = __float_as_uint(a[t]);
+  if (t >= nb * S) return;
+  const int64_t e = t / S;
+  const int k = (int)(t - e * S);
+  if (k >= D * D) {
+    b[t] = 0;
+    return;
+  }
+  uint32_t u = __float_as_uint(a[e * D * D + k]);
   u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest, ties to even (finite values)
   b[t] = (uint16_t)(u >> 16);
 }
@@ -1111,7 +1133,7 @@ struct Amg {
   void* inv[2] = {nullptr, nullptr};
   void* mask[2] = {nullptr, nullptr};  // finest-level free-DOF mask per precision
   int64_t version[2] = {-1, -1};
-  uint16_t* fine16 = nullptr;  // 2D FP32: the finest level's values as bfloat16, for the smoothers
+  uint16_t* fine16 = nullptr;  // FP32: the finest level's values as bfloat16, for the smoothers
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   std::vector<uintptr_t> graph_sig[2];  // pointers and shapes captured into graph[P]
   hipGraphExec_t graph_tail[2] = {nullptr, nullptr};  // the short chunk that ends a solve
@@ -1590,8 +1612,8 @@ int amg_alloc_prec(Context* c) {
     L.dinv[P] = di;
     for (int k = 0; k < 5; ++k) L.vec[P][k] = w[k];
   }
-  if (P == 1 && D == 2 && !A->fine16)
-    if (amg_alloc(c, &A->fine16, (size_t)c->nnzb * 4)) return -3;
+  if (P == 1 && !A->fine16)
+    if (amg_alloc(c, &A->fine16, (size_t)c->nnzb * bf16_stride(D))) return -3;
   if (!A->inv[P]) {
     T *iv = nullptr, *mk = nullptr;
     if (amg_alloc(c, &iv, (size_t)A->nc * A->nc) || amg_alloc(c, &mk, (size_t)c->N * D)) return -3;
@@ -1763,16 +1785,16 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
   T* bc = (T*)C.vec[P][0];
   T* xc = (l + 1 == L) ? (T*)C.vec[P][2] : (T*)C.vec[P][3];
   const int g = rows_grid(N, lpr);
-  // 2D FP32: the finest level's smoothers read the bfloat16 copy of the values
+  // FP32: the finest level's smoothers read the bfloat16 copy of the values
   const uint16_t* v16 = nullptr;
-  if constexpr (D == 2 && std::is_same<T, float>::value) v16 = l == 0 ? A->fine16 : nullptr;
+  if constexpr (std::is_same<T, float>::value) v16 = l == 0 ? A->fine16 : nullptr;
   const T* mask = l == 0 ? (const T*)A->mask[P] : nullptr;
   if (A->smoothed) {
     // the smoothed hierarchy: x1 = om D^-1 b came with b (from the CG's
     // vector kernel at the finest level, from the restriction below it)
     if (l > 0) x1 = (const T*)A->lv[l - 1].vec[P][4];
     with_shape(lpr, u, [&](auto LP, auto UC) {
-      if constexpr (D == 2 && std::is_same<T, float>::value) {
+      if constexpr (std::is_same<T, float>::value) {
         if (v16) {
           amg_presmooth_x1_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
               <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, x1, r, c->sc);
@@ -1793,7 +1815,7 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
     sa_prolong_kernel<D, T><<<(N + 255) / 256, 256, 0, st>>>(C.pptr, C.pcol, (const T*)C.pval[P], N, x1, xc, r,
                                                              c->sc);
     with_shape(lpr, u, [&](auto LP, auto UC) {
-      if constexpr (D == 2 && std::is_same<T, float>::value) {
+      if constexpr (std::is_same<T, float>::value) {
         if (v16) {
           amg_postsmooth_x_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
               <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, r, dinv, om, mask, out, l == 0 ? rz_part : nullptr,
@@ -1808,7 +1830,7 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
   }
   // the tentative hierarchy: pre-smoothing forms om D^-1 b where it reads it
   with_shape(lpr, u, [&](auto LP, auto UC) {
-    if constexpr (D == 2 && std::is_same<T, float>::value) {
+    if constexpr (std::is_same<T, float>::value) {
       if (v16) {
         amg_presmooth_resid_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
             <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, dinv, om, r, c->sc);
@@ -1821,7 +1843,7 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
   amg_restrict_kernel<D, T><<<(C.N + 3) / 4, 256, 0, st>>>(C.mptr, C.midx, C.N, r, bc, c->sc);
   vcycle<D, T>(c, st, l + 1, bc, xc, nullptr, nullptr, nullptr);
   with_shape(lpr, u, [&](auto LP, auto UC) {
-    if constexpr (D == 2 && std::is_same<T, float>::value) {
+    if constexpr (std::is_same<T, float>::value) {
       if (v16) {
         amg_postsmooth_kernel<D, T, decltype(LP)::value, decltype(UC)::value, uint16_t>
             <<<g, kSpBS, 0, st>>>(cptr, conns, v16, N, b, C.agg, xc, dinv, om, mask, out,
@@ -1882,9 +1904,9 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
                                                                  c->floating, N, dinv0, (T*)A->mask[P]);
   if (A->version[P] != c->asm_version) {
     if (int rc = amg_values<D, T>(c)) return rc;
-    if (P == 1 && D == 2) {
-      const int64_t nv = (int64_t)c->nnzb * 4;
-      bf16_round_kernel<<<(unsigned)((nv + 255) / 256), 256, 0, c->stream>>>(c->Kval32, A->fine16, nv);
+    if (P == 1) {
+      const int64_t nv = (int64_t)c->nnzb * bf16_stride(D);
+      bf16_round_kernel<D><<<(unsigned)((nv + 255) / 256), 256, 0, c->stream>>>(c->Kval32, A->fine16, c->nnzb);
       LPMG_HIP(hipGetLastError());
     }
     A->version[P] = c->asm_version;
