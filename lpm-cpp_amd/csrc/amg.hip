@@ -748,6 +748,171 @@ __global__ void sa_rows_kernel(int nrows, const int* __restrict__ sptr, const in
   for (int u = t; u < no * DD; u += LPR) out[(size_t)o0 * DD + u] = my[u];
 }
 
+// ---- smoothed aggregation, symbolic lists on the device ----
+//
+// The slot lists of Y = A P and of P^T Y (and their row patterns) as the host
+// builds them -- each output row's distinct columns ascending, and for every
+// term, in source then run order, its column's position -- one wavefront (a
+// 64-thread workgroup) per output row: the row's sources' run offsets by a
+// wavefront scan, its columns deduplicated in an LDS hash table (bounded
+// probing), sorted by a bitonic sort in LDS, each term's slot by a binary
+// search.  MODE 1: rows of Y (sources: A's blocks of row i, runs: the P rows
+// of their columns, pcol); MODE 2: coarse rows (sources: P's blocks of column
+// a in column order, runs: the Y rows of their rows, ycol).  A row with more
+// sources or distinct columns than the LDS arrays hold sets *overflow, and the
+// level's lists are then built on the host.  On the 1.0M-particle BCC
+// polycrystal the host took 0.39 s for these lists.
+constexpr int kSymMaxSrc = 1024;            // sources per output row
+constexpr int kSymHash = 2048;              // hash slots per output row
+constexpr int kSymMaxU = kSymHash / 2;      // distinct columns per output row
+
+__device__ __forceinline__ int sym_source(const int* off, int ns, int t) {  // k with off[k] <= t < off[k+1]
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// FILL 0: ucnt[row] = distinct columns, ecnt[row] = terms, soff[s0 + k] =
+// source k's first term within the row.  FILL 1: ocol[optr[row] ...] = the
+// distinct columns ascending, oslot[eptr[row] + t] = term t's position, and
+// (MODE 2) odiag[row] = the position of column `row` (-1 if absent).
+template <int FILL>
+__global__ void __launch_bounds__(64) sa_sym_kernel(int nrows, const int* __restrict__ sptr,
+                                                    const int* __restrict__ sidx, const int* __restrict__ rptr,
+                                                    const int* __restrict__ rcol, int* __restrict__ ucnt,
+                                                    int64_t* __restrict__ ecnt, int* __restrict__ soff,
+                                                    const int* __restrict__ optr, const int64_t* __restrict__ eptr,
+                                                    int* __restrict__ ocol, uint16_t* __restrict__ oslot,
+                                                    int* __restrict__ odiag, int* __restrict__ overflow) {
+  __shared__ int off[kSymMaxSrc + 1];
+  __shared__ int tab[kSymHash];
+  __shared__ int U[kSymMaxU];
+  __shared__ int nu, bad;
+  const int row = blockIdx.x, lane = threadIdx.x;
+  if (row >= nrows) return;  // the whole workgroup (one wavefront)
+  const int s0 = sptr[row], ns = sptr[row + 1] - s0;
+  if (ns > kSymMaxSrc) {
+    if (lane == 0) atomicOr(overflow, 1);
+    return;
+  }
+  // runs' exclusive offsets: off[k] = terms of sources before k
+  int carry = 0;
+  for (int b = 0; b < ns; b += 64) {
+    const int k = b + lane;
+    int m = 0;
+    if (k < ns) {
+      const int j = sidx[s0 + k];
+      m = rptr[j + 1] - rptr[j];
+    }
+    const int incl = wave_incl_scan(m) + carry;
+    if (k < ns) off[k + 1] = incl;
+    carry = __shfl(incl, 63, 64);
+  }
+  if (lane == 0) {
+    off[0] = 0;
+    nu = 0;
+    bad = 0;
+  }
+  for (int h = lane; h < kSymHash; h += 64) tab[h] = -1;
+  __syncthreads();
+  const int E = ns > 0 ? off[ns] : 0;
+  if (FILL == 0)
+    for (int k = lane; k < ns; k += 64) soff[s0 + k] = off[k];
+  // distinct columns: insert every term's column (at most kSymHash probes)
+  for (int t = lane; t < E; t += 64) {
+    const int k = sym_source(off, ns, t);
+    const int col = rcol[rptr[sidx[s0 + k]] + (t - off[k])];
+    int h = (int)(((uint32_t)col * 2654435761u) >> 21) & (kSymHash - 1);
+    bool placed = false;
+    for (int probe = 0; probe < kSymHash; ++probe) {
+      const int prev = atomicCAS(&tab[h], -1, col);
+      if (prev == -1) {
+        atomicAdd(&nu, 1);
+        placed = true;
+        break;
+      }
+      if (prev == col) {
+        placed = true;
+        break;
+      }
+      h = (h + 1) & (kSymHash - 1);
+    }
+    if (!placed) atomicOr(&bad, 1);
+  }
+  __syncthreads();
+  const int u = nu;
+  if (bad || u > kSymMaxU) {
+    if (lane == 0) atomicOr(overflow, 1);
+    return;
+  }
+  if (FILL == 0) {
+    if (lane == 0) {
+      ucnt[row] = u;
+      ecnt[row] = E;
+    }
+    return;
+  }
+  // the distinct columns, sorted (bitonic, padded to a power of two)
+  int P = 1;
+  while (P < u) P <<= 1;
+  __syncthreads();
+  if (lane == 0) nu = 0;
+  __syncthreads();
+  for (int h = lane; h < kSymHash; h += 64) {
+    const int v = tab[h];
+    if (v >= 0) U[atomicAdd(&nu, 1)] = v;
+  }
+  __syncthreads();
+  for (int i = u + lane; i < P; i += 64) U[i] = 0x7fffffff;
+  __syncthreads();
+  for (int kk = 2; kk <= P; kk <<= 1)
+    for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+      for (int i = lane; i < P; i += 64) {
+        const int l = i ^ jj;
+        if (l > i) {
+          const int x = U[i], y = U[l];
+          const bool up = (i & kk) == 0;
+          if ((x > y) == up) {
+            U[i] = y;
+            U[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  int* oc = ocol + optr[row];
+  for (int k = lane; k < u; k += 64) oc[k] = U[k];
+  if (odiag && lane == 0) {
+    int d = -1, lo = 0, hi = u - 1;
+    while (lo <= hi) {
+      const int mid = (lo + hi) >> 1;
+      if (U[mid] == row) {
+        d = mid;
+        break;
+      }
+      if (U[mid] < row) lo = mid + 1;
+      else hi = mid - 1;
+    }
+    odiag[row] = d;
+  }
+  uint16_t* os = oslot + eptr[row];
+  for (int t = lane; t < E; t += 64) {
+    const int k = sym_source(off, ns, t);
+    const int col = rcol[rptr[sidx[s0 + k]] + (t - off[k])];
+    int lo = 0, hi = u - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (U[mid] < col) lo = mid + 1;
+      else hi = mid;
+    }
+    os[t] = (uint16_t)lo;
+  }
+}
+
 // Launch sa_rows_kernel: LPR from the longest run (16, 32 or 64 lanes), rows
 // per workgroup so that their accumulators take at most 32 KB of LDS.
 template <int D, int MODE>
@@ -1185,6 +1350,8 @@ struct SaSym {
   std::vector<int> yso, cso;        // each source's first slot within its row's list
   std::vector<int> ccptr, cconns, dpos;
   int pmax = 0, ymax = 0, cmax = 0;  // longest P, Y and coarse row
+  int64_t nY = 0;                    // blocks of Y
+  bool device_lists = false;         // the Y and coarse lists were built on the device
 };
 
 // Runs f(t, r0, r1) over the row ranges of nt host threads (thread 0 is the
@@ -1264,8 +1431,8 @@ int slot_rows(int nrows, int ncols, Gen gen, std::vector<int>* rptr, std::vector
   return lmax;
 }
 
-int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<int>& conns,
-                const std::vector<int>& agg, SaSym* s) {
+int sa_symbolic_p(int N, int Nc, const std::vector<int>& cptr, const std::vector<int>& conns,
+                  const std::vector<int>& agg, SaSym* s) {
   // P: a row's blocks are the distinct aggregates of its columns (ascending);
   // each lists the row's level blocks of that aggregate in order (atidx, the
   // row's entries at the row's own offset: every level block belongs to one
@@ -1350,6 +1517,12 @@ int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<i
     }
   });
   for (int i = 0; i < N; ++i) s->pmax = std::max(s->pmax, s->pptr[i + 1] - s->pptr[i]);
+  return 0;
+}
+
+// The Y and coarse lists on the host (when the device cannot hold a row).
+int sa_symbolic_yc(int N, int Nc, const std::vector<int>& cptr, const std::vector<int>& conns, SaSym* s) {
+  const int nP = (int)s->pcol.size();
   // Y = A P: row i's blocks by column b; a slot per (A block e, P block q) in
   // (e, q) order
   s->ymax = slot_rows(N, Nc, [&](int i, auto emit) {
@@ -1395,6 +1568,7 @@ int sa_symbolic(int N, int Nc, const std::vector<int>& cptr, const std::vector<i
       return -1;
     }
   }
+  s->nY = (int64_t)s->ycol.size();
   return 0;
 }
 
@@ -1402,6 +1576,123 @@ template <typename T>
 int sa_upload(Context* c, T** d, const std::vector<T>& h) {
   if (amg_alloc(c, d, h.size())) return -3;
   if (!h.empty()) LPMG_HIP(copy_sync(c, *d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// The Y and coarse lists on the device (sa_sym_kernel) for a level of N rows
+// with block pattern (dcptr, dconns) on the device and P's lists already in
+// L (pptr, pcol, qptr, qrow).  Fills L's yptr, ysp, ys, yso, csp, cs, cso and,
+// on the host, s's coarse pattern (ccptr, cconns, dpos), ymax, cmax and nY.
+// *done stays false when a row does not fit the kernel's LDS arrays (the
+// caller then builds the lists on the host).
+int sa_symbolic_yc_device(Context* c, int N, int Nc, int64_t nnzb, const int* dcptr, const int* dconns,
+                          AmgLevel* L, SaSym* s, bool* done) {
+  *done = false;
+  const int nP = (int)s->pcol.size();
+  const int nr = std::max(N, Nc);
+  int* flag = nullptr;
+  int* ucnt = nullptr;
+  int64_t* ecnt = nullptr;
+  int* ycol = nullptr;
+  int* dccptr = nullptr;
+  int* dcconns = nullptr;
+  int* ddiag = nullptr;
+  struct Tmp {
+    Context* c;
+    std::vector<void*> p;
+    ~Tmp() {
+      for (void* q : p)
+        if (q) (void)hipFreeAsync(q, c->stream);
+    }
+  } tmp{c, {}};
+  auto talloc = [&](auto** q, size_t bytes) -> int {
+    void* v = nullptr;
+    hipError_t e = hipMallocAsync(&v, std::max<size_t>(bytes, 8), c->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMallocAsync");
+    tmp.p.push_back(v);
+    *q = static_cast<std::remove_pointer_t<std::remove_reference_t<decltype(q)>>>(v);
+    return 0;
+  };
+  if (talloc(&flag, sizeof(int)) || talloc(&ucnt, sizeof(int) * nr) || talloc(&ecnt, sizeof(int64_t) * nr))
+    return -3;
+  LPMG_HIP(hipMemsetAsync(flag, 0, sizeof(int), c->stream));
+  int hflag = 0;
+  std::vector<int> hu(nr);
+  std::vector<int64_t> he(nr);
+  auto counts = [&](int rows) -> int {
+    LPMG_HIP(copy_sync(c, &hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (hflag) return 0;
+    LPMG_HIP(copy_sync(c, hu.data(), ucnt, sizeof(int) * rows, hipMemcpyDeviceToHost));
+    LPMG_HIP(copy_sync(c, he.data(), ecnt, sizeof(int64_t) * rows, hipMemcpyDeviceToHost));
+    return 0;
+  };
+  // Y = A P: counts and the sources' offsets
+  if (amg_alloc(c, &L->yso, (size_t)nnzb)) return -3;
+  sa_sym_kernel<0><<<N, 64, 0, c->stream>>>(N, dcptr, dconns, L->pptr, L->pcol, ucnt, ecnt, L->yso, nullptr,
+                                            nullptr, nullptr, nullptr, nullptr, flag);
+  LPMG_HIP(hipGetLastError());
+  if (int rc = counts(N)) return rc;
+  if (hflag) return 0;
+  std::vector<int> yptr(N + 1, 0);
+  std::vector<int64_t> ysp(N + 1, 0);
+  int ymax = 0;
+  for (int i = 0; i < N; ++i) {
+    yptr[i + 1] = yptr[i] + hu[i];
+    ysp[i + 1] = ysp[i] + he[i];
+    ymax = std::max(ymax, hu[i]);
+    if ((int64_t)yptr[i] + hu[i] >= ((int64_t)1 << 31)) {
+      set_error("multigrid: smoothed-aggregation rows too long for the device sums");
+      return -1;
+    }
+  }
+  const int64_t nY = yptr[N];
+  if (sa_upload(c, &L->yptr, yptr) || sa_upload(c, &L->ysp, ysp)) return -3;
+  if (amg_alloc(c, &L->ys, (size_t)ysp[N]) || talloc(&ycol, sizeof(int) * (size_t)nY)) return -3;
+  sa_sym_kernel<1><<<N, 64, 0, c->stream>>>(N, dcptr, dconns, L->pptr, L->pcol, nullptr, nullptr, nullptr, L->yptr,
+                                            L->ysp, ycol, L->ys, nullptr, flag);
+  LPMG_HIP(hipGetLastError());
+  // coarse rows P^T Y: counts and the sources' offsets
+  if (amg_alloc(c, &L->cso, (size_t)std::max(nP, 1))) return -3;
+  sa_sym_kernel<0><<<Nc, 64, 0, c->stream>>>(Nc, L->qptr, L->qrow, L->yptr, ycol, ucnt, ecnt, L->cso, nullptr,
+                                             nullptr, nullptr, nullptr, nullptr, flag);
+  LPMG_HIP(hipGetLastError());
+  if (int rc = counts(Nc)) return rc;
+  if (hflag) return 0;
+  s->ccptr.assign(Nc + 1, 0);
+  std::vector<int64_t> csp(Nc + 1, 0);
+  int cmax = 0;
+  for (int a = 0; a < Nc; ++a) {
+    s->ccptr[a + 1] = s->ccptr[a] + hu[a];
+    csp[a + 1] = csp[a] + he[a];
+    cmax = std::max(cmax, hu[a]);
+  }
+  const int nC = s->ccptr[Nc];
+  if (sa_upload(c, &L->csp, csp)) return -3;
+  if (amg_alloc(c, &L->cs, (size_t)csp[Nc]) || talloc(&dccptr, sizeof(int) * (Nc + 1)) ||
+      talloc(&dcconns, sizeof(int) * (size_t)nC) || talloc(&ddiag, sizeof(int) * Nc))
+    return -3;
+  LPMG_HIP(copy_sync(c, dccptr, s->ccptr.data(), sizeof(int) * (Nc + 1), hipMemcpyHostToDevice));
+  sa_sym_kernel<1><<<Nc, 64, 0, c->stream>>>(Nc, L->qptr, L->qrow, L->yptr, ycol, nullptr, nullptr, nullptr, dccptr,
+                                             L->csp, dcconns, L->cs, ddiag, flag);
+  LPMG_HIP(hipGetLastError());
+  LPMG_HIP(copy_sync(c, &hflag, flag, sizeof(int), hipMemcpyDeviceToHost));
+  if (hflag) return 0;
+  s->cconns.resize(nC);
+  s->dpos.resize(Nc);
+  LPMG_HIP(copy_sync(c, s->cconns.data(), dcconns, sizeof(int) * (size_t)nC, hipMemcpyDeviceToHost));
+  LPMG_HIP(copy_sync(c, s->dpos.data(), ddiag, sizeof(int) * Nc, hipMemcpyDeviceToHost));
+  for (int a = 0; a < Nc; ++a) {
+    if (s->dpos[a] < 0) {
+      set_error("multigrid: aggregate without a diagonal block");
+      return -1;
+    }
+    s->dpos[a] += s->ccptr[a];
+  }
+  s->ymax = ymax;
+  s->cmax = cmax;
+  s->nY = nY;
+  s->device_lists = true;
+  *done = true;
   return 0;
 }
 
@@ -1464,11 +1755,25 @@ int amg_build(Context* c) {
     std::vector<int64_t> gptr(1, 0);
     std::vector<std::pair<int, int>> pairs;
     SaSym sa;
+    AmgLevel L;
     if (A->smoothed) {
-      if (int rc = sa_symbolic(N, Nc, cptr, conns, agg, &sa)) return rc;
+      if (int rc = sa_symbolic_p(N, Nc, cptr, conns, agg, &sa)) return rc;
+      // P's lists on the device, then the Y and coarse lists: built on the
+      // device, or on the host when a row does not fit the device kernel
+      const std::pair<int**, const std::vector<int>*> ups[] = {
+          {&L.pptr, &sa.pptr}, {&L.pcol, &sa.pcol}, {&L.prow, &sa.prow}, {&L.atptr, &sa.atptr},
+          {&L.atidx, &sa.atidx}, {&L.qptr, &sa.qptr}, {&L.qrow, &sa.qrow}, {&L.qpos, &sa.qpos}};
+      for (const auto& up : ups)
+        if (int r2 = sa_upload(c, up.first, *up.second)) return r2;
+      const int* dcptr = A->lv.empty() ? c->cptr : A->lv.back().cptr;
+      const int* dconns = A->lv.empty() ? c->conns : A->lv.back().conns;
+      bool done = false;
+      if (int rc = sa_symbolic_yc_device(c, N, Nc, (int64_t)cptr[N], dcptr, dconns, &L, &sa, &done)) return rc;
+      if (!done)
+        if (int rc = sa_symbolic_yc(N, Nc, cptr, conns, &sa)) return rc;
       // a row's accumulators live in LDS (sa_rows_kernel)
       if ((int64_t)std::max(sa.ymax, sa.cmax) * D * D * (int64_t)sizeof(double) > 65536 ||
-          sa.ycol.size() >= ((size_t)1 << 31)) {
+          sa.nY >= ((int64_t)1 << 31)) {
         set_error("multigrid: smoothed-aggregation rows too long for the device sums");
         return -1;
       }
@@ -1498,7 +1803,6 @@ int amg_build(Context* c) {
         return -1;
       }
     }
-    AmgLevel L;
     L.N = Nc;
     L.nnzb = (int64_t)cconns.size();
     int maxrow = 0;
@@ -1527,21 +1831,21 @@ int amg_build(Context* c) {
       LPMG_HIP(copy_sync(c, L.gidx, gidx.data(), sizeof(int) * gidx.size(), hipMemcpyHostToDevice));
     } else {
       L.nP = (int)sa.pcol.size();
-      L.nY = (int64_t)sa.ycol.size();
+      L.nY = sa.nY;
       L.sa_gp = sa_outputs_per_wave((int64_t)sa.atidx.size(), L.nP);
       L.pmax = sa.pmax;
       L.ymax = sa.ymax;
       L.cmax = sa.cmax;
-      const std::pair<int**, const std::vector<int>*> ups[] = {
-          {&L.pptr, &sa.pptr}, {&L.pcol, &sa.pcol}, {&L.prow, &sa.prow}, {&L.atptr, &sa.atptr},
-          {&L.atidx, &sa.atidx}, {&L.qptr, &sa.qptr}, {&L.qrow, &sa.qrow},
-          {&L.qpos, &sa.qpos}, {&L.yptr, &sa.yptr}, {&L.yso, &sa.yso}, {&L.cso, &sa.cso}};
-      for (const auto& u : ups)
-        if (int r2 = sa_upload(c, u.first, *u.second)) return r2;
-      if (int r2 = sa_upload(c, &L.ysp, sa.ysp)) return r2;
-      if (int r2 = sa_upload(c, &L.csp, sa.csp)) return r2;
-      if (int r2 = sa_upload(c, &L.ys, sa.ys)) return r2;
-      if (int r2 = sa_upload(c, &L.cs, sa.cs)) return r2;
+      if (!sa.device_lists) {
+        const std::pair<int**, const std::vector<int>*> ups[] = {
+            {&L.yptr, &sa.yptr}, {&L.yso, &sa.yso}, {&L.cso, &sa.cso}};
+        for (const auto& u : ups)
+          if (int r2 = sa_upload(c, u.first, *u.second)) return r2;
+        if (int r2 = sa_upload(c, &L.ysp, sa.ysp)) return r2;
+        if (int r2 = sa_upload(c, &L.csp, sa.csp)) return r2;
+        if (int r2 = sa_upload(c, &L.ys, sa.ys)) return r2;
+        if (int r2 = sa_upload(c, &L.cs, sa.cs)) return r2;
+      }
     }
     A->lv.push_back(L);
     // next level: the aggregates' centroids and pattern
