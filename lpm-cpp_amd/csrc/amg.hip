@@ -1767,8 +1767,11 @@ int amg_build(Context* c) {
         if (int r2 = sa_upload(c, up.first, *up.second)) return r2;
       const int* dcptr = A->lv.empty() ? c->cptr : A->lv.back().cptr;
       const int* dconns = A->lv.empty() ? c->conns : A->lv.back().conns;
+      // LPMG_SA_HOST_SYMBOLIC set: the host builder only (the two give the
+      // same lists; tests compare them through their results)
       bool done = false;
-      if (int rc = sa_symbolic_yc_device(c, N, Nc, (int64_t)cptr[N], dcptr, dconns, &L, &sa, &done)) return rc;
+      if (!std::getenv("LPMG_SA_HOST_SYMBOLIC"))
+        if (int rc = sa_symbolic_yc_device(c, N, Nc, (int64_t)cptr[N], dcptr, dconns, &L, &sa, &done)) return rc;
       if (!done)
         if (int rc = sa_symbolic_yc(N, Nc, cptr, conns, &sa)) return rc;
       // a row's accumulators live in LDS (sa_rows_kernel)

@@ -314,3 +314,27 @@ def test_served_sa_request_within_its_memory_estimate():
             assert st["bytes"] <= st["bytes_est"], (st["bytes"], st["bytes_est"])
             out.append(rq.out["XYZ"].copy())
         np.testing.assert_array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("prob", [P.ct_plate(12000, 0.35, steps=3, opening=7.5e-6, crit=1e-3, tol=1e-9),
+                                  P.bcc_j2_bar(nx=9, ny=9, nz=18, steps=3, strain_step=5e-4)],
+                         ids=["ct", "bcc"])
+def test_device_and_host_symbolic_lists_agree(prob, precision, monkeypatch):
+    # the Y and coarse symbolic lists are built on the device (the host builds
+    # them for rows the device kernel cannot hold, or with
+    # LPMG_SA_HOST_SYMBOLIC set): both must give the same hierarchy, so the
+    # same bits in a full quasi-static run
+    opts = dict(precond=4, rtol=1e-8 if precision == 0 else 1e-4, atol=0.0, precision=precision)
+    runs = []
+    for host in (False, True):
+        if host:
+            monkeypatch.setenv("LPMG_SA_HOST_SYMBOLIC", "1")
+        else:
+            monkeypatch.delenv("LPMG_SA_HOST_SYMBOLIC", raising=False)
+        s, stats = _run(prob, lpmg.System, opts, tol=1e-9)
+        runs.append((s.get("XYZ"), s.multigrid_levels(), [tuple(x) for x in stats]))
+        s.close()
+    assert runs[0][1] == runs[1][1] and len(runs[0][1]) >= 3
+    assert runs[0][2] == runs[1][2]
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
