@@ -120,7 +120,7 @@ def test_polycrystal_served_with_ct_requests(poly):
 # CT requests of the serving stream's workload (2D hex CT, a/W in [0.2, 0.5],
 # critical bond stretch 1e-3, 30 load steps of pin opening 7.5e-6 W, Newton tol 1e-9)
 CT_TOL = 1e-9
-SERVED = dict(precond=-1, rtol=1e-4, atol=0.0, precision=1, predict=1.0)
+SERVED = dict(precond=-1, rtol=1e-5, atol=0.0, precision=1, predict=1.0)
 
 
 def test_large_served_ct_matches_plain_fp64():
