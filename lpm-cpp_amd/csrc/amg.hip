@@ -1318,6 +1318,13 @@ struct Amg {
   // the hierarchy's device arrays (freed with it) and their bytes
   std::vector<void*> allocs;
   size_t bytes = 0;
+  // smoothed hierarchy: the finest level's grid cell and aggregate of each
+  // node at build time, host copies of the bond lists, the context's
+  // break_version the aggregates were last checked against, and whether a
+  // load step has begun since (the check runs once per step)
+  std::vector<int> cell0, agg0, bptr_h, p2_h;
+  int64_t break_version = -1;
+  bool recheck = false;
 };
 
 namespace {
@@ -1696,6 +1703,117 @@ int sa_symbolic_yc_device(Context* c, int N, int Nc, int64_t nnzb, const int* dc
   return 0;
 }
 
+// The intact bonds (|bdamage - 1| >= kEPS, the fragment labelling's test) as
+// node pairs (min, max), one per intact bond slot.  The bond lists are copied
+// to the host once per hierarchy.
+int intact_edges(Context* c, std::vector<int>& bptr, std::vector<int>& p2, std::vector<std::pair<int, int>>* e) {
+  const int N = c->N;
+  const int64_t NB = c->NB;
+  if (bptr.empty()) {
+    bptr.resize(N + 1);
+    p2.resize(std::max<int64_t>(NB, 1));
+    LPMG_HIP(copy_sync(c, bptr.data(), c->bptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
+    if (NB) LPMG_HIP(copy_sync(c, p2.data(), c->p2, sizeof(int) * NB, hipMemcpyDeviceToHost));
+  }
+  std::vector<double> bd(std::max<int64_t>(NB, 1));
+  if (NB) LPMG_HIP(copy_sync(c, bd.data(), c->bdamage, sizeof(double) * NB, hipMemcpyDeviceToHost));
+  e->clear();
+  e->reserve(NB);
+  for (int i = 0; i < N; ++i)
+    for (int b = bptr[i]; b < bptr[i + 1]; ++b)
+      if (fabs(bd[b] - 1.0) >= kEPS) e->emplace_back(std::min(i, p2[b]), std::max(i, p2[b]));
+  return 0;
+}
+
+// Union-find over the edges whose two nodes share a group; each node's root
+// is the smallest node of its component (union by the smaller index).
+std::vector<int> group_components(int N, const std::vector<std::pair<int, int>>& e, const std::vector<int>& group) {
+  std::vector<int> par(N);
+  std::iota(par.begin(), par.end(), 0);
+  auto find = [&](int x) {
+    while (par[x] != x) {
+      par[x] = par[par[x]];
+      x = par[x];
+    }
+    return x;
+  };
+  for (const auto& uv : e) {
+    if (group[uv.first] != group[uv.second]) continue;
+    const int a = find(uv.first), b = find(uv.second);
+    if (a != b) par[std::max(a, b)] = std::min(a, b);
+  }
+  for (int i = 0; i < N; ++i) par[i] = find(i);
+  return par;
+}
+
+// Split every aggregate into the connected components of the intact bonds
+// (at coarser levels: the bonds between aggregates) inside it, so that no
+// aggregate spans a crack.  The new aggregates keep the order of their cells;
+// a cell's components follow their smallest node.  Members stay in node order.
+// A split that would leave more than 80% of the nodes as aggregates (a level
+// falling apart into fragments) is not made.
+void split_aggregates(int N, const std::vector<std::pair<int, int>>& e, std::vector<int>& agg,
+                      std::vector<int>& mptr, std::vector<int>& midx, int& Nc) {
+  const std::vector<int> root = group_components(N, e, agg);
+  std::vector<int> rid(N, -1);
+  int n2 = 0;
+  for (int q = 0; q < N; ++q) {  // members run in cell order, node order inside a cell
+    const int r = root[midx[q]];
+    if (rid[r] < 0) rid[r] = n2++;
+  }
+  if (n2 == Nc || n2 > (int)(0.8 * N)) return;
+  std::vector<int> nptr(n2 + 1, 0), nmidx(N);
+  for (int i = 0; i < N; ++i) {
+    agg[i] = rid[root[i]];
+    ++nptr[agg[i] + 1];
+  }
+  for (int a = 0; a < n2; ++a) nptr[a + 1] += nptr[a];
+  std::vector<int> fill(nptr.begin(), nptr.end() - 1);
+  for (int q = 0; q < N; ++q) nmidx[fill[agg[midx[q]]]++] = midx[q];
+  mptr.swap(nptr);
+  midx.swap(nmidx);
+  Nc = n2;
+}
+
+// The next level's edges: pairs of distinct aggregates joined by an edge.
+void coarse_edges(const std::vector<int>& agg, std::vector<std::pair<int, int>>& e) {
+  size_t k = 0;
+  for (const auto& uv : e) {
+    const int a = agg[uv.first], b = agg[uv.second];
+    if (a != b) e[k++] = {std::min(a, b), std::max(a, b)};
+  }
+  e.resize(k);
+  std::sort(e.begin(), e.end());
+  e.erase(std::unique(e.begin(), e.end()), e.end());
+}
+
+// Whether the finest level's cells, split by the bonds intact now, give other
+// aggregates than the hierarchy's (bonds broke across a cell, or healed).
+int amg_split_stale(Context* c, bool* stale) {
+  Amg* A = c->amg;
+  *stale = false;
+  if (A->cell0.empty()) return 0;  // a single-level hierarchy: nothing aggregated
+  std::vector<std::pair<int, int>> e;
+  if (int rc = intact_edges(c, A->bptr_h, A->p2_h, &e)) return rc;
+  const int N = c->N;
+  std::vector<int> root = group_components(N, e, A->cell0);
+  int n2 = 0;
+  for (int i = 0; i < N; ++i) n2 += root[i] == i;
+  if (n2 > (int)(0.8 * N)) root = A->cell0;  // such a split is not made
+  // the two partitions agree when each aggregate maps to one root and back
+  std::vector<int> a2r(N, -1), r2a(N, -1);
+  for (int i = 0; i < N; ++i) {
+    const int a = A->agg0[i], r = root[i];
+    if (a2r[a] < 0) a2r[a] = r;
+    if (r2a[r] < 0) r2a[r] = a;
+    if (a2r[a] != r || r2a[r] != a) {
+      *stale = true;
+      return 0;
+    }
+  }
+  return 0;
+}
+
 // Build the aggregation hierarchy on the host from the initial positions and
 // the level-0 block pattern.
 int amg_build(Context* c) {
@@ -1714,6 +1832,12 @@ int amg_build(Context* c) {
     pos[3 * i + 2] = x4[i].z;
   }
   double h = 3.0 * 2.0 * c->cell.radius;
+  // the smoothed hierarchy's aggregates follow the intact bonds
+  std::vector<std::pair<int, int>> edges;
+  if (A->smoothed) {
+    if (int rc = intact_edges(c, A->bptr_h, A->p2_h, &edges)) return rc;
+    A->break_version = c->break_version;
+  }
   while ((int64_t)N * D > kCoarsestMax) {
     // aggregates: nonempty cells of a grid of edge h, numbered in cell order
     double lo[3] = {1e300, 1e300, 1e300};
@@ -1747,6 +1871,11 @@ int amg_build(Context* c) {
       h *= 2.0;
       if (h > 1e6 * c->cell.radius) break;
       continue;
+    }
+    if (A->smoothed) {
+      if (A->lv.empty()) A->cell0 = agg;
+      split_aggregates(N, edges, agg, mptr, midx, Nc);
+      if (A->lv.empty()) A->agg0 = agg;
     }
     // coarse pattern and gather lists: per aggregate row, the (column, fine
     // block) pairs sorted by column then fine block; each column run is one
@@ -1862,6 +1991,7 @@ int amg_build(Context* c) {
     pos.swap(cen);
     cptr.swap(ccptr);
     conns.swap(cconns);
+    if (A->smoothed) coarse_edges(agg, edges);
     N = Nc;
     h *= 3.5;
   }
@@ -2166,6 +2296,10 @@ void vcycle(Context* c, hipStream_t st, int l, const T* b, T* out, T* scratch_r,
 
 }  // namespace
 
+void amg_step_begins(Context* c) {
+  if (c->amg && c->amg->smoothed) c->amg->recheck = true;
+}
+
 void amg_free(Context* c) {
   if (!c->amg) return;
   for (auto& g : c->amg->graph)
@@ -2186,6 +2320,18 @@ template <int D, typename T>
 int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   const bool smoothed = o.precond == 4;
   if (c->amg && c->amg->smoothed != smoothed) amg_free(c);  // the other kind of hierarchy
+  // at a load step's first solve, when bonds broke or healed since the last
+  // check: rebuild a smoothed hierarchy whose finest aggregates no longer
+  // follow the intact bonds
+  if (c->amg && c->amg->smoothed && c->amg->recheck) {
+    c->amg->recheck = false;
+    if (c->amg->break_version != c->break_version) {
+      bool stale = false;
+      if (int rc = amg_split_stale(c, &stale)) return rc;
+      c->amg->break_version = c->break_version;
+      if (stale) amg_free(c);
+    }
+  }
   if (!c->amg) {
     c->amg = new Amg;
     c->amg->smoothed = smoothed;

@@ -825,6 +825,7 @@ int lpmg_solve_step(lpmg_handle h, int undamaged, int max_iter, double tol, cons
   bool new_damaged = false;
   if (so.predict > 0 && c->inc_valid)
     if (int rc = launch_predict(c, so.predict)) return rc;
+  amg_step_begins(c);
   do {
     if (int rc = launch_assemble(c, so.precision)) return rc;
     double st[4] = {0, 0, 0, 0};
@@ -881,6 +882,7 @@ int lpmg_get_field(lpmg_handle h, int f, double* out) {
 int lpmg_set_field(lpmg_handle h, int f, const double* in) {
   CHECK_HANDLE(h);
   h->c.floating_dirty = true;
+  ++h->c.break_version;
   Context* c = &h->c;
   const size_t n = field_len(c, f);
   const bool writable = f == LPMG_F_XYZ || f == LPMG_F_XYZ_LAST || f == LPMG_F_PEX || f == LPMG_F_DAMAGE ||
@@ -924,6 +926,7 @@ int lpmg_restart(lpmg_handle h) {
   API_HIP(hipMemcpyAsync(c->dis_last, c->dis0, sizeof(double) * NB, hipMemcpyDeviceToDevice, c->stream));
   API_HIP(hipMemsetAsync(c->floating, 0, sizeof(int) * N, c->stream));
   c->floating_dirty = true;
+  ++c->break_version;
   c->step_valid = c->inc_valid = false;
   c->reac_sumsq = 0;
   c->assembled64 = c->assembled32 = false;

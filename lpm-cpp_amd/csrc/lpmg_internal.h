@@ -196,6 +196,10 @@ struct Context {
   int *comp = nullptr, *compfix = nullptr, *floating = nullptr, *changed = nullptr;
   bool floating_dirty = true;
   int n_floating = 0;
+  // bumped whenever bonds may have broken or healed (damage passes, state
+  // resets, field writes): the smoothed hierarchy's aggregates follow the
+  // intact bonds
+  int64_t break_version = 0;
   Amg* amg = nullptr;
 };
 
@@ -291,6 +295,7 @@ int run_bicgstab(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 template <int D, typename T>
 int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final);
 void amg_free(Context* c);
+void amg_step_begins(Context* c);  // a load step starts: the aggregates may be rechecked
 int amg_levels(Context* c, int* out, int cap);
 int cg_class(int dim, int fp32, int chunks);
 void cg_batch_layout(CgSys* tab, int nsys, CgBatchHdr* h);

@@ -270,7 +270,10 @@ static int read_damage_flags(Context* c, int* any) {
   LPMG_HIP(hipStreamSynchronize(c->stream));
   LPMG_HIP(hipMemsetAsync(c->err_flag, 0, sizeof(int), c->stream));
   *any = h & 1;
-  if (h & 2) c->floating_dirty = true;
+  if (h & 2) {
+    c->floating_dirty = true;
+    ++c->break_version;
+  }
   return 0;
 }
 
@@ -448,6 +451,7 @@ int record_step_end(Context* c) {
 int reset_state(Context* c, int reset_xyz) {
   const size_t N = c->N, NB = c->NB;
   c->floating_dirty = true;
+  ++c->break_version;
   auto cp = [&](void* d, const void* s, size_t bytes) {
     return hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToDevice, c->stream);
   };

@@ -7,7 +7,10 @@ oracle: displacements and forces within 1e-8 relative L2 at a tight Newton
 tolerance, damage flags exactly equal.  The hierarchy itself is pinned by
 `sa_pcg_np`, a plain numpy/scipy restatement of the method on the oracle's
 matrix: grid-cell aggregates (cell edge 3 lattice spacings, x3.5 per level,
-doubled while a level keeps more than 80% of its nodes), the prolongation
+doubled while a level keeps more than 80% of its nodes), each cell split into
+the connected components of the intact bonds inside it (coarser levels: of
+the bonds between aggregates; cells first, a cell's components by smallest
+node; no split that leaves more than 80% of the nodes), the prolongation
 P = (I - om_p D^-1 A) T of the piecewise-constant T, Galerkin coarse
 operators P^T A P, a dense coarsest inverse, one V-cycle of damped Jacobi
 before and after the coarse correction, z = 0 on the eliminated DOFs.  Per
@@ -22,6 +25,7 @@ CT plates).
 import numpy as np
 import pytest
 import scipy.sparse as sp
+from scipy.sparse.csgraph import connected_components
 
 import lpmg
 from lpmg import lattice as lat
@@ -51,6 +55,39 @@ def _aggregate(pos, h):
     key = (cc[:, 2] * dims[1] + cc[:, 1]) * dims[0] + cc[:, 0]
     _, agg = np.unique(key, return_inverse=True)
     return agg.ravel(), int(agg.max()) + 1
+
+
+def intact_edges(o):
+    """Node pairs (min, max) of every intact bond slot (|bdamage - 1| >= 1e-6)."""
+    bptr, _, p2 = o.bonds()
+    bd = o.get("BDAMAGE")[: len(p2)]
+    src = np.repeat(np.arange(o.N), np.diff(bptr))
+    ok = np.abs(bd - 1.0) >= 1e-6
+    return np.stack([np.minimum(src[ok], p2[ok]), np.maximum(src[ok], p2[ok])], 1)
+
+
+def split_aggregates(agg, nc, e):
+    """Each aggregate split into the connected components of the edges inside
+    it, in cell order and a cell's components by smallest node; unchanged when
+    more than 80% of the nodes would be aggregates."""
+    n = len(agg)
+    inside = agg[e[:, 0]] == agg[e[:, 1]]
+    G = sp.csr_matrix((np.ones(int(inside.sum())), (e[inside, 0], e[inside, 1])), shape=(n, n))
+    ncc, lab = connected_components(G, directed=False)
+    low = np.full(ncc, n)
+    np.minimum.at(low, lab, np.arange(n))
+    _, agg2 = np.unique(agg.astype(np.int64) * (n + 1) + low[lab], return_inverse=True)
+    nc2 = int(agg2.max()) + 1
+    return (agg, nc) if nc2 > 0.8 * n else (agg2.ravel(), nc2)
+
+
+def coarse_edges(agg, e):
+    """The next level's edges: distinct aggregate pairs joined by an edge."""
+    a, b = agg[e[:, 0]], agg[e[:, 1]]
+    k = a != b
+    if not k.any():
+        return np.zeros((0, 2), np.int64)
+    return np.unique(np.stack([np.minimum(a[k], b[k]), np.maximum(a[k], b[k])], 1), axis=0)
 
 
 def lanczos_start(n):
@@ -86,8 +123,9 @@ def dampings(A):
     return min(OMEGA_MAX, OMEGA_TARGET / lam), min(1.0, P_TARGET / lam)
 
 
-def sa_hierarchy(A, pos, radius, dim, coarsest=128):
-    """Levels [(A_l, P_l, dinv_l, smoother damping_l)] and the coarsest matrix."""
+def sa_hierarchy(A, pos, radius, dim, coarsest=128, edges=None):
+    """Levels [(A_l, P_l, dinv_l, smoother damping_l)] and the coarsest matrix;
+    with edges (intact bonds as node pairs) the cells are split along them."""
     levels = []
     h = 3.0 * 2.0 * radius
     while A.shape[0] > coarsest:
@@ -96,6 +134,9 @@ def sa_hierarchy(A, pos, radius, dim, coarsest=128):
         if nc > 0.8 * n:
             h *= 2.0
             continue
+        if edges is not None:
+            agg, nc = split_aggregates(agg, nc, edges)
+            edges = coarse_edges(agg, edges)
         rows = np.arange(n * dim)
         T = sp.csr_matrix((np.ones(n * dim), (rows, np.repeat(agg, dim) * dim + np.tile(np.arange(dim), n))),
                           shape=(n * dim, nc * dim))
@@ -111,10 +152,10 @@ def sa_hierarchy(A, pos, radius, dim, coarsest=128):
     return levels, A.toarray()
 
 
-def sa_pcg_np(A, b, mask, pos, radius, dim, rtol, maxit):
+def sa_pcg_np(A, b, mask, pos, radius, dim, rtol, maxit, edges=None):
     """PCG with one smoothed-aggregation V-cycle as M; stops when
     ||r||^2 <= rtol ||r0||^2.  Returns (x, iterations, level sizes)."""
-    levels, Ac = sa_hierarchy(A, pos, radius, dim)
+    levels, Ac = sa_hierarchy(A, pos, radius, dim, edges=edges)
     inv = np.linalg.inv(Ac)
 
     def vcycle(l, rhs):
@@ -188,7 +229,7 @@ def test_sa_levels_and_iterations_match_numpy_restatement(name, L, gen):
     b = o.get("RESIDUAL")
     x0 = g.get("XYZ")
     st = g.solve_linear(precond=4, rtol=1e-16, atol=0.0)
-    x_np, it_np, sizes = sa_pcg_np(A, b, 1.0 - fixed, xyz, R, o.dim, 1e-16, 2000)
+    x_np, it_np, sizes = sa_pcg_np(A, b, 1.0 - fixed, xyz, R, o.dim, 1e-16, 2000, edges=intact_edges(o))
     assert g.multigrid_levels() == sizes
     assert len(sizes) >= 3
     assert abs(int(st[0]) - it_np) <= max(2, it_np // 20), (int(st[0]), it_np)
@@ -198,6 +239,105 @@ def test_sa_levels_and_iterations_match_numpy_restatement(name, L, gen):
     g.set("XYZ", x0)
     st2 = g.solve_linear(precond=2, rtol=1e-16, atol=0.0)
     assert st2[0] > st[0], (st2[0], st[0])
+
+
+def _crack(o, xyz):
+    """Bond damage of a straight crack: every bond crossing the line midway
+    between two middle rows, over the left half of the plate, broken."""
+    bptr, _, p2 = o.bonds()
+    src = np.repeat(np.arange(o.N), np.diff(bptr))
+    x, y = xyz[:, 0], xyz[:, 1]
+    rows = np.unique(np.round(y, 9))
+    ym = 0.5 * (rows[len(rows) // 2] + rows[len(rows) // 2 + 1])
+    xm = 0.5 * (x.min() + x.max())
+    cut = ((y[src] - ym) * (y[p2] - ym) < 0) & (np.minimum(x[src], x[p2]) < xm)
+    return cut.astype(np.float64)
+
+
+def _level_sizes(xyz, dim, edges, coarsest=128):
+    """The hierarchy's level sizes from its aggregation alone."""
+    sizes, pos, h, n = [len(xyz) * dim], xyz, 6.0 * R, len(xyz)
+    while n * dim > coarsest:
+        agg, nc = _aggregate(pos, h)
+        if nc > 0.8 * n:
+            h *= 2.0
+            continue
+        if edges is not None:
+            agg, nc = split_aggregates(agg, nc, edges)
+            edges = coarse_edges(agg, edges)
+        cen = np.zeros((nc, 3))
+        np.add.at(cen, agg, pos)
+        pos, h, n = cen / np.bincount(agg)[:, None], h * 3.5, nc
+        sizes.append(nc * dim)
+    return sizes
+
+
+def test_sa_aggregates_split_along_a_crack():
+    # cells cut by a crack become one aggregate per side: the level sizes and
+    # the PCG path follow the restatement with the split, and the split cells
+    # make more aggregates than the cells alone
+    xyz = lat.hex_plate(60, 52, R)
+    g, o, fixed = _loaded_pair(lat.HEXAGON2D, xyz)
+    bd = _crack(o, xyz)
+    assert bd.sum() > 50
+    for s in (g, o):
+        s.set("BDAMAGE", bd)
+        s.forward()
+        s.assemble()
+        s.update_rr()
+    A = _full_matrix(o)
+    b = o.get("RESIDUAL")
+    x0 = g.get("XYZ")
+    e = intact_edges(o)
+    st = g.solve_linear(precond=4, rtol=1e-16, atol=0.0)
+    x_np, it_np, sizes = sa_pcg_np(A, b, 1.0 - fixed, xyz, R, 2, 1e-16, 2000, edges=e)
+    assert g.multigrid_levels() == sizes
+    assert sizes == _level_sizes(xyz, 2, e)
+    unsplit = _level_sizes(xyz, 2, None)
+    assert sizes[1] > unsplit[1], (sizes, unsplit)
+    assert abs(int(st[0]) - it_np) <= max(2, it_np // 20), (int(st[0]), it_np)
+    du_g = (g.get("XYZ") - x0).reshape(-1, 3)[:, :2].ravel()
+    assert _relerr(du_g, x_np) < 1e-6
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sa_hierarchy_follows_bonds_broken_before_a_step(precision):
+    # a hierarchy built on the intact plate is kept by plain solves and rebuilt
+    # at the first solve of the next load step once bonds broke across its
+    # cells; the rebuilt one solves bit for bit as one built on the cracked
+    # plate from the start
+    xyz = lat.hex_plate(60, 52, R)
+    opts = dict(precond=4, rtol=1e-20 if precision == 0 else 1e-5, atol=0.0, precision=precision)
+    g, o, _ = _loaded_pair(lat.HEXAGON2D, xyz)
+    x0 = g.get("XYZ")
+    g.assemble(precision=precision)
+    g.solve_linear(**opts)
+    assert g.multigrid_levels() == _level_sizes(xyz, 2, None)
+    bd = _crack(o, xyz)
+    o.set("BDAMAGE", bd)
+
+    def cracked(s):
+        s.set("BDAMAGE", bd)
+        s.set("XYZ", x0)
+        s.forward()
+        s.assemble(precision=precision)
+        s.update_rr()
+
+    cracked(g)
+    g.solve_linear(**opts)  # not a load step: the hierarchy stays
+    assert g.multigrid_levels() == _level_sizes(xyz, 2, None)
+    g.set("XYZ", x0)
+    g.solve_step(-1, 1, 1e-11, **opts)  # one Newton iteration of a load step
+    assert g.multigrid_levels() == _level_sizes(xyz, 2, intact_edges(o))
+    fresh, _, _ = _loaded_pair(lat.HEXAGON2D, xyz)
+    out = []
+    for s in (g, fresh):
+        cracked(s)
+        st = s.solve_linear(**opts)
+        out.append((int(st[0]), s.get("XYZ") - x0))
+    assert fresh.multigrid_levels() == g.multigrid_levels()
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
 def _run(prob, backend, opts, tol=1e-11):
