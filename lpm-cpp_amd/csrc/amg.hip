@@ -1322,7 +1322,7 @@ struct Amg {
   // node at build time, host copies of the bond lists, the context's
   // break_version the aggregates were last checked against, and whether a
   // load step has begun since (the check runs once per step)
-  std::vector<int> cell0, agg0, bptr_h, p2_h;
+  std::vector<int> cell0, agg0, bonds_h;
   int64_t break_version = -1;
   bool recheck = false;
 };
@@ -1703,25 +1703,30 @@ int sa_symbolic_yc_device(Context* c, int N, int Nc, int64_t nnzb, const int* dc
   return 0;
 }
 
-// The intact bonds (|bdamage - 1| >= kEPS, the fragment labelling's test) as
-// node pairs (min, max), one per intact bond slot.  The bond lists are copied
-// to the host once per hierarchy.
-int intact_edges(Context* c, std::vector<int>& bptr, std::vector<int>& p2, std::vector<std::pair<int, int>>* e) {
+// The intact bonds as node pairs (i, j), i < j: a bond joins its particles
+// while either of its two slots is intact (|bdamage - 1| >= kEPS, the fragment
+// labelling's test).  The bond lists are copied to the host once per hierarchy.
+int intact_edges(Context* c, std::vector<int>& bl, std::vector<std::pair<int, int>>* e) {
   const int N = c->N;
   const int64_t NB = c->NB;
-  if (bptr.empty()) {
-    bptr.resize(N + 1);
-    p2.resize(std::max<int64_t>(NB, 1));
-    LPMG_HIP(copy_sync(c, bptr.data(), c->bptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
-    if (NB) LPMG_HIP(copy_sync(c, p2.data(), c->p2, sizeof(int) * NB, hipMemcpyDeviceToHost));
+  if (bl.empty()) {  // bptr (N + 1), then p2 and rev (NB each)
+    bl.resize(N + 1 + 2 * NB);
+    LPMG_HIP(copy_sync(c, bl.data(), c->bptr, sizeof(int) * (N + 1), hipMemcpyDeviceToHost));
+    if (NB) {
+      LPMG_HIP(copy_sync(c, bl.data() + N + 1, c->p2, sizeof(int) * NB, hipMemcpyDeviceToHost));
+      LPMG_HIP(copy_sync(c, bl.data() + N + 1 + NB, c->rev, sizeof(int) * NB, hipMemcpyDeviceToHost));
+    }
   }
+  const int* bptr = bl.data();
+  const int* p2 = bptr + N + 1;
+  const int* rev = p2 + NB;
   std::vector<double> bd(std::max<int64_t>(NB, 1));
   if (NB) LPMG_HIP(copy_sync(c, bd.data(), c->bdamage, sizeof(double) * NB, hipMemcpyDeviceToHost));
   e->clear();
-  e->reserve(NB);
+  e->reserve(NB / 2 + 1);
   for (int i = 0; i < N; ++i)
     for (int b = bptr[i]; b < bptr[i + 1]; ++b)
-      if (fabs(bd[b] - 1.0) >= kEPS) e->emplace_back(std::min(i, p2[b]), std::max(i, p2[b]));
+      if (i < p2[b] && (fabs(bd[b] - 1.0) >= kEPS || fabs(bd[rev[b]] - 1.0) >= kEPS)) e->emplace_back(i, p2[b]);
   return 0;
 }
 
@@ -1775,16 +1780,28 @@ void split_aggregates(int N, const std::vector<std::pair<int, int>>& e, std::vec
   Nc = n2;
 }
 
-// The next level's edges: pairs of distinct aggregates joined by an edge.
-void coarse_edges(const std::vector<int>& agg, std::vector<std::pair<int, int>>& e) {
-  size_t k = 0;
+// The next level's edges: each pair of distinct aggregates joined by an edge,
+// once (bucketed by the smaller aggregate, repeats dropped with a marker).
+void coarse_edges(const std::vector<int>& agg, int Nc, std::vector<std::pair<int, int>>& e) {
+  std::vector<int> start(Nc + 1, 0);
   for (const auto& uv : e) {
     const int a = agg[uv.first], b = agg[uv.second];
-    if (a != b) e[k++] = {std::min(a, b), std::max(a, b)};
+    if (a != b) ++start[std::min(a, b) + 1];
   }
-  e.resize(k);
-  std::sort(e.begin(), e.end());
-  e.erase(std::unique(e.begin(), e.end()), e.end());
+  for (int a = 0; a < Nc; ++a) start[a + 1] += start[a];
+  std::vector<int> fill(start.begin(), start.end() - 1), other(start[Nc]);
+  for (const auto& uv : e) {
+    const int a = agg[uv.first], b = agg[uv.second];
+    if (a != b) other[fill[std::min(a, b)]++] = std::max(a, b);
+  }
+  std::vector<int> mark(Nc, -1);
+  e.clear();
+  for (int a = 0; a < Nc; ++a)
+    for (int k = start[a]; k < start[a + 1]; ++k)
+      if (mark[other[k]] != a) {
+        mark[other[k]] = a;
+        e.emplace_back(a, other[k]);
+      }
 }
 
 // Whether the finest level's cells, split by the bonds intact now, give other
@@ -1794,7 +1811,7 @@ int amg_split_stale(Context* c, bool* stale) {
   *stale = false;
   if (A->cell0.empty()) return 0;  // a single-level hierarchy: nothing aggregated
   std::vector<std::pair<int, int>> e;
-  if (int rc = intact_edges(c, A->bptr_h, A->p2_h, &e)) return rc;
+  if (int rc = intact_edges(c, A->bonds_h, &e)) return rc;
   const int N = c->N;
   std::vector<int> root = group_components(N, e, A->cell0);
   int n2 = 0;
@@ -1835,7 +1852,7 @@ int amg_build(Context* c) {
   // the smoothed hierarchy's aggregates follow the intact bonds
   std::vector<std::pair<int, int>> edges;
   if (A->smoothed) {
-    if (int rc = intact_edges(c, A->bptr_h, A->p2_h, &edges)) return rc;
+    if (int rc = intact_edges(c, A->bonds_h, &edges)) return rc;
     A->break_version = c->break_version;
   }
   while ((int64_t)N * D > kCoarsestMax) {
@@ -1991,7 +2008,7 @@ int amg_build(Context* c) {
     pos.swap(cen);
     cptr.swap(ccptr);
     conns.swap(cconns);
-    if (A->smoothed) coarse_edges(agg, edges);
+    if (A->smoothed) coarse_edges(agg, Nc, edges);
     N = Nc;
     h *= 3.5;
   }
