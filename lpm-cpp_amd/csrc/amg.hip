@@ -1256,6 +1256,12 @@ int rows_grid(int N, int lpr) {
   return (N + rows_per_block - 1) / rows_per_block;
 }
 
+// flag = 1 when any bond slot is broken (|bdamage - 1| < kEPS)
+__global__ void any_broken_kernel(const double* __restrict__ bdamage, int64_t nb, int* __restrict__ flag) {
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x)
+    if (fabs(bdamage[b] - 1.0) < kEPS) atomicOr(flag, 1);
+}
+
 }  // namespace
 
 // ---- hierarchy ------------------------------------------------------------
@@ -1325,6 +1331,7 @@ struct Amg {
   std::vector<int> cell0, agg0, bonds_h;
   int64_t break_version = -1;
   bool recheck = false;
+  int* flag = nullptr;  // device word for any_broken
 };
 
 namespace {
@@ -1730,6 +1737,24 @@ int intact_edges(Context* c, std::vector<int>& bl, std::vector<std::pair<int, in
   return 0;
 }
 
+// Whether the aggregates should follow the bonds: in 2D always (a notch cut
+// into the lattice splits cells before any bond breaks; 1.3M bonds cost 15 ms
+// on the 205k CT request), in 3D once some bond is broken (the 1.0M-particle
+// polycrystal's 7M bonds cost 0.17 s per build and split no cell).
+int split_by_bonds(Context* c, bool* split) {
+  *split = true;
+  if (c->dim == 2 || c->NB == 0) return 0;
+  Amg* A = c->amg;
+  if (!A->flag && amg_alloc(c, &A->flag, 1)) return -3;
+  LPMG_HIP(hipMemsetAsync(A->flag, 0, sizeof(int), c->stream));
+  any_broken_kernel<<<kMaxGrid, kBS, 0, c->stream>>>(c->bdamage, c->NB, A->flag);
+  LPMG_HIP(hipGetLastError());
+  int any = 0;
+  LPMG_HIP(copy_sync(c, &any, A->flag, sizeof(int), hipMemcpyDeviceToHost));
+  *split = any != 0;
+  return 0;
+}
+
 // Union-find over the edges whose two nodes share a group; each node's root
 // is the smallest node of its component (union by the smaller index).
 std::vector<int> group_components(int N, const std::vector<std::pair<int, int>>& e, const std::vector<int>& group) {
@@ -1810,13 +1835,18 @@ int amg_split_stale(Context* c, bool* stale) {
   Amg* A = c->amg;
   *stale = false;
   if (A->cell0.empty()) return 0;  // a single-level hierarchy: nothing aggregated
-  std::vector<std::pair<int, int>> e;
-  if (int rc = intact_edges(c, A->bonds_h, &e)) return rc;
   const int N = c->N;
-  std::vector<int> root = group_components(N, e, A->cell0);
-  int n2 = 0;
-  for (int i = 0; i < N; ++i) n2 += root[i] == i;
-  if (n2 > (int)(0.8 * N)) root = A->cell0;  // such a split is not made
+  bool split = false;
+  if (int rc = split_by_bonds(c, &split)) return rc;
+  std::vector<int> root = A->cell0;
+  if (split) {
+    std::vector<std::pair<int, int>> e;
+    if (int rc = intact_edges(c, A->bonds_h, &e)) return rc;
+    root = group_components(N, e, A->cell0);
+    int n2 = 0;
+    for (int i = 0; i < N; ++i) n2 += root[i] == i;
+    if (n2 > (int)(0.8 * N)) root = A->cell0;  // such a split is not made
+  }
   // the two partitions agree when each aggregate maps to one root and back
   std::vector<int> a2r(N, -1), r2a(N, -1);
   for (int i = 0; i < N; ++i) {
@@ -1851,8 +1881,11 @@ int amg_build(Context* c) {
   double h = 3.0 * 2.0 * c->cell.radius;
   // the smoothed hierarchy's aggregates follow the intact bonds
   std::vector<std::pair<int, int>> edges;
+  bool split = false;
   if (A->smoothed) {
-    if (int rc = intact_edges(c, A->bonds_h, &edges)) return rc;
+    if (int rc = split_by_bonds(c, &split)) return rc;
+    if (split)
+      if (int rc = intact_edges(c, A->bonds_h, &edges)) return rc;
     A->break_version = c->break_version;
   }
   while ((int64_t)N * D > kCoarsestMax) {
@@ -1891,7 +1924,7 @@ int amg_build(Context* c) {
     }
     if (A->smoothed) {
       if (A->lv.empty()) A->cell0 = agg;
-      split_aggregates(N, edges, agg, mptr, midx, Nc);
+      if (split) split_aggregates(N, edges, agg, mptr, midx, Nc);
       if (A->lv.empty()) A->agg0 = agg;
     }
     // coarse pattern and gather lists: per aggregate row, the (column, fine
@@ -2008,7 +2041,7 @@ int amg_build(Context* c) {
     pos.swap(cen);
     cptr.swap(ccptr);
     conns.swap(cconns);
-    if (A->smoothed) coarse_edges(agg, Nc, edges);
+    if (split) coarse_edges(agg, Nc, edges);
     N = Nc;
     h *= 3.5;
   }
