@@ -86,6 +86,8 @@ enum {
  *       counterpart; same stopping test, -4 with a message on a breakdown);
  *       4 smoothed-aggregation multigrid (as 2, with the prolongation
  *       smoothed by one damped-Jacobi step and Galerkin coarse operators;
+ *       aggregates split along broken bonds, in 3D once a bond is broken,
+ *       rebuilt at a lpmg_solve_step's first solve when that split changed;
  *       fewer iterations, more setup and memory; no reference counterpart)
  *   [1] rtol, [2] atol: stop when ||r||^2 <= rtol*||r0||^2 + atol
  *       (the reference's dpar[0], dpar[1], solver.h:312-313)
