@@ -1303,7 +1303,9 @@ struct Amg {
   int nc = 0;                  // unknowns of the coarsest level
   void* inv[2] = {nullptr, nullptr};
   void* mask[2] = {nullptr, nullptr};  // finest-level free-DOF mask per precision
-  int64_t version[2] = {-1, -1};
+  int64_t version[2] = {-1, -1};       // the assembly the values were built from
+  int64_t values_step[2] = {-1, -1};   // the load step they were built in (-1: none)
+  int64_t version16 = -1;              // the assembly the bfloat16 copy was rounded from
   uint16_t* fine16 = nullptr;  // FP32: the finest level's values as bfloat16, for the smoothers
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   std::vector<uintptr_t> graph_sig[2];  // pointers and shapes captured into graph[P]
@@ -2406,13 +2408,20 @@ int run_pcg_amg(Context* c, const SolveOpts& o, int* iters, double* rr_final) {
   amg_dinv0_kernel<D, T><<<(N + 255) / 256, 256, 0, c->stream>>>(c->dpos, val0, c->cons, c->damage_visual,
                                                                  c->floating, N, dinv0, (T*)A->mask[P]);
   if (A->version[P] != c->asm_version) {
-    if (int rc = amg_values<D, T>(c)) return rc;
-    if (P == 1) {
+    // inside a load step the hierarchy's values from the step's first solve
+    // serve its later damage passes: a preconditioner from a slightly older
+    // stiffness (the CG's operator and the finest smoother stay current)
+    if (!(c->in_step && A->values_step[P] == c->step_serial)) {
+      if (int rc = amg_values<D, T>(c)) return rc;
+      A->version[P] = c->asm_version;
+      A->values_step[P] = c->in_step ? c->step_serial : -1;
+    }
+    if (P == 1 && A->version16 != c->asm_version) {
       const int64_t nv = (int64_t)c->nnzb * bf16_stride(D);
       bf16_round_kernel<D><<<(unsigned)((nv + 255) / 256), 256, 0, c->stream>>>(c->Kval32, A->fine16, c->nnzb);
       LPMG_HIP(hipGetLastError());
+      A->version16 = c->asm_version;
     }
-    A->version[P] = c->asm_version;
   }
   int lpr = 64, u = 1;
   fine_shape(c, &lpr, &u);

@@ -825,6 +825,14 @@ int lpmg_solve_step(lpmg_handle h, int undamaged, int max_iter, double tol, cons
   bool new_damaged = false;
   if (so.predict > 0 && c->inc_valid)
     if (int rc = launch_predict(c, so.predict)) return rc;
+  struct StepScope {
+    Context* c;
+    explicit StepScope(Context* x) : c(x) {
+      c->in_step = true;
+      ++c->step_serial;
+    }
+    ~StepScope() { c->in_step = false; }
+  } scope(c);
   amg_step_begins(c);
   do {
     if (int rc = launch_assemble(c, so.precision)) return rc;

@@ -200,6 +200,9 @@ struct Context {
   // resets, field writes): the smoothed hierarchy's aggregates follow the
   // intact bonds
   int64_t break_version = 0;
+  // lpmg_solve_step in progress, and the count of load steps begun
+  bool in_step = false;
+  int64_t step_serial = 0;
   Amg* amg = nullptr;
 };
 
