@@ -1740,7 +1740,7 @@ int intact_edges(Context* c, std::vector<int>& bl, std::vector<std::pair<int, in
 }
 
 // Whether the aggregates should follow the bonds: in 2D always (a notch cut
-// into the lattice splits cells before any bond breaks; 1.3M bonds cost 15 ms
+// into the lattice splits cells before any bond breaks; 1.3M bonds cost ~10 ms
 // on the 205k CT request), in 3D once some bond is broken (the 1.0M-particle
 // polycrystal's 7M bonds cost 0.17 s per build and split no cell).
 int split_by_bonds(Context* c, bool* split) {
