@@ -86,7 +86,10 @@ SERVE_OPENING = 7.5e-6  # pin opening per load step, in units of W
 # floor), at 1e-4 1.6e-9.  With the multigrid CG, 1e-5 takes fewer Newton
 # iterations per damage pass than 1e-4 (90 against 117 on the 205,645-particle
 # request) and fewer CG iterations in all: the 20-request replay served 0.958
-# solves/s against 0.839 (3e-5: 0.920; 1e-6 over-solves).
+# solves/s against 0.839 (3e-5: 0.920; 1e-6 over-solves); with the aggregates
+# split along the broken bonds 1.353 against 1.251 (3e-5: 1.360, 3e-6: 1.301,
+# 1e-6: 1.259).  Slots and fusion limit: 8 and 10k at 1.606 solves/s against
+# 12 slots 1.512, 16 slots 1.476, a 20k limit 1.574, a 5k limit 1.608.
 SERVED_OPTS = dict(precond=-1, rtol=1e-5, atol=0.0, precision=1, predict=1.0)
 # Plain FP64 GPU path (the drop-in): FP64 stiffness, Jacobi-PCG stopped at
 # ||r||^2 <= 1e-12 ||r0||^2 (the reference's dpar[0], solver.h:312), the
